@@ -1,0 +1,257 @@
+"""Benchmark: env-steps/s at 1024 mixed-size environments, p50/p99 step
+latency, batched vs plain, on one MI355X.
+
+An env-step is one frame of one environment: S substeps x I iterations of the
+constraint projection, the BVH refit and a depth render.  A bench step is one
+frame of all environments: the tool move, the batched frame kernel (physics +
+refit) and the depth render.  Inputs are resident in HBM before timing.
+
+Prints one JSON line (rank 0, one GPU).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "ellipsoids", "cube", "large"])
+    ap.add_argument("--envs", type=int, default=None)
+    ap.add_argument("--block", type=int, default=1024)
+    ap.add_argument("--plain-frames", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    return ap.parse_args()
+
+
+def make_workload(wl, args):
+    if args.workload == "mixed":
+        return wl.mixed(args.envs or 1024, seed=args.seed)
+    if args.workload == "ellipsoids":
+        return wl.ellipsoids(args.envs or 64, seed=args.seed)
+    if args.workload == "large":
+        return wl.large(args.envs or 8192, seed=args.seed)
+    return wl.cube(args.seed)
+
+
+def algorithmic_bytes(w, model):
+    """Compulsory bytes of one frame kernel launch (all envs), per env:
+    S x [predict 40 V + I x (28 V + 12 E + T x (56 with Neo-Hookean, else 20))
+    + integrate 48 V] + 8 B per lambda-carrying constraint per extra iteration
+    + refit (12 V_render + 16 F + 40 N)."""
+    kw = w.model_kwargs
+    S, I = kw["simSubsteps"], kw["simConstraintsSteps"]
+    nh = kw.get("neoHookean", False)
+    has_vol = kw.get("volume") is not None
+    has_dist = kw.get("distance") is not None
+    total = 0
+    for e, k in enumerate(w.env_mesh):
+        m = w.meshes[k]
+        topo = model.topo[k]
+        V, E, T, F, N = m.num_vertices, m.num_edges, m.num_tets, m.num_tris, topo.num_nodes
+        t_bytes = 56 if nh else (20 if has_vol else 0)
+        proj = 28 * V + (12 * E if has_dist else 0) + t_bytes * T
+        lam = 0
+        if I > 1:
+            lam = 8 * ((E if kw.get("distance") == "xpbd" else 0) + (T if kw.get("volume") == "xpbd" else 0))
+        per_sub = 40 * V + I * proj + (I - 1) * lam + 48 * V
+        vr = m.surface_vertices().size
+        total += S * per_sub + 12 * vr + 16 * F + 40 * N
+    return total
+
+
+def cpu_baseline(pkg, orc, w, model, ren, seconds, seed):
+    """The oracle (1 core, fp32) on a random sample of the same envs: one frame
+    (physics + refit + depth render) per sampled env until `seconds` elapse."""
+    rng = np.random.default_rng(seed + 17)
+    order = rng.permutation(w.num_envs)
+    kw = w.model_kwargs
+    cfg = orc.StepConfig(substeps=kw["simSubsteps"], iterations=kw["simConstraintsSteps"],
+                         fps=kw["simFrameRate"], gravity=kw["environmentGravity"],
+                         damping=kw["environmentVelocityDampening"],
+                         dist_mode={None: 0, "pbd": 1, "xpbd": 2}[kw.get("distance")],
+                         vol_mode={None: 0, "pbd": 1, "xpbd": 2}[kw.get("volume")],
+                         nh_on=int(kw.get("neoHookean", False)),
+                         nh_vol_c=kw.get("globalVolumeCompliance", 1.0),
+                         nh_dev_c=kw.get("globalDeviatoricCompliance", 1.0),
+                         dist_compliance=kw.get("distanceCompliance", 0.0),
+                         vol_compliance=kw.get("volumeComplianceXPBD", 0.0), drag_ks=kw.get("globalKsDrag", 1.0))
+    dirs = ren.dirs.cpu().numpy()
+    origin = np.asarray(ren.camera.origin[:], np.float32)
+    done, tets = 0, 0
+    t0 = time.perf_counter()
+    for e in order:
+        k = w.env_mesh[e]
+        m = w.meshes[k]
+        sim = orc.OracleSim([m], [0], np.float32, tips=w.tips[e:e + 1] if w.tips is not None else None,
+                            initial_velocity=w.initial_velocity)
+        if w.grasp is not None and w.grasp[e] is not None:
+            sim.drag[w.grasp[e]] = 1.0
+        sim.step(cfg, 1)
+        t = model.topo[k]
+        bvh = {"left": t.left, "first": t.first, "count": t.count, "level": t.level, "tri_map": t.tri_map}
+        mn, mx = orc.refit(bvh, m.tris, sim.x)
+        orc.render_depth(ren.height, ren.width, origin, dirs, bvh, mn, mx, m.tris, sim.x)
+        done += 1
+        tets += m.num_tets
+        if time.perf_counter() - t0 >= seconds and done >= 4:
+            break
+    el = time.perf_counter() - t0
+    mean_t_sample = tets / done
+    mean_t_pop = sum(w.meshes[k].num_tets for k in w.env_mesh) / w.num_envs
+    # scale the sampled per-env cost to the population's mean env size
+    rate = done / el * (mean_t_sample / mean_t_pop)
+    return {"value": rate, "unit": "env-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle fp32 serial C restatement, {done} randomly sampled envs x 1 frame "
+                      f"(physics + refit + {ren.height}x{ren.width} depth) in {el:.1f} s, scaled by mean tets "
+                      f"{mean_t_sample:.0f}/{mean_t_pop:.0f}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is by environment and is "
+                                   "described in DESIGN.md, not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    wl = importlib.import_module("ff-srl_amd.workloads")
+    t_setup = time.perf_counter()
+    w = make_workload(wl, args)                       # meshes generated before the GPU runtime starts
+    import torch
+    pkg = importlib.import_module("ff-srl_amd")
+    dev = torch.device("cuda:0")
+    model = wl.build_model(pkg, w, dev)
+    ren = pkg.RaycastRendererDO(dev, model, cameraPos=w.camera_pos, resolution=w.resolution)
+    ren.lookAt(w.look_at)
+    integ = pkg.SimIntegratorDO(dev, block=args.block)
+    lib = pkg.lib
+    n = w.num_envs
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed)
+    total_frames = args.warmup + args.steps
+    walk = torch.randn((total_frames, n, 3), generator=gen, device=dev) * w.walk_sigma
+    depth_out = ren.depth
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+
+    def step(f, sh):
+        lib.ffsrl_move_tools(model.batch, walk[f].data_ptr(), None, None, sh)
+        integ.stepModel(model, refit=True)       # on torch's current stream, which is sh
+        lib.ffsrl_render(model.batch, ren.camera, 0, n, depth_out.data_ptr(), None, None, None, None, None, None, sh)
+
+    pkg.SimBVH(dev, model)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+    for f in range(args.warmup):
+        step(f, s)
+    torch.cuda.synchronize()
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        f = args.warmup + i
+        e0, e1, e2, e3 = ev[i]
+        e0.record(stream)
+        lib.ffsrl_move_tools(model.batch, walk[f].data_ptr(), None, None, s)
+        e1.record(stream)
+        integ.stepModel(model, refit=True)
+        e2.record(stream)
+        lib.ffsrl_render(model.batch, ren.camera, 0, n, depth_out.data_ptr(), None, None, None, None, None, None, s)
+        e3.record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    step_ms = np.array([ev[i][0].elapsed_time(ev[i][3]) for i in range(args.steps)])
+    frame_ms = np.array([ev[i][1].elapsed_time(ev[i][2]) for i in range(args.steps)])
+    render_ms = np.array([ev[i][2].elapsed_time(ev[i][3]) for i in range(args.steps)])
+    value = n * args.steps / elapsed
+    finite = bool(torch.isfinite(model.x).all().item())
+
+    out = {
+        "metric": "env-steps/s @1024 mixed-size envs + p50/p99 step latency, batched vs plain",
+        "value": value, "unit": "env-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded meshes, seeded tool random walk)",
+        "config": {"workload": w.name, "envs": n, **{k: int(v) for k, v in w.totals().items()},
+                   "substeps": w.model_kwargs["simSubsteps"], "iterations": w.model_kwargs["simConstraintsSteps"],
+                   "render": f"{w.resolution}x{w.resolution} depth (2x2 supersampled)", "block": args.block,
+                   "form": "batched (1 frame kernel + 1 render kernel per step)"},
+        "p50_step_ms": float(np.percentile(step_ms, 50)), "p99_step_ms": float(np.percentile(step_ms, 99)),
+        "frame_kernel_ms": float(frame_ms.mean()), "render_kernel_ms": float(render_ms.mean()),
+        "rays_per_s": float(4 * w.resolution ** 2 * n / (render_ms.mean() * 1e-3)),
+        "finite": finite, "setup_s": setup_s,
+    }
+    alg = algorithmic_bytes(w, model)
+    achieved = alg / (frame_ms.mean() * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_frame (physics + refit)",
+                       "algorithmic_bytes_per_launch": alg}
+
+    if not args.no_graph:
+        g = C.c_void_p()
+        cs = torch.cuda.Stream()
+        cs.wait_stream(stream)
+        with torch.cuda.stream(cs):
+            lib.ffsrl_graph_begin(cs.cuda_stream)
+            step(total_frames - 1, cs.cuda_stream)
+            lib.ffsrl_graph_end(cs.cuda_stream, C.byref(g))
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                lib.ffsrl_graph_launch(g, cs.cuda_stream)
+            torch.cuda.synchronize()
+            out["graph_env_steps_per_s"] = n * args.steps / (time.perf_counter() - t1)
+        lib.ffsrl_graph_destroy(g)
+
+    if not args.no_plain and args.plain_frames > 0:
+        model.enable_plain()
+
+        def plain_step(f):
+            lib.ffsrl_move_tools(model.batch, walk[f % total_frames].data_ptr(), None, None, s)
+            integ.stepModelPlain(model)
+            lib.ffsrl_bvh_refit_plain(model.batch, 0, n, s)
+            for e in range(n):
+                lib.ffsrl_render(model.batch, ren.camera, e, e + 1, depth_out[e].data_ptr(), None, None, None, None,
+                                 None, None, s)
+
+        plain_step(0)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        pms = []
+        for f in range(args.plain_frames):
+            a = time.perf_counter()
+            plain_step(f)
+            torch.cuda.synchronize()
+            pms.append((time.perf_counter() - a) * 1e3)
+        plain_el = time.perf_counter() - tp
+        out["plain_env_steps_per_s"] = n * args.plain_frames / plain_el
+        out["plain_p50_step_ms"] = float(np.percentile(pms, 50))
+        out["batched_over_plain"] = value / out["plain_env_steps_per_s"]
+
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        out["cpu_baseline"] = cpu_baseline(pkg, orc, w, model, ren, args.cpu_seconds, args.seed)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

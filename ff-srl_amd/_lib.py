@@ -1,0 +1,142 @@
+"""ctypes binding of the C ABI declared in include/ffsrl.h.
+
+The library is the only compute path: if ``_build/libffsrl.so`` is missing
+this module raises on first use; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libffsrl.so")
+
+OFF, PBD, XPBD = 0, 1, 2
+E_ARG, E_LIMIT, E_STATE = -1, -2, -3
+
+
+class EnvDesc(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "vbase", "nv", "ebase", "ne", "tbase", "nt", "fbase", "nf", "nodebase", "nn",
+        "boundbase", "lvlbase", "nlvl", "lam_ebase", "lam_tbase", "flags")]
+
+
+ENV_DESC_FIELDS = [f[0] for f in EnvDesc._fields_]
+assert C.sizeof(EnvDesc) == 64
+
+_vp = C.c_void_p
+
+
+class Batch(C.Structure):
+    _fields_ = [
+        ("n_env", C.c_int32), ("max_nv", C.c_int32), ("n_tet_total", C.c_int64),
+        ("envs", _vp),
+        ("edges", _vp), ("edge_rest", _vp), ("tets", _vp), ("tet_rest_volume", _vp),
+        ("tet_inv_rest", _vp), ("tris", _vp), ("node_left", _vp), ("node_first", _vp),
+        ("node_count", _vp), ("level_nodes", _vp), ("level_off", _vp),
+        ("x", _vp), ("p", _vp), ("v", _vp), ("drag", _vp), ("lam_e", _vp), ("lam_t", _vp),
+        ("bvh_lo", _vp), ("bvh_hi", _vp), ("tips", _vp),
+        ("dp", _vp), ("cnt", _vp),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("substeps", C.c_int32), ("iterations", C.c_int32), ("dt", C.c_float),
+        ("gravity", C.c_float * 3), ("damping", C.c_float), ("ground", C.c_float),
+        ("dist_mode", C.c_int32), ("dist_k", C.c_float),
+        ("vol_mode", C.c_int32), ("vol_k", C.c_float),
+        ("nh_on", C.c_int32), ("nh_vol_c", C.c_float), ("nh_dev_c", C.c_float),
+        ("drag_ks", C.c_float), ("lambda_legacy", C.c_int32),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("origin", C.c_float * 3), ("dirs", _vp), ("height", C.c_int32), ("width", C.c_int32),
+        ("mode", C.c_int32), ("light_dir", C.c_float * 3), ("light_intensity", C.c_float),
+    ]
+
+
+_i32, _f32, _i64 = C.c_int32, C.c_float, C.c_int64
+_P = C.POINTER
+
+# name -> (argtypes).  Every function returns int (0 = ok).
+SIGNATURES = {
+    "ffsrl_frame": [_P(Batch), _P(Params), _vp, _i32, _i32, _vp],
+    "ffsrl_bvh_refit": [_P(Batch), _i32, _i32, _vp],
+    "ffsrl_render": [_P(Batch), _P(Camera), _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "ffsrl_vertex_normals": [_P(Batch), _i32, _i32, _vp, _vp],
+    "ffsrl_move_tools": [_P(Batch), _vp, _vp, _vp, _vp],
+    "ffsrl_grasp_select": [_P(Batch), _f32, _vp, _vp, _vp],
+    "ffsrl_predict": [_P(Batch), _P(Params), _i32, _i32, _vp],
+    "ffsrl_project_volume": [_P(Batch), _P(Params), _i32, _i32, _vp],
+    "ffsrl_project_distance": [_P(Batch), _P(Params), _i32, _i32, _vp],
+    "ffsrl_project_neohookean": [_P(Batch), _P(Params), _i32, _i32, _vp],
+    "ffsrl_apply": [_P(Batch), _i32, _i32, _vp],
+    "ffsrl_grasp": [_P(Batch), _P(Params), _i32, _i32, _vp],
+    "ffsrl_bounds_integrate": [_P(Batch), _P(Params), _i32, _i32, _vp],
+    "ffsrl_zero_lambdas": [_P(Batch), _i32, _i32, _vp],
+    "ffsrl_bvh_refit_plain": [_P(Batch), _i32, _i32, _vp],
+    "ffsrl_remap": [_vp, _vp, _vp, _i64, _vp],
+    "ffsrl_bvh_build": [_vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _P(_i32), _P(_i32)],
+    "ffsrl_graph_begin": [_vp],
+    "ffsrl_graph_end": [_vp, _P(_vp)],
+    "ffsrl_graph_launch": [_vp, _vp],
+    "ffsrl_graph_destroy": [_vp],
+}
+# Exported functions that do not return an error code.
+PLAIN_SIGNATURES = {
+    "ffsrl_abi_version": ([], C.c_int),
+    "ffsrl_last_error": ([], C.c_char_p),
+    "ffsrl_frame_max_vertices": ([], C.c_int),
+}
+
+
+class FFSRLError(RuntimeError):
+    pass
+
+
+class _Lib:
+    def __init__(self):
+        self._h = None
+
+    def load(self):
+        if self._h is None:
+            if not os.path.exists(LIB_PATH):
+                raise FFSRLError(
+                    f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+            h = C.CDLL(LIB_PATH)
+            for name, args in SIGNATURES.items():
+                fn = getattr(h, name)
+                fn.argtypes = args
+                fn.restype = C.c_int
+            for name, (args, res) in PLAIN_SIGNATURES.items():
+                fn = getattr(h, name)
+                fn.argtypes = args
+                fn.restype = res
+            self._h = h
+        return self._h
+
+    def __getattr__(self, name):
+        h = self.load()
+        fn = getattr(h, name)
+        if name in PLAIN_SIGNATURES:
+            return fn
+
+        def call(*args):
+            rc = fn(*args)
+            if rc != 0:
+                msg = h.ffsrl_last_error().decode(errors="replace")
+                raise FFSRLError(f"{name} failed ({rc}): {msg}")
+            return rc
+
+        call.__name__ = name
+        return call
+
+
+lib = _Lib()
+
+
+def exported_symbols():
+    return list(SIGNATURES) + list(PLAIN_SIGNATURES)

@@ -1,0 +1,204 @@
+// Per-constraint projection math, shared by the batched (LDS) and plain
+// (global-atomic) kernels.  Each function follows the reference kernel's
+// expression order; positions arrive as float4 (x, y, z, inverse mass).
+#pragma once
+
+#include "common.hpp"
+
+namespace ffsrl {
+
+// distanceConstraints, PBD form (integrator.py:65-109), all edges active.
+// On success returns 1 and e = edgeDP; the caller applies dP[a] -= e*wa and
+// dP[b] += e*wb and counts both ends.
+__device__ __forceinline__ int distance_pbd(float4 A, float4 B, float rest, float kS, f3* e) {
+  if (rest < kFeps) return 0;
+  f3 dir = xyz(B) - xyz(A);
+  float len = length(dir);
+  if (len < kFeps) return 0;
+  float w = A.w + B.w;
+  if (w < kFeps) return 0;
+  float C = len - rest;
+  *e = ((dir / len) * (-C)) * kS / w;
+  return 1;
+}
+
+// distanceConstraintsXPBD (integrator.py:191-240); alpha = compliance/dt^2.
+// *lam is read and updated in place.
+__device__ __forceinline__ int distance_xpbd(float4 A, float4 B, float rest, float alpha, float* lam, f3* e) {
+  if (rest < kFeps) return 0;
+  f3 dir = xyz(B) - xyz(A);
+  float len = length(dir);
+  if (len < kFeps) return 0;
+  float w = A.w + B.w;
+  if (w < kFeps) return 0;
+  float C = len - rest;
+  float dl = -(C + alpha * (*lam)) / (w + alpha);
+  *lam = *lam + dl;
+  *e = normalize(dir) * dl;
+  return 1;
+}
+
+struct VolTerms {
+  f3 q[4];
+  float six_vol, w;
+};
+
+// q vectors (q1 = p2 x p3, q2 = p3 x p1, q3 = p1 x p2, q0 = -q1-q2-q3), the
+// signed 6V and w = sum m_i |q_i|^2 of the volume kernels (integrator.py:474-490).
+__device__ __forceinline__ VolTerms vol_terms(const float4 P[4]) {
+  VolTerms t;
+  f3 a = xyz(P[0]);
+  f3 p1 = xyz(P[1]) - a, p2 = xyz(P[2]) - a, p3 = xyz(P[3]) - a;
+  t.q[2] = cross(p3, p1);
+  t.q[1] = cross(p2, p3);
+  t.q[3] = cross(p1, p2);
+  t.q[0] = ((-t.q[1]) - t.q[2]) - t.q[3];
+  t.six_vol = dot(cross(p1, p2), p3);
+  t.w = P[0].w * dot(t.q[0], t.q[0]) + P[1].w * dot(t.q[1], t.q[1]) + P[2].w * dot(t.q[2], t.q[2]) +
+        P[3].w * dot(t.q[3], t.q[3]);
+  return t;
+}
+
+// volumeConstraints, PBD form (integrator.py:444-504): |V|, C = V0 - |V|.
+__device__ __forceinline__ int volume_pbd(const float4 P[4], float rest, float kS, f3 d[4]) {
+  VolTerms t = vol_terms(P);
+  float volume = fabsf(t.six_vol) / 6.0f;
+  if (fabsf(t.w) < kFeps) return 0;
+  float C = rest - volume;
+  float dl = -kS * C / t.w;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) d[i] = (t.q[i] * dl) * P[i].w;
+  return 1;
+}
+
+// volumeConstraintsXPBD (integrator.py:507-576): signed V, C = V - V0.
+__device__ __forceinline__ int volume_xpbd(const float4 P[4], float rest, float alpha, float* lam, f3 d[4]) {
+  VolTerms t = vol_terms(P);
+  float volume = t.six_vol / 6.0f;
+  if (fabsf(t.w) < kFeps) return 0;
+  float C = volume - rest;
+  float dl = -(C + alpha * (*lam)) / (t.w + alpha);
+  *lam = *lam + dl;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) d[i] = (t.q[i] * dl) * P[i].w;
+  return 1;
+}
+
+// Constants of neoHookeanConstraints that do not depend on the tet.
+struct NHConst {
+  float gamma;     // invD / invV = (1/devC) / (1/volC)
+  float alpha_h;   // volC / dT / dT  (divided by the rest volume per tet)
+  float alpha_d;   // devC / dT / dT
+};
+
+__host__ __device__ inline NHConst nh_const(float volC, float devC, float dT) {
+  NHConst c;
+  float invV = 1.0f / volC, invD = 1.0f / devC;
+  c.gamma = invD / invV;
+  c.alpha_h = volC / dT / dT;
+  c.alpha_d = devC / dT / dT;
+  return c;
+}
+
+// m = row-major 3x3; F = Ds * M with Ds = columns (p1, p2, p3); k summed 0,1,2.
+// neoHookeanConstraints (integrator.py:578-698): hydrostatic det F - 1 - gamma
+// with gradient rows of transpose(mat33(F1xF2, F2xF0, F0xF1) * M^T), then
+// deviatoric |F|_F with gradient rows of transpose(F * M^T / |F|); both
+// deltas from the same positions; lambda is not carried.
+__device__ __forceinline__ int neohookean(const float4 P[4], float rest, const float M[9], const NHConst& k,
+                                          f3 d[4]) {
+  f3 a = xyz(P[0]);
+  f3 p1 = xyz(P[1]) - a, p2 = xyz(P[2]) - a, p3 = xyz(P[3]) - a;
+  const float mA = P[0].w, mB = P[1].w, mC = P[2].w, mD = P[3].w;
+  // Ds columns p1,p2,p3 -> Ds[i][j] = p_{j+1}[i]
+  const float Ds[9] = {p1.x, p2.x, p3.x, p1.y, p2.y, p3.y, p1.z, p2.z, p3.z};
+  float F[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float s = 0.0f;
+      s = s + Ds[3 * i + 0] * M[0 * 3 + j];
+      s = s + Ds[3 * i + 1] * M[1 * 3 + j];
+      s = s + Ds[3 * i + 2] * M[2 * 3 + j];
+      F[3 * i + j] = s;
+    }
+  f3 F0 = mk(F[0], F[1], F[2]), F1 = mk(F[3], F[4], F[5]), F2 = mk(F[6], F[7], F[8]);
+  f3 c1 = cross(F1, F2), c2 = cross(F2, F0), c3 = cross(F0, F1);
+  // dF = columns (c1, c2, c3); g = dF * M^T ; rows of g^T = columns of g.
+  const float dF[9] = {c1.x, c2.x, c3.x, c1.y, c2.y, c3.y, c1.z, c2.z, c3.z};
+  float g[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float s = 0.0f;
+      s = s + dF[3 * i + 0] * M[j * 3 + 0];
+      s = s + dF[3 * i + 1] * M[j * 3 + 1];
+      s = s + dF[3 * i + 2] * M[j * 3 + 2];
+      g[3 * i + j] = s;
+    }
+  f3 g1 = mk(g[0], g[3], g[6]), g2 = mk(g[1], g[4], g[7]), g3 = mk(g[2], g[5], g[8]);
+  f3 g0 = ((-g1) - g2) - g3;
+  float volume = dot(F0, cross(F1, F2));
+  float C = volume - 1.0f - k.gamma;
+  float l, w = 0.0f;
+  l = length(g0); w = w + l * l * mA;
+  l = length(g1); w = w + l * l * mB;
+  l = length(g2); w = w + l * l * mC;
+  l = length(g3); w = w + l * l * mD;
+  if (w < kFeps) return 0;
+  float alpha = k.alpha_h / rest;
+  float dl = -C / (w + alpha);
+  f3 dA = (g0 * dl) * mA, dB = (g1 * dl) * mB, dC = (g2 * dl) * mC, dD = (g3 * dl) * mD;
+  float l0 = length(F0), l1 = length(F1), l2 = length(F2);
+  float rs = sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
+  float rsi = 1.0f / rs;
+  C = rs;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float s = 0.0f;
+      s = s + F[3 * i + 0] * M[j * 3 + 0];
+      s = s + F[3 * i + 1] * M[j * 3 + 1];
+      s = s + F[3 * i + 2] * M[j * 3 + 2];
+      g[3 * i + j] = s * rsi;
+    }
+  g1 = mk(g[0], g[3], g[6]);
+  g2 = mk(g[1], g[4], g[7]);
+  g3 = mk(g[2], g[5], g[8]);
+  g0 = ((-g1) - g2) - g3;
+  w = 0.0f;
+  l = length(g0); w = w + l * l * mA;
+  l = length(g1); w = w + l * l * mB;
+  l = length(g2); w = w + l * l * mC;
+  l = length(g3); w = w + l * l * mD;
+  if (w < kFeps) return 0;
+  alpha = k.alpha_d / rest;
+  dl = -C / (w + alpha);
+  d[0] = dA + (g0 * dl) * mA;
+  d[1] = dB + (g1 * dl) * mB;
+  d[2] = dC + (g2 * dl) * mC;
+  d[3] = dD + (g3 * dl) * mD;
+  return 1;
+}
+
+// gravity (integrator.py:40-63): v' = v + g*w*dt*(1-damp) (v' = 0 when w < eps);
+// returns p = x + v'*dt.
+__device__ __forceinline__ f3 predict(f3 x, f3 v, float w, f3 g, float dt, float damp1) {
+  v = v + ((g * w) * dt) * damp1;
+  if (w < kFeps) v = mk(0.0f, 0.0f, 0.0f);
+  return x + v * dt;
+}
+
+// dragConstraintsDO (integrator.py:412-442): p -= |p-tip| * n * drag * ks / 2.
+__device__ __forceinline__ f3 grasp_pull(f3 p, f3 tip, float dr, float ks) {
+  f3 dir = p - tip;
+  float len = length(dir);
+  if (len < kFeps) return p;
+  f3 e = (((normalize(dir) * len) * dr) * ks) / 2.0f;
+  return p - e;
+}
+
+}  // namespace ffsrl

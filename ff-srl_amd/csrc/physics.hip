@@ -1,0 +1,704 @@
+// Physics kernels: the batched frame kernel and the plain per-kernel form.
+//
+// Batched form (ffsrl_frame): one workgroup owns one environment for a whole
+// frame.  The Jacobi deltas and constraint counts of the env's vertices are
+// reduced in LDS with ds_add (12 B of dP + a 16-bit counter per vertex), so
+// no global atomics are issued; phases (predict, project, apply) are separated
+// by workgroup barriers instead of kernel boundaries, and the integrate of
+// substep s is fused with the predict of substep s+1.  The BVH refit of the
+// env runs at the end of the same workgroup.  Environments of any mix of sizes
+// run in one launch; dispatching them largest-first balances the CUs.
+//
+// Plain form (ffsrl_predict ... ffsrl_bounds_integrate): the reference's
+// launch model (integrator.py:1644-1741), one launch per kernel per env, dP
+// and counts accumulated with global float / int atomics.
+#include <algorithm>
+#include <cstdarg>
+#include <cstring>
+
+#include "constraints.hpp"
+#include "refit.hpp"
+
+namespace ffsrl {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  return fail(static_cast<int>(e), "%s: %s", what, hipGetErrorString(e));
+}
+
+// LDS budget of one workgroup on gfx950.
+constexpr int kLdsBytes = 160 * 1024;
+constexpr int lds_bytes_for(int nv) {
+  int cap = (nv + 1) & ~1;
+  return cap * 12 + (cap / 2) * 4;
+}
+constexpr int kMaxFrameVertices = (kLdsBytes / 14) & ~1;
+
+struct FrameCtx {
+  float* dpx;
+  float* dpy;
+  float* dpz;
+  uint32_t* cnt;  // two 16-bit counters per word
+};
+
+__device__ __forceinline__ void lds_add(const FrameCtx& c, int i, f3 d) {
+  atomicAdd(&c.dpx[i], d.x);
+  atomicAdd(&c.dpy[i], d.y);
+  atomicAdd(&c.dpz[i], d.z);
+}
+__device__ __forceinline__ void lds_sub(const FrameCtx& c, int i, f3 d) {
+  atomicAdd(&c.dpx[i], -d.x);
+  atomicAdd(&c.dpy[i], -d.y);
+  atomicAdd(&c.dpz[i], -d.z);
+}
+__device__ __forceinline__ void lds_count(const FrameCtx& c, int i, uint32_t n) {
+  atomicAdd(&c.cnt[i >> 1], n << (16 * (i & 1)));
+}
+
+__device__ __forceinline__ float4 with_w(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// Predict + (optionally) everything of the apply phase for one vertex.
+struct VertexStep {
+  f3 g;
+  float dt, damp1, inv_dt, ground, ks;
+};
+
+template <int BLOCK, bool CARRY>
+__global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm, const int32_t* order,
+                                                 int refit) {
+  extern __shared__ float lds[];
+  const int env = order ? order[blockIdx.x] : (int)blockIdx.x;
+  const ffsrl_env_desc e = b.envs[env];
+  const int nv = e.nv;
+  const int cap = (nv + 1) & ~1;
+  FrameCtx c{lds, lds + cap, lds + 2 * cap, reinterpret_cast<uint32_t*>(lds + 3 * cap)};
+  const int tid = threadIdx.x;
+  const int64_t vb = e.vbase;
+  float* X = b.x;
+  float* P = b.p;
+  float* Vv = b.v;
+
+  VertexStep vs;
+  vs.g = mk(prm.gravity[0], prm.gravity[1], prm.gravity[2]);
+  vs.dt = prm.dt;
+  vs.damp1 = 1.0f - prm.damping;
+  vs.inv_dt = 1.0f / prm.dt;
+  vs.ground = prm.ground;
+  vs.ks = prm.drag_ks;
+  const NHConst nhk = nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt);
+  const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
+  const int64_t T = b.n_tet_total;
+
+  for (int i = tid; i < cap; i += BLOCK) {
+    c.dpx[i] = 0.0f;
+    c.dpy[i] = 0.0f;
+    c.dpz[i] = 0.0f;
+  }
+  for (int i = tid; i < cap / 2; i += BLOCK) c.cnt[i] = 0u;
+  // predict of substep 0
+  for (int i = tid; i < nv; i += BLOCK) {
+    float4 x = ld4(X, vb + i), v = ld4(Vv, vb + i);
+    st4(P, vb + i, with_w(predict(xyz(x), xyz(v), x.w, vs.g, vs.dt, vs.damp1), x.w));
+  }
+  __syncthreads();
+
+  const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
+  for (int s = 0; s < prm.substeps; ++s) {
+    if (CARRY && reset_lambda) {
+      // Each thread zeroes the lambdas of exactly the constraints it projects.
+      if (prm.vol_mode == FFSRL_XPBD)
+        for (int t = tid; t < e.nt; t += BLOCK) b.lam_t[e.lam_tbase + t] = 0.0f;
+      if (prm.dist_mode == FFSRL_XPBD)
+        for (int k = tid; k < e.ne; k += BLOCK) b.lam_e[e.lam_ebase + k] = 0.0f;
+    }
+    for (int it = 0; it < prm.iterations; ++it) {
+      // ---- tets: volume (PBD / XPBD) and Neo-Hookean ----
+      if (prm.vol_mode != FFSRL_OFF || prm.nh_on) {
+        for (int t = tid; t < e.nt; t += BLOCK) {
+          const int64_t ti = e.tbase + t;
+          const int4 q = ldi4(b.tets, ti);
+          float4 Pq[4] = {ld4(P, vb + q.x), ld4(P, vb + q.y), ld4(P, vb + q.z), ld4(P, vb + q.w)};
+          const float rest = b.tet_rest_volume[ti];
+          f3 d[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
+          uint32_t n = 0;
+          if (prm.vol_mode == FFSRL_PBD) {
+            n += volume_pbd(Pq, rest, prm.vol_k, d);
+          } else if (prm.vol_mode == FFSRL_XPBD) {
+            float lam = CARRY ? b.lam_t[e.lam_tbase + t] : 0.0f;
+            n += volume_xpbd(Pq, rest, prm.vol_k, &lam, d);
+            if (CARRY) b.lam_t[e.lam_tbase + t] = lam;
+          }
+          if (prm.nh_on) {
+            float M[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) M[k] = b.tet_inv_rest[k * T + ti];
+            f3 dn[4];
+            if (neohookean(Pq, rest, M, nhk, dn)) {
+              n += 1;
+#pragma unroll
+              for (int k = 0; k < 4; ++k) d[k] = d[k] + dn[k];
+            }
+          }
+          if (n) {
+            lds_add(c, q.x, d[0]);
+            lds_add(c, q.y, d[1]);
+            lds_add(c, q.z, d[2]);
+            lds_add(c, q.w, d[3]);
+            lds_count(c, q.x, n);
+            lds_count(c, q.y, n);
+            lds_count(c, q.z, n);
+            lds_count(c, q.w, n);
+          }
+        }
+      }
+      // ---- edges: distance (PBD / XPBD) ----
+      if (prm.dist_mode != FFSRL_OFF) {
+        for (int k = tid; k < e.ne; k += BLOCK) {
+          const int64_t ei = e.ebase + k;
+          const int2 q = ldi2(b.edges, ei);
+          const float4 A = ld4(P, vb + q.x), B = ld4(P, vb + q.y);
+          const float rest = b.edge_rest[ei];
+          f3 d;
+          int ok;
+          if (prm.dist_mode == FFSRL_PBD) {
+            ok = distance_pbd(A, B, rest, prm.dist_k, &d);
+          } else {
+            float lam = CARRY ? b.lam_e[e.lam_ebase + k] : 0.0f;
+            ok = distance_xpbd(A, B, rest, prm.dist_k, &lam, &d);
+            if (CARRY && ok) b.lam_e[e.lam_ebase + k] = lam;
+          }
+          if (ok) {
+            lds_sub(c, q.x, d * A.w);
+            lds_add(c, q.y, d * B.w);
+            lds_count(c, q.x, 1u);
+            lds_count(c, q.y, 1u);
+          }
+        }
+      }
+      __syncthreads();
+      // ---- apply (+ grasp; + ground, integrate and next predict) ----
+      const bool last_it = (it == prm.iterations - 1);
+      const bool next_sub = (s + 1 < prm.substeps);
+      for (int j = tid; j < cap / 2; j += BLOCK) {
+        const uint32_t word = c.cnt[j];
+        c.cnt[j] = 0u;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int i = 2 * j + h;
+          if (i >= nv) break;
+          const uint32_t n = (word >> (16 * h)) & 0xffffu;
+          float4 p4 = ld4(P, vb + i);
+          f3 p = xyz(p4);
+          if (n > 0) {
+            const float N = (float)n;
+            p = p + mk(c.dpx[i] / N, c.dpy[i] / N, c.dpz[i] / N);
+          }
+          c.dpx[i] = 0.0f;
+          c.dpy[i] = 0.0f;
+          c.dpz[i] = 0.0f;
+          const float dr = b.drag[vb + i];
+          if (dr != 0.0f) p = grasp_pull(p, tip, dr, vs.ks);
+          if (!last_it) {
+            st4(P, vb + i, with_w(p, p4.w));
+            continue;
+          }
+          if (p.y < vs.ground) p.y = vs.ground;
+          const float4 x4 = ld4(X, vb + i);
+          const f3 v = (p - xyz(x4)) * vs.inv_dt;
+          st4(X, vb + i, with_w(p, x4.w));
+          st4(Vv, vb + i, with_w(v, 0.0f));
+          if (next_sub) p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
+          st4(P, vb + i, with_w(p, p4.w));
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (refit) refit_env<BLOCK>(b, e);
+}
+
+template <int BLOCK>
+static int launch_frame(const ffsrl_batch& b, const ffsrl_params& prm, const int32_t* order, int refit,
+                        bool carry, int lds, hipStream_t st) {
+  auto kern = carry ? k_frame<BLOCK, true> : k_frame<BLOCK, false>;
+  FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  hipLaunchKernelGGL(kern, dim3(b.n_env), dim3(BLOCK), lds, st, b, prm, order, refit);
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+static int check_batch(const ffsrl_batch* b, const ffsrl_params* prm) {
+  if (!b || !b->envs || !b->x || !b->p || !b->v || !b->drag || !b->tips)
+    return fail(FFSRL_E_ARG, "batch has a NULL state or env array");
+  if (b->n_env < 0) return fail(FFSRL_E_ARG, "n_env < 0");
+  if (prm) {
+    if (prm->substeps < 1 || prm->iterations < 1) return fail(FFSRL_E_ARG, "substeps and iterations must be >= 1");
+    if (!(prm->dt > 0.0f)) return fail(FFSRL_E_ARG, "dt must be > 0");
+    if (prm->dist_mode < 0 || prm->dist_mode > 2 || prm->vol_mode < 0 || prm->vol_mode > 2)
+      return fail(FFSRL_E_ARG, "constraint mode out of range");
+    if ((prm->vol_mode || prm->nh_on) && (!b->tets || !b->tet_rest_volume))
+      return fail(FFSRL_E_ARG, "tet constraints enabled without tet arrays");
+    if (prm->nh_on && !b->tet_inv_rest) return fail(FFSRL_E_ARG, "Neo-Hookean enabled without inverse rest matrices");
+    if (prm->dist_mode && (!b->edges || !b->edge_rest)) return fail(FFSRL_E_ARG, "distance enabled without edge arrays");
+  }
+  return 0;
+}
+
+static bool carries_lambda(const ffsrl_params& prm) {
+  const bool xpbd = prm.dist_mode == FFSRL_XPBD || prm.vol_mode == FFSRL_XPBD;
+  return xpbd && (prm.iterations > 1 || prm.lambda_legacy);
+}
+
+// ---------------------------------------------------------------- plain form
+
+__global__ void k_predict(ffsrl_batch b, ffsrl_params prm, int64_t vb, int nv) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  float4 x = ld4(b.x, vb + i), v = ld4(b.v, vb + i);
+  f3 g = mk(prm.gravity[0], prm.gravity[1], prm.gravity[2]);
+  st4(b.p, vb + i, with_w(predict(xyz(x), xyz(v), x.w, g, prm.dt, 1.0f - prm.damping), x.w));
+}
+
+__device__ __forceinline__ void g_add(const ffsrl_batch& b, int64_t V, int64_t i, f3 d) {
+  unsafeAtomicAdd(&b.dp[i], d.x);
+  unsafeAtomicAdd(&b.dp[V + i], d.y);
+  unsafeAtomicAdd(&b.dp[2 * V + i], d.z);
+}
+
+__global__ void k_volume(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, int64_t V) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= e.nt) return;
+  const int64_t ti = e.tbase + t, vb = e.vbase;
+  const int4 q = ldi4(b.tets, ti);
+  float4 Pq[4] = {ld4(b.p, vb + q.x), ld4(b.p, vb + q.y), ld4(b.p, vb + q.z), ld4(b.p, vb + q.w)};
+  f3 d[4];
+  int ok;
+  if (prm.vol_mode == FFSRL_PBD) {
+    ok = volume_pbd(Pq, b.tet_rest_volume[ti], prm.vol_k, d);
+  } else {
+    float lam = b.lam_t ? b.lam_t[e.lam_tbase + t] : 0.0f;
+    ok = volume_xpbd(Pq, b.tet_rest_volume[ti], prm.vol_k, &lam, d);
+    if (b.lam_t && ok) b.lam_t[e.lam_tbase + t] = lam;
+  }
+  if (!ok) return;
+  const int ids[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    g_add(b, V, vb + ids[k], d[k]);
+    atomicAdd(&b.cnt[vb + ids[k]], 1);
+  }
+}
+
+__global__ void k_neohookean(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, int64_t V) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= e.nt) return;
+  const int64_t ti = e.tbase + t, vb = e.vbase, T = b.n_tet_total;
+  const int4 q = ldi4(b.tets, ti);
+  float4 Pq[4] = {ld4(b.p, vb + q.x), ld4(b.p, vb + q.y), ld4(b.p, vb + q.z), ld4(b.p, vb + q.w)};
+  float M[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) M[k] = b.tet_inv_rest[k * T + ti];
+  f3 d[4];
+  if (!neohookean(Pq, b.tet_rest_volume[ti], M, nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt), d)) return;
+  const int ids[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    g_add(b, V, vb + ids[k], d[k]);
+    atomicAdd(&b.cnt[vb + ids[k]], 1);
+  }
+}
+
+__global__ void k_distance(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, int64_t V) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= e.ne) return;
+  const int64_t ei = e.ebase + k, vb = e.vbase;
+  const int2 q = ldi2(b.edges, ei);
+  const float4 A = ld4(b.p, vb + q.x), B = ld4(b.p, vb + q.y);
+  f3 d;
+  int ok;
+  if (prm.dist_mode == FFSRL_PBD) {
+    ok = distance_pbd(A, B, b.edge_rest[ei], prm.dist_k, &d);
+  } else {
+    float lam = b.lam_e ? b.lam_e[e.lam_ebase + k] : 0.0f;
+    ok = distance_xpbd(A, B, b.edge_rest[ei], prm.dist_k, &lam, &d);
+    if (b.lam_e && ok) b.lam_e[e.lam_ebase + k] = lam;
+  }
+  if (!ok) return;
+  g_add(b, V, vb + q.x, -(d * A.w));
+  g_add(b, V, vb + q.y, d * B.w);
+  atomicAdd(&b.cnt[vb + q.x], 1);
+  atomicAdd(&b.cnt[vb + q.y], 1);
+}
+
+__global__ void k_apply(ffsrl_batch b, int64_t vb, int nv, int64_t V) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int64_t g = vb + i;
+  const int n = b.cnt[g];
+  if (n > 0) {
+    float4 p = ld4(b.p, g);
+    const float N = (float)n;
+    f3 q = xyz(p) + mk(b.dp[g] / N, b.dp[V + g] / N, b.dp[2 * V + g] / N);
+    st4(b.p, g, with_w(q, p.w));
+  }
+  b.dp[g] = 0.0f;
+  b.dp[V + g] = 0.0f;
+  b.dp[2 * V + g] = 0.0f;
+  b.cnt[g] = 0;
+}
+
+__global__ void k_grasp(ffsrl_batch b, ffsrl_params prm, int env, int64_t vb, int nv) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const float dr = b.drag[vb + i];
+  if (dr == 0.0f) return;
+  const float* t = b.tips + (env * 4 + 2) * 3;
+  float4 p = ld4(b.p, vb + i);
+  st4(b.p, vb + i, with_w(grasp_pull(xyz(p), mk(t[0], t[1], t[2]), dr, prm.drag_ks), p.w));
+}
+
+__global__ void k_bounds_integrate(ffsrl_batch b, ffsrl_params prm, int64_t vb, int nv) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  float4 p = ld4(b.p, vb + i), x = ld4(b.x, vb + i);
+  if (p.y < prm.ground) p.y = prm.ground;  // bounds (integrator.py:12-21)
+  st4(b.p, vb + i, p);
+  const f3 v = (xyz(p) - xyz(x)) * (1.0f / prm.dt);  // PBDStep (integrator.py:23-38)
+  st4(b.x, vb + i, with_w(xyz(p), x.w));
+  st4(b.v, vb + i, with_w(v, 0.0f));
+}
+
+__global__ void k_zero(float* a, int64_t base, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[base + i] = 0.0f;
+}
+
+__global__ void k_move_tools(ffsrl_batch b, const float* delta, const float* lo, const float* hi) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n_env * 4) return;
+  const int env = i >> 2;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float t = b.tips[i * 3 + k] + delta[env * 3 + k];
+    if (lo) t = fmax_(t, lo[k]);
+    if (hi) t = fmin_(t, hi[k]);
+    b.tips[i * 3 + k] = t;
+  }
+}
+
+__global__ void k_grasp_select(ffsrl_batch b, int env, float radius, const int32_t* state,
+                               const int32_t* update) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const ffsrl_env_desc e = b.envs[env];
+  if (i >= e.nv) return;
+  if (update[env * 3] == 0) return;
+  const int64_t g = (int64_t)e.vbase + i;
+  if (state[env * 3] == 1) {
+    const float* t = b.tips + (env * 4 + 3) * 3;
+    const float len = length(xyz(ld4(b.x, g)) - mk(t[0], t[1], t[2]));
+    b.drag[g] = len < radius ? 1.0f : 0.0f;
+  } else {
+    b.drag[g] = 0.0f;
+  }
+}
+
+__global__ void k_remap(float* dst, const float* src, const int32_t* map, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  st4(dst, map[2 * i], ld4(src, map[2 * i + 1]));
+}
+
+static int host_envs(const ffsrl_batch* b, int begin, int end, ffsrl_env_desc* out, hipStream_t st) {
+  if (begin < 0 || end > b->n_env || begin > end) return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", begin, end, b->n_env);
+  if (end == begin) return 0;
+  FFSRL_HIP(hipMemcpyAsync(out, b->envs + begin, sizeof(ffsrl_env_desc) * (end - begin), hipMemcpyDeviceToHost, st));
+  FFSRL_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+static int64_t total_vertices(const ffsrl_env_desc* e, int n) {
+  int64_t v = 0;
+  for (int i = 0; i < n; ++i) v = std::max<int64_t>(v, (int64_t)e[i].vbase + e[i].nv);
+  return v;
+}
+
+}  // namespace ffsrl
+
+using namespace ffsrl;
+
+// Env descriptors of the plain form are cached on the host per batch pointer
+// (they are read once per call otherwise, which would synchronise the stream).
+#include <unordered_map>
+#include <vector>
+#include <mutex>
+
+namespace {
+struct PlainCache {
+  const ffsrl_env_desc* dev = nullptr;
+  int n = 0;
+  std::vector<ffsrl_env_desc> host;
+  int64_t vtotal = 0;
+};
+std::mutex g_cache_mu;
+PlainCache g_cache;
+
+int plain_envs(const ffsrl_batch* b, int begin, int end, hipStream_t st, const ffsrl_env_desc** out,
+               int64_t* vtotal) {
+  if (begin < 0 || end > b->n_env || begin > end)
+    return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", begin, end, b->n_env);
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  if (g_cache.dev != b->envs || g_cache.n != b->n_env) {
+    g_cache.host.resize(b->n_env);
+    if (b->n_env) {
+      int rc = host_envs(b, 0, b->n_env, g_cache.host.data(), st);
+      if (rc) return rc;
+    }
+    g_cache.dev = b->envs;
+    g_cache.n = b->n_env;
+    g_cache.vtotal = total_vertices(g_cache.host.data(), b->n_env);
+  }
+  *out = g_cache.host.data();
+  *vtotal = g_cache.vtotal;
+  return 0;
+}
+
+inline unsigned nblk(int n, int t) { return (unsigned)((n + t - 1) / t); }
+}  // namespace
+
+extern "C" {
+
+int ffsrl_abi_version(void) { return FFSRL_ABI_VERSION; }
+const char* ffsrl_last_error(void) { return g_err.c_str(); }
+int ffsrl_frame_max_vertices(void) { return kMaxFrameVertices; }
+
+int ffsrl_frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* order, int32_t block,
+                int32_t refit, void* stream) {
+  if (!prm) return fail(FFSRL_E_ARG, "params NULL");
+  int rc = check_batch(b, prm);
+  if (rc) return rc;
+  if (b->max_nv < 0 || b->max_nv > kMaxFrameVertices)
+    return fail(FFSRL_E_LIMIT, "max_nv %d exceeds the LDS capacity of the batched frame (%d vertices)", b->max_nv,
+                kMaxFrameVertices);
+  const bool carry = carries_lambda(*prm);
+  if (carry && ((prm->dist_mode == FFSRL_XPBD && !b->lam_e) || (prm->vol_mode == FFSRL_XPBD && !b->lam_t)))
+    return fail(FFSRL_E_ARG, "XPBD with iterations > 1 needs lambda arrays");
+  if (refit && (!b->tris || !b->node_left || !b->level_nodes || !b->level_off || !b->bvh_lo || !b->bvh_hi))
+    return fail(FFSRL_E_ARG, "refit requested without BVH arrays");
+  if (b->n_env == 0) return 0;
+  const int lds = lds_bytes_for(b->max_nv < 2 ? 2 : b->max_nv);
+  hipStream_t st = as_stream(stream);
+  switch (block) {
+    case 256: return launch_frame<256>(*b, *prm, order, refit, carry, lds, st);
+    case 512: return launch_frame<512>(*b, *prm, order, refit, carry, lds, st);
+    case 1024: return launch_frame<1024>(*b, *prm, order, refit, carry, lds, st);
+    default: return fail(FFSRL_E_ARG, "block must be 256, 512 or 1024 (got %d)", block);
+  }
+}
+
+int ffsrl_predict(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {
+  int rc = check_batch(b, prm);
+  if (rc) return rc;
+  const ffsrl_env_desc* h;
+  int64_t V;
+  hipStream_t st = as_stream(stream);
+  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  for (int i = eb; i < ee; ++i) {
+    if (!h[i].nv) continue;
+    hipLaunchKernelGGL(k_predict, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, *prm, (int64_t)h[i].vbase, h[i].nv);
+  }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_project_volume(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {
+  int rc = check_batch(b, prm);
+  if (rc) return rc;
+  if (prm->vol_mode == FFSRL_OFF) return 0;
+  if (!b->dp || !b->cnt) return fail(FFSRL_E_ARG, "plain form needs dp/cnt scratch");
+  const ffsrl_env_desc* h;
+  int64_t V;
+  hipStream_t st = as_stream(stream);
+  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  for (int i = eb; i < ee; ++i) {
+    if (!h[i].nt) continue;
+    hipLaunchKernelGGL(k_volume, dim3(nblk(h[i].nt, 256)), dim3(256), 0, st, *b, *prm, h[i], V);
+  }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_project_neohookean(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {
+  int rc = check_batch(b, prm);
+  if (rc) return rc;
+  if (!prm->nh_on) return 0;
+  if (!b->dp || !b->cnt) return fail(FFSRL_E_ARG, "plain form needs dp/cnt scratch");
+  const ffsrl_env_desc* h;
+  int64_t V;
+  hipStream_t st = as_stream(stream);
+  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  for (int i = eb; i < ee; ++i) {
+    if (!h[i].nt) continue;
+    hipLaunchKernelGGL(k_neohookean, dim3(nblk(h[i].nt, 256)), dim3(256), 0, st, *b, *prm, h[i], V);
+  }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_project_distance(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {
+  int rc = check_batch(b, prm);
+  if (rc) return rc;
+  if (prm->dist_mode == FFSRL_OFF) return 0;
+  if (!b->dp || !b->cnt) return fail(FFSRL_E_ARG, "plain form needs dp/cnt scratch");
+  const ffsrl_env_desc* h;
+  int64_t V;
+  hipStream_t st = as_stream(stream);
+  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  for (int i = eb; i < ee; ++i) {
+    if (!h[i].ne) continue;
+    hipLaunchKernelGGL(k_distance, dim3(nblk(h[i].ne, 256)), dim3(256), 0, st, *b, *prm, h[i], V);
+  }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_apply(const ffsrl_batch* b, int32_t eb, int32_t ee, void* stream) {
+  int rc = check_batch(b, nullptr);
+  if (rc) return rc;
+  if (!b->dp || !b->cnt) return fail(FFSRL_E_ARG, "plain form needs dp/cnt scratch");
+  const ffsrl_env_desc* h;
+  int64_t V;
+  hipStream_t st = as_stream(stream);
+  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  for (int i = eb; i < ee; ++i) {
+    if (!h[i].nv) continue;
+    hipLaunchKernelGGL(k_apply, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, (int64_t)h[i].vbase, h[i].nv, V);
+  }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_grasp(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {
+  int rc = check_batch(b, prm);
+  if (rc) return rc;
+  const ffsrl_env_desc* h;
+  int64_t V;
+  hipStream_t st = as_stream(stream);
+  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  for (int i = eb; i < ee; ++i) {
+    if (!h[i].nv) continue;
+    hipLaunchKernelGGL(k_grasp, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, *prm, i, (int64_t)h[i].vbase, h[i].nv);
+  }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_bounds_integrate(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {
+  int rc = check_batch(b, prm);
+  if (rc) return rc;
+  const ffsrl_env_desc* h;
+  int64_t V;
+  hipStream_t st = as_stream(stream);
+  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  for (int i = eb; i < ee; ++i) {
+    if (!h[i].nv) continue;
+    hipLaunchKernelGGL(k_bounds_integrate, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, *prm, (int64_t)h[i].vbase,
+                       h[i].nv);
+  }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_zero_lambdas(const ffsrl_batch* b, int32_t eb, int32_t ee, void* stream) {
+  int rc = check_batch(b, nullptr);
+  if (rc) return rc;
+  const ffsrl_env_desc* h;
+  int64_t V;
+  hipStream_t st = as_stream(stream);
+  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  for (int i = eb; i < ee; ++i) {
+    if (b->lam_e && h[i].ne)
+      hipLaunchKernelGGL(k_zero, dim3(nblk(h[i].ne, 256)), dim3(256), 0, st, b->lam_e, (int64_t)h[i].lam_ebase, h[i].ne);
+    if (b->lam_t && h[i].nt)
+      hipLaunchKernelGGL(k_zero, dim3(nblk(h[i].nt, 256)), dim3(256), 0, st, b->lam_t, (int64_t)h[i].lam_tbase, h[i].nt);
+  }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_move_tools(const ffsrl_batch* b, const float* delta, const float* lo, const float* hi, void* stream) {
+  if (!b || !b->tips || !delta) return fail(FFSRL_E_ARG, "tips and delta must be non-NULL");
+  if (b->n_env == 0) return 0;
+  hipLaunchKernelGGL(k_move_tools, dim3(nblk(b->n_env * 4, 256)), dim3(256), 0, as_stream(stream), *b, delta, lo, hi);
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_grasp_select(const ffsrl_batch* b, float radius, const int32_t* state, const int32_t* update,
+                       void* stream) {
+  int rc = check_batch(b, nullptr);
+  if (rc) return rc;
+  if (!state || !update) return fail(FFSRL_E_ARG, "state/update NULL");
+  const ffsrl_env_desc* h;
+  int64_t V;
+  hipStream_t st = as_stream(stream);
+  if ((rc = plain_envs(b, 0, b->n_env, st, &h, &V))) return rc;
+  for (int i = 0; i < b->n_env; ++i) {
+    if (!h[i].nv) continue;
+    hipLaunchKernelGGL(k_grasp_select, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, i, radius, state, update);
+  }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_remap(float* dst, const float* src, const int32_t* map, int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && (!dst || !src || !map))) return fail(FFSRL_E_ARG, "remap arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_remap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), dst, src, map, n);
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_graph_begin(void* stream) {
+  FFSRL_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+  return 0;
+}
+
+int ffsrl_graph_end(void* stream, void** graph_exec) {
+  if (!graph_exec) return fail(FFSRL_E_ARG, "graph_exec NULL");
+  hipGraph_t g = nullptr;
+  FFSRL_HIP(hipStreamEndCapture(as_stream(stream), &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
+  *graph_exec = ex;
+  return 0;
+}
+
+int ffsrl_graph_launch(void* graph_exec, void* stream) {
+  if (!graph_exec) return fail(FFSRL_E_ARG, "graph_exec NULL");
+  FFSRL_HIP(hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graph_exec), as_stream(stream)));
+  return 0;
+}
+
+int ffsrl_graph_destroy(void* graph_exec) {
+  if (!graph_exec) return 0;
+  FFSRL_HIP(hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(graph_exec)));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,234 @@
+// Ray-cast renderer: drawSimBVHKernelManyEnvs + meshQueryRayManyEnvs +
+// downsample_kernel (render.py:276-498, 699-724) and the vertex normals
+// (render.py:81-121).
+//
+// One wavefront renders an 8x8 tile of the 2x-supersampled ray grid of one
+// environment; each lane owns one ray and a 32-entry traversal stack in LDS
+// (slot-major, so the 64 lanes of a slot fall in distinct banks).  The 2x2
+// box filter is done across lanes with shuffles in the reference's summation
+// order, so no ray buffer goes to HBM.  Traversal order, slab test, the
+// Moller-Trumbore test and the strict "0 < t < tMin" acceptance follow the
+// reference expression by expression, so hit triangles are bit-exact.
+#include "common.hpp"
+
+namespace ffsrl {
+
+constexpr int kStack = 32;
+constexpr int kWaves = 4;
+
+__device__ __forceinline__ int isect_aabb(f3 o, f3 d, f3 lo, f3 hi, float* tmin_out) {
+  const float t1 = (lo.x - o.x) / d.x, t2 = (hi.x - o.x) / d.x;
+  const float t3 = (lo.y - o.y) / d.y, t4 = (hi.y - o.y) / d.y;
+  const float t5 = (lo.z - o.z) / d.z, t6 = (hi.z - o.z) / d.z;
+  const float tmin = fmax_(fmax_(fmin_(t1, t2), fmin_(t3, t4)), fmin_(t5, t6));
+  const float tmax = fmin_(fmin_(fmax_(t1, t2), fmax_(t3, t4)), fmax_(t5, t6));
+  *tmin_out = tmin;
+  return (tmax >= 0.0f) && (tmax >= tmin);
+}
+
+__device__ __forceinline__ float mt_isect(f3 a, f3 b, f3 c, f3 o, f3 d) {
+  const f3 e1 = b - a, e2 = c - a;
+  const f3 pv = cross(d, e2);
+  const float det = dot(e1, pv);
+  if (fabsf(det) < 0.0001f) return -1.0f;
+  const float inv = 1.0f / det;
+  const f3 tv = o - a;
+  const float u = dot(tv, pv) * inv;
+  if (u < 0.0f || u > 1.0f) return -1.0f;
+  const f3 qv = cross(tv, e1);
+  const float v = dot(d, qv) * inv;
+  if (v < 0.0f || u + v > 1.0f) return -1.0f;
+  return dot(e2, qv) * inv;
+}
+
+struct RenderOut {
+  float* depth;
+  float* rgb;
+  const float* normals;
+  const float* vcolor;
+  const float* tcolor;
+  int32_t* hit;
+  float* thit;
+};
+
+__global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_camera cam, int env_begin, RenderOut out) {
+  __shared__ int stack[kWaves][kStack][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int RW = 2 * cam.width, RH = 2 * cam.height;
+  const int tiles_x = RW >> 3, tiles = tiles_x * (RH >> 3);
+  const int tile = blockIdx.x * kWaves + wave;
+  if (tile >= tiles) return;  // whole wavefront exits; no block barrier below
+  const int el = blockIdx.y, env = env_begin + el;
+  const int col = (tile % tiles_x) * 8 + (lane & 7), row = (tile / tiles_x) * 8 + (lane >> 3);
+  const int r = row * RW + col;
+  const ffsrl_env_desc e = b.envs[env];
+  const int64_t vb = e.vbase, sb = e.boundbase, fb = e.fbase;
+  const f3 o = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
+  const f3 d = xyz(ld4(cam.dirs, r));
+
+  int* stk = &stack[wave][0][lane];
+  stk[0] = 0;
+  int count = 1, best = -1;
+  float tMin = 1e6f;
+  while (count) {
+    const int node = stk[(count - 1) * 64];
+    count -= 1;
+    const float4 lo = ld4(b.bvh_lo, sb + node), hi = ld4(b.bvh_hi, sb + node);
+    float tb;
+    if (isect_aabb(o, d, xyz(lo), xyz(hi), &tb) && tb < tMin) {
+      const int n = __float_as_int(hi.w);
+      if (n > 0) {
+        const int first = __float_as_int(lo.w);
+        for (int k = 0; k < n; ++k) {
+          const int4 t = ldi4(b.tris, fb + first + k);
+          const float tt = mt_isect(xyz(ld4(b.x, vb + t.x)), xyz(ld4(b.x, vb + t.y)), xyz(ld4(b.x, vb + t.z)), o, d);
+          if (tt < tMin && tt > 0.0f) {
+            tMin = tt;
+            best = first + k;
+          }
+        }
+      } else if (count + 2 <= kStack) {  // depth checked at admission: never false
+        const int l = __float_as_int(lo.w);
+        stk[count * 64] = l;
+        stk[(count + 1) * 64] = l + 1;
+        count += 2;
+      }
+    }
+  }
+  const bool hitp = tMin < 1e6f;
+  int4 tri = make_int4(0, 0, 0, -1);
+  if (hitp) tri = ldi4(b.tris, fb + best);
+  const int64_t rays_env = (int64_t)RW * RH;
+  if (out.hit) out.hit[el * rays_env + r] = hitp ? tri.w : -1;
+  if (out.thit) out.thit[el * rays_env + r] = hitp ? tMin : 0.0f;
+
+  f3 color = mk(0.0f, 0.0f, 0.0f);
+  if (hitp) {
+    if (cam.mode == 4) {
+      const float c = tMin / 50.0f * 1.0f;
+      color = mk(c, c, c);
+    } else {
+      const f3 A = xyz(ld4(b.x, vb + tri.x)), B = xyz(ld4(b.x, vb + tri.y)), C = xyz(ld4(b.x, vb + tri.z));
+      const f3 p = o + d * tMin;
+      const float area = length(cross(B - A, C - A));
+      const float w = length(cross(B - A, p - A)) / area;
+      const float v = length(cross(p - A, C - A)) / area;
+      const float u = 1.0f - w - v;
+      const f3 na = xyz(ld4(out.normals, vb + tri.x)), nb = xyz(ld4(out.normals, vb + tri.y)),
+               nc = xyz(ld4(out.normals, vb + tri.z));
+      const f3 normal = (na * u + nb * v) + nc * w;
+      if (cam.mode == 2) {
+        color = normal * 0.5f + mk(0.5f, 0.5f, 0.5f);
+      } else {
+        if (cam.mode == 0) {
+          color = mk(1.0f, 1.0f, 1.0f);
+        } else if (cam.mode == 1) {
+          const float* tc = out.tcolor + 3 * (int64_t)(fb + best);
+          color = mk(tc[0], tc[1], tc[2]);
+        } else {
+          const float* ca = out.vcolor + 3 * (vb + tri.x);
+          const float* cb = out.vcolor + 3 * (vb + tri.y);
+          const float* cc = out.vcolor + 3 * (vb + tri.z);
+          color = ((mk(ca[0], ca[1], ca[2]) + mk(cb[0], cb[1], cb[2])) + mk(cc[0], cc[1], cc[2])) / 3.0f;
+        }
+        float val = cam.light_intensity * dot(normal, mk(cam.light_dir[0], cam.light_dir[1], cam.light_dir[2]));
+        if (val < 0.0f) val = 0.0f;
+        const float lambert = 0.0f + val;
+        color = color * lambert;
+      }
+    }
+    if (color.x > 1.0f) color.x = 1.0f;
+    if (color.y > 1.0f) color.y = 1.0f;
+    if (color.z > 1.0f) color.z = 1.0f;
+  }
+  // 2x2 box: rays (2py+i, 2px+j) summed in (i, j) = (0,0), (0,1), (1,0), (1,1) order.
+  const bool owner = ((lane & 1) == 0) && (((lane >> 3) & 1) == 0);
+  const int64_t pix = (int64_t)el * cam.height * cam.width + (row >> 1) * cam.width + (col >> 1);
+  if (cam.mode == 4) {
+    const float c01 = __shfl_down(color.x, 1), c10 = __shfl_down(color.x, 8), c11 = __shfl_down(color.x, 9);
+    if (owner && out.depth) out.depth[pix] = (((color.x + c01) + c10) + c11) / 4.0f;
+  } else {
+    float ch[3] = {color.x, color.y, color.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float c01 = __shfl_down(ch[k], 1), c10 = __shfl_down(ch[k], 8), c11 = __shfl_down(ch[k], 9);
+      if (owner && out.rgb) out.rgb[3 * pix + k] = (((ch[k] + c01) + c10) + c11) / 4.0f;
+    }
+  }
+}
+
+// Area-weighted vertex normals of one env's render mesh, summed in LDS.
+__global__ __launch_bounds__(256) void k_normals(ffsrl_batch b, int env_begin, float* normals) {
+  extern __shared__ float lds[];
+  const ffsrl_env_desc e = b.envs[env_begin + blockIdx.x];
+  const int nv = e.nv;
+  float* sx = lds;
+  float* sy = lds + nv;
+  float* sz = lds + 2 * nv;
+  for (int i = threadIdx.x; i < nv; i += 256) sx[i] = sy[i] = sz[i] = 0.0f;
+  __syncthreads();
+  const int64_t vb = e.vbase;
+  for (int f = threadIdx.x; f < e.nf; f += 256) {
+    const int4 t = ldi4(b.tris, (int64_t)e.fbase + f);
+    const f3 A = xyz(ld4(b.x, vb + t.x)), B = xyz(ld4(b.x, vb + t.y)), C = xyz(ld4(b.x, vb + t.z));
+    const f3 n = cross(B - A, C - A);
+    const int ids[3] = {t.x, t.y, t.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      atomicAdd(&sx[ids[k]], n.x);
+      atomicAdd(&sy[ids[k]], n.y);
+      atomicAdd(&sz[ids[k]], n.z);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nv; i += 256) {
+    const f3 n = normalize(mk(sx[i], sy[i], sz[i]));
+    st4(normals, vb + i, make_float4(n.x, n.y, n.z, 0.0f));
+  }
+}
+
+}  // namespace ffsrl
+
+using namespace ffsrl;
+
+extern "C" {
+
+int ffsrl_render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t eb, int32_t ee, float* depth, float* rgb,
+                 const float* normals, const float* vcolor, const float* tcolor, int32_t* hit, float* thit,
+                 void* stream) {
+  if (!b || !cam || !b->envs || !b->x || !b->tris || !b->bvh_lo || !b->bvh_hi || !cam->dirs)
+    return fail(FFSRL_E_ARG, "render needs positions, triangles, BVH bounds and ray directions");
+  if (cam->width <= 0 || cam->height <= 0 || (cam->width & 3) || (cam->height & 3))
+    return fail(FFSRL_E_ARG, "render size %dx%d: width and height must be positive multiples of 4", cam->height,
+                cam->width);
+  if (eb < 0 || ee > b->n_env || eb > ee) return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", eb, ee, b->n_env);
+  const int m = cam->mode;
+  if (m != 0 && m != 1 && m != 2 && m != 3 && m != 4) return fail(FFSRL_E_ARG, "render mode %d", m);
+  if (m == 4 && !depth && !hit) return fail(FFSRL_E_ARG, "depth mode needs a depth (or hit) output");
+  if (m != 4 && (!rgb || !normals)) return fail(FFSRL_E_ARG, "colour modes need rgb output and normals");
+  if (m == 3 && !vcolor) return fail(FFSRL_E_ARG, "vertex colour mode needs vertex colours");
+  if (m == 1 && !tcolor) return fail(FFSRL_E_ARG, "triangle colour mode needs triangle colours");
+  if (ee == eb) return 0;
+  if (ee - eb > 65535) return fail(FFSRL_E_LIMIT, "at most 65535 environments per render call");
+  const int tiles = (2 * cam->width / 8) * (2 * cam->height / 8);
+  RenderOut out{depth, rgb, normals, vcolor, tcolor, hit, thit};
+  hipLaunchKernelGGL(k_render, dim3((tiles + kWaves - 1) / kWaves, ee - eb), dim3(64 * kWaves), 0, as_stream(stream),
+                     *b, *cam, eb, out);
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_vertex_normals(const ffsrl_batch* b, int32_t eb, int32_t ee, float* normals, void* stream) {
+  if (!b || !b->envs || !b->x || !b->tris || !normals) return fail(FFSRL_E_ARG, "normals: NULL argument");
+  if (eb < 0 || ee > b->n_env || eb > ee) return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", eb, ee, b->n_env);
+  if (b->max_nv > 160 * 1024 / 12) return fail(FFSRL_E_LIMIT, "max_nv %d exceeds the LDS capacity of normals", b->max_nv);
+  if (ee == eb) return 0;
+  const int lds = 12 * (b->max_nv < 1 ? 1 : b->max_nv);
+  FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_normals), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                lds));
+  hipLaunchKernelGGL(k_normals, dim3(ee - eb), dim3(256), lds, as_stream(stream), *b, eb, normals);
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

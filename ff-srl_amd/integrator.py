@@ -1,0 +1,52 @@
+"""SimIntegratorDO: one frame of every environment.
+
+``stepModel`` mirrors ``SimIntegratorDO.stepModel`` (``FF_SRL/FF_SRL/
+integrator.py:1640-1744``): S substeps of predict, I iterations of the
+constraint projection with Jacobi averaging and the grasp pull, the ground
+clamp and the integrate.  Two launch forms run the same arithmetic:
+
+* batched (default): ``ffsrl_frame``, one launch for all environments and
+  the whole substep loop (and optionally the BVH refit);
+* plain: the reference's launch model with one launch per kernel per
+  environment (``integrator.py:1779-1929`` is the reference's per-env
+  precedent), used as the comparison baseline.
+"""
+from __future__ import annotations
+
+import torch
+
+from ._lib import lib
+from .model import SimModelDO
+
+
+class SimIntegratorDO:
+    def __init__(self, device="cuda:0", block: int = 1024, dispatch_sorted: bool = True):
+        self.device = device
+        self.block = block
+        self.dispatch_sorted = dispatch_sorted
+
+    def stepModel(self, simModel: SimModelDO, refit: bool = False) -> None:
+        """One frame of all environments in one launch (``ffsrl_frame``)."""
+        m = simModel
+        order = m.dispatch_order().data_ptr() if self.dispatch_sorted else None
+        lib.ffsrl_frame(m.batch, m.params, order, self.block, int(bool(refit)), m.stream())
+
+    def stepModelPlain(self, simModel: SimModelDO, env_begin: int = 0, env_end: int = None) -> None:
+        """One frame with one launch per kernel per environment (the library
+        loops over the environments of each call)."""
+        m = simModel
+        m.enable_plain()
+        eb, ee = env_begin, m.numEnvs if env_end is None else env_end
+        b, prm, s = m.batch, m.params, m.stream()
+        reset = m.carries_lambda and (not m.lambda_legacy or m.simConstraints > 1)
+        for _ in range(m.simSubsteps):
+            lib.ffsrl_predict(b, prm, eb, ee, s)
+            if reset:
+                lib.ffsrl_zero_lambdas(b, eb, ee, s)
+            for _ in range(m.simConstraints):
+                lib.ffsrl_project_volume(b, prm, eb, ee, s)
+                lib.ffsrl_project_distance(b, prm, eb, ee, s)
+                lib.ffsrl_project_neohookean(b, prm, eb, ee, s)
+                lib.ffsrl_apply(b, eb, ee, s)
+                lib.ffsrl_grasp(b, prm, eb, ee, s)
+            lib.ffsrl_bounds_integrate(b, prm, eb, ee, s)

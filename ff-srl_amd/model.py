@@ -1,0 +1,356 @@
+"""SimModelDO: data-oriented multi-environment state on the GPU.
+
+Mirrors ``FF_SRL/FF_SRL/modelDO.py:772-1645`` (``SimModelDO``): the state of
+all environments lives in flat arrays, every kernel processes all of them, and
+observations are torch tensors.  Differences by design:
+
+* Environments may use different meshes.  A CSR environment table (one
+  ``ffsrl_env_desc`` per env) gives each env its state range and the ranges of
+  its mesh's shared topology (edges, tets, render triangles, BVH nodes), so
+  identical envs share one copy of the topology instead of the reference's
+  per-env replicas (``modelDO.py:916-999``).
+* Positions are float4 (x, y, z, inverse mass) so a constraint gathers a
+  vertex and its mass in one 16-byte load.
+* The BVH topology of each mesh is built here (host binned SAH in the C++
+  library) because the batched frame kernel refits it in the same launch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from ._lib import Batch, EnvDesc, ENV_DESC_FIELDS, OFF, PBD, XPBD, Params, lib
+from .scenes import TetMesh
+
+MAX_STACK = 32  # traversal stack entries per ray (render.py:424, 924)
+
+_MODES = {None: OFF, "off": OFF, "pbd": PBD, "xpbd": XPBD}
+
+
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+class MeshTopology:
+    """Host-side topology of one mesh plus its BVH (built once)."""
+
+    def __init__(self, mesh: TetMesh, bins: int = 8):
+        self.mesh = mesh
+        surf = mesh.surface_vertices()
+        local = np.full(mesh.num_vertices, -1, dtype=np.int32)
+        local[surf] = np.arange(surf.size, dtype=np.int32)
+        faces_s = np.ascontiguousarray(local[mesh.tris]).astype(np.int32)
+        verts_s = np.ascontiguousarray(mesh.verts[surf]).astype(np.float32)
+        nf = mesh.num_tris
+        left = np.empty(2 * nf, np.int32)
+        first = np.empty(2 * nf, np.int32)
+        count = np.empty(2 * nf, np.int32)
+        level = np.empty(2 * nf, np.int32)
+        tri_map = np.empty(nf, np.int32)
+        maxl, nn = C.c_int32(0), C.c_int32(0)
+        lib.ffsrl_bvh_build(verts_s.ctypes.data, verts_s.shape[0], faces_s.ctypes.data, nf, bins,
+                            left.ctypes.data, first.ctypes.data, count.ctypes.data, level.ctypes.data,
+                            tri_map.ctypes.data, C.byref(nn), C.byref(maxl))
+        n = int(nn.value)
+        self.num_nodes = n
+        self.max_level = int(maxl.value)
+        if self.max_level + 1 > MAX_STACK:
+            raise ValueError(f"BVH of {mesh.name} has {self.max_level + 1} levels; the ray stack holds {MAX_STACK}")
+        self.left = left[:n].copy()
+        self.first = first[:n].copy()
+        self.count = count[:n].copy()
+        self.level = level[:n].copy()
+        self.tri_map = tri_map
+        # Unsplit root: the reference marks it with leftNode 0 (bvh.py:480); -1 is used here.
+        # Leaf-order triangles: (a, b, c, original face id).
+        self.tris_leaf = np.concatenate([mesh.tris[tri_map], tri_map[:, None]], axis=1).astype(np.int32)
+        order = []
+        self.level_off = [0]
+        for L in range(self.max_level, -1, -1):
+            ids = np.nonzero(self.level == L)[0]
+            order.append(ids)
+            self.level_off.append(self.level_off[-1] + ids.size)
+        self.level_nodes = np.concatenate(order).astype(np.int32)
+        self.level_off = np.asarray(self.level_off, dtype=np.int32)
+        self.tri_colors_leaf = mesh.tri_colors[tri_map] if mesh.tri_colors is not None else None
+
+
+class SimModelDO:
+    """Batched environments (``modelDO.py:772``) on one GPU.
+
+    ``meshes[env_mesh[i]]`` is the mesh of env ``i``.  Physics parameters keep
+    the reference's constructor names (``modelDO.py:774-802``); the constraint
+    forms are chosen with ``distance`` / ``volume`` ("pbd", "xpbd" or None) and
+    ``neoHookean``.  XPBD compliances are given as alpha and converted to
+    alpha~ = alpha / dt^2 (``modelDO.py:828-830``).
+    """
+
+    def __init__(self, meshes: Sequence[TetMesh], env_mesh: Optional[Sequence[int]] = None, device="cuda:0",
+                 simSubsteps=8, simFrameRate=60, simConstraintsSteps=1,
+                 environmentGravity=(0.0, -9.8, 0.0), environmentVelocityDampening=0.1,
+                 environmentGroundLevel=0.0, globalKsDistance=1.0, globalKsVolume=1.0, globalKsDrag=1.0,
+                 distance="pbd", volume="pbd", neoHookean=False, distanceCompliance=0.0,
+                 volumeComplianceXPBD=0.0, globalVolumeCompliance=1.0, globalDeviatoricCompliance=1.0,
+                 globalLaparoscopeDragLookupRadius=0.15, lambdaLegacy=False, bins=8,
+                 initialVelocity: Optional[np.ndarray] = None, tips: Optional[np.ndarray] = None):
+        self.device = torch.device(device)
+        self.meshes = list(meshes)
+        env_mesh = list(range(len(self.meshes))) if env_mesh is None else list(env_mesh)
+        self.env_mesh = np.asarray(env_mesh, dtype=np.int64)
+        self.numEnvs = len(env_mesh)
+        self.simSubsteps = int(simSubsteps)
+        self.simFrameRate = simFrameRate
+        self.simConstraints = int(simConstraintsSteps)
+        self.simDt = (1.0 / self.simFrameRate) / self.simSubsteps
+        self.gravity = tuple(float(g) for g in environmentGravity)
+        self.velocityDampening = environmentVelocityDampening
+        self.groundLevel = environmentGroundLevel
+        self.globalKsDistance = globalKsDistance
+        self.globalKsVolume = globalKsVolume
+        self.globalKsDrag = globalKsDrag
+        self.globalLaparoscopeDragLookupRadius = globalLaparoscopeDragLookupRadius
+        self.dist_mode = _MODES[distance]
+        self.vol_mode = _MODES[volume]
+        self.nh_on = bool(neoHookean)
+        self.distanceCompliance = distanceCompliance
+        self.volumeComplianceXPBD = volumeComplianceXPBD
+        self.globalVolumeComplianceNeoHookean = globalVolumeCompliance
+        self.gloabalDeviatoricComplianceNeoHookean = globalDeviatoricCompliance
+        self.lambda_legacy = bool(lambdaLegacy)
+        self.params = self._make_params()
+
+        self.topo = [MeshTopology(m, bins) for m in self.meshes]
+        self._build_topology()
+        self._build_envs()
+        self._build_state(initialVelocity, tips)
+        self._batch = self._make_batch()
+        self._order = None
+
+    # ------------------------------------------------------------------ setup
+    def _make_params(self) -> Params:
+        p = Params()
+        p.substeps = self.simSubsteps
+        p.iterations = self.simConstraints
+        p.dt = self.simDt
+        p.gravity[:] = self.gravity
+        p.damping = self.velocityDampening
+        p.ground = self.groundLevel
+        p.dist_mode = self.dist_mode
+        p.dist_k = self.globalKsDistance if self.dist_mode == PBD else self.distanceCompliance / (self.simDt * self.simDt)
+        p.vol_mode = self.vol_mode
+        p.vol_k = self.globalKsVolume if self.vol_mode == PBD else self.volumeComplianceXPBD / (self.simDt * self.simDt)
+        p.nh_on = int(self.nh_on)
+        p.nh_vol_c = self.globalVolumeComplianceNeoHookean
+        p.nh_dev_c = self.gloabalDeviatoricComplianceNeoHookean
+        p.drag_ks = self.globalKsDrag
+        p.lambda_legacy = int(self.lambda_legacy)
+        return p
+
+    @property
+    def carries_lambda(self) -> bool:
+        xpbd = self.dist_mode == XPBD or self.vol_mode == XPBD
+        return xpbd and (self.simConstraints > 1 or self.lambda_legacy)
+
+    def _build_topology(self):
+        dev = self.device
+        off = {"e": [0], "t": [0], "f": [0], "n": [0], "l": [0]}
+        for t in self.topo:
+            m = t.mesh
+            off["e"].append(off["e"][-1] + m.num_edges)
+            off["t"].append(off["t"][-1] + m.num_tets)
+            off["f"].append(off["f"][-1] + m.num_tris)
+            off["n"].append(off["n"][-1] + t.num_nodes)
+            off["l"].append(off["l"][-1] + t.level_off.size)
+        self.mesh_off = {k: np.asarray(v, dtype=np.int64) for k, v in off.items()}
+        cat = np.concatenate
+        self.t_edges = torch.from_numpy(cat([t.mesh.edges for t in self.topo]).astype(np.int32)).to(dev)
+        self.t_edge_rest = torch.from_numpy(cat([t.mesh.edge_rest for t in self.topo])).to(dev)
+        self.t_tets = torch.from_numpy(cat([t.mesh.tets for t in self.topo]).astype(np.int32)).to(dev)
+        self.t_tet_rest_volume = torch.from_numpy(cat([t.mesh.tet_rest_volume for t in self.topo])).to(dev)
+        inv_rest = cat([t.mesh.tet_inv_rest.reshape(-1, 9) for t in self.topo]).astype(np.float32)
+        self.t_tet_inv_rest = torch.from_numpy(np.ascontiguousarray(inv_rest.T)).to(dev)
+        self.t_tris = torch.from_numpy(cat([t.tris_leaf for t in self.topo])).to(dev)
+        self.t_node_left = torch.from_numpy(cat([t.left for t in self.topo])).to(dev)
+        self.t_node_first = torch.from_numpy(cat([t.first for t in self.topo])).to(dev)
+        self.t_node_count = torch.from_numpy(cat([t.count for t in self.topo])).to(dev)
+        self.t_level_nodes = torch.from_numpy(cat([t.level_nodes for t in self.topo])).to(dev)
+        self.t_level_off = torch.from_numpy(cat([t.level_off for t in self.topo])).to(dev)
+        tc = [t.tri_colors_leaf if t.tri_colors_leaf is not None else np.ones((t.mesh.num_tris, 3), np.float32)
+              for t in self.topo]
+        self.t_tri_colors = torch.from_numpy(cat(tc).astype(np.float32)).to(dev)
+        self.n_tet_total = int(self.t_tets.shape[0])
+
+    def _build_envs(self):
+        n = self.numEnvs
+        desc = np.zeros((n, len(ENV_DESC_FIELDS)), dtype=np.int32)
+        F = {f: i for i, f in enumerate(ENV_DESC_FIELDS)}
+        vb = nb = le = lt = 0
+        carry = self.carries_lambda
+        self.env_vbase = np.zeros(n + 1, dtype=np.int64)
+        self.env_nbase = np.zeros(n + 1, dtype=np.int64)
+        for i in range(n):
+            k = int(self.env_mesh[i])
+            t = self.topo[k]
+            m = t.mesh
+            row = desc[i]
+            row[F["vbase"]], row[F["nv"]] = vb, m.num_vertices
+            row[F["ebase"]], row[F["ne"]] = self.mesh_off["e"][k], m.num_edges
+            row[F["tbase"]], row[F["nt"]] = self.mesh_off["t"][k], m.num_tets
+            row[F["fbase"]], row[F["nf"]] = self.mesh_off["f"][k], m.num_tris
+            row[F["nodebase"]], row[F["nn"]] = self.mesh_off["n"][k], t.num_nodes
+            row[F["boundbase"]] = nb
+            row[F["lvlbase"]], row[F["nlvl"]] = self.mesh_off["l"][k], t.max_level + 1
+            row[F["lam_ebase"]], row[F["lam_tbase"]] = (le, lt) if carry else (0, 0)
+            vb += m.num_vertices
+            nb += t.num_nodes
+            if carry:
+                le += m.num_edges
+                lt += m.num_tets
+            self.env_vbase[i + 1] = vb
+            self.env_nbase[i + 1] = nb
+        if vb >= 2 ** 31 or nb >= 2 ** 31:
+            raise ValueError("state exceeds int32 indexing")
+        self.env_desc_host = desc
+        self.env_desc = torch.from_numpy(desc).to(self.device)
+        self.numVertices = vb
+        self.numNodes = nb
+        self.num_lam_e, self.num_lam_t = le, lt
+        self.max_nv = int(max(m.num_vertices for m in self.meshes)) if self.meshes else 0
+        work = np.array([self.meshes[k].num_tets * 4 + self.meshes[k].num_edges for k in self.env_mesh])
+        self.work = work
+
+    def _build_state(self, initial_velocity, tips):
+        dev = self.device
+        V = self.numVertices
+        x0 = np.zeros((V, 4), dtype=np.float32)
+        for i in range(self.numEnvs):
+            m = self.meshes[int(self.env_mesh[i])]
+            a, b = self.env_vbase[i], self.env_vbase[i + 1]
+            x0[a:b, :3] = m.verts
+            x0[a:b, 3] = m.inv_mass
+        v0 = np.zeros((V, 4), dtype=np.float32)
+        if initial_velocity is not None:
+            v0[:, :3] = np.asarray(initial_velocity, dtype=np.float32).reshape(V, 3)
+        self.initialVertex = torch.from_numpy(x0).to(dev)
+        self.initialVelocity = torch.from_numpy(v0).to(dev)
+        self.x = self.initialVertex.clone()
+        self.p = self.initialVertex.clone()
+        self.v = self.initialVelocity.clone()
+        self.drag = torch.zeros(V, dtype=torch.float32, device=dev)
+        self.lam_e = torch.zeros(max(self.num_lam_e, 1), dtype=torch.float32, device=dev) if self.carries_lambda else None
+        self.lam_t = torch.zeros(max(self.num_lam_t, 1), dtype=torch.float32, device=dev) if self.carries_lambda else None
+        self.bvh_lo = torch.zeros((max(self.numNodes, 1), 4), dtype=torch.float32, device=dev)
+        self.bvh_hi = torch.zeros((max(self.numNodes, 1), 4), dtype=torch.float32, device=dev)
+        t0 = np.zeros((self.numEnvs, 4, 3), dtype=np.float32) if tips is None else np.asarray(tips, np.float32)
+        self.initialTips = torch.from_numpy(t0.reshape(self.numEnvs, 4, 3).copy()).to(dev)
+        self.laparoscopeTip = self.initialTips.clone()
+        vc = np.concatenate([self.meshes[int(k)].colors for k in self.env_mesh]) if self.numEnvs else np.zeros((0, 3))
+        self.vertexColors = torch.from_numpy(vc.astype(np.float32)).to(dev)
+        self.normals = torch.zeros((V, 4), dtype=torch.float32, device=dev)
+        self.dp = None
+        self.cnt = None
+
+    def _make_batch(self) -> Batch:
+        b = Batch()
+        b.n_env = self.numEnvs
+        b.max_nv = self.max_nv
+        b.n_tet_total = self.n_tet_total
+        b.envs = _ptr(self.env_desc)
+        b.edges = _ptr(self.t_edges)
+        b.edge_rest = _ptr(self.t_edge_rest)
+        b.tets = _ptr(self.t_tets)
+        b.tet_rest_volume = _ptr(self.t_tet_rest_volume)
+        b.tet_inv_rest = _ptr(self.t_tet_inv_rest)
+        b.tris = _ptr(self.t_tris)
+        b.node_left = _ptr(self.t_node_left)
+        b.node_first = _ptr(self.t_node_first)
+        b.node_count = _ptr(self.t_node_count)
+        b.level_nodes = _ptr(self.t_level_nodes)
+        b.level_off = _ptr(self.t_level_off)
+        b.x = _ptr(self.x)
+        b.p = _ptr(self.p)
+        b.v = _ptr(self.v)
+        b.drag = _ptr(self.drag)
+        b.lam_e = _ptr(self.lam_e)
+        b.lam_t = _ptr(self.lam_t)
+        b.bvh_lo = _ptr(self.bvh_lo)
+        b.bvh_hi = _ptr(self.bvh_hi)
+        b.tips = _ptr(self.laparoscopeTip)
+        b.dp = _ptr(self.dp)
+        b.cnt = _ptr(self.cnt)
+        return b
+
+    def enable_plain(self):
+        """Allocate the dP / count scratch of the plain (per-kernel) form."""
+        if self.dp is None:
+            self.dp = torch.zeros((3, self.numVertices), dtype=torch.float32, device=self.device)
+            self.cnt = torch.zeros(self.numVertices, dtype=torch.int32, device=self.device)
+            self._batch.dp = _ptr(self.dp)
+            self._batch.cnt = _ptr(self.cnt)
+
+    @property
+    def batch(self) -> Batch:
+        return self._batch
+
+    def dispatch_order(self) -> torch.Tensor:
+        """Environments largest first: the hardware dispatcher then assigns
+        them to CUs in longest-processing-time order."""
+        if self._order is None:
+            order = np.argsort(-self.work, kind="stable").astype(np.int32)
+            self._order = torch.from_numpy(order).to(self.device)
+        return self._order
+
+    @staticmethod
+    def stream() -> int:
+        return torch.cuda.current_stream().cuda_stream
+
+    # ---------------------------------------------------------------- getters
+    def env_slice(self, i: int) -> slice:
+        return slice(int(self.env_vbase[i]), int(self.env_vbase[i + 1]))
+
+    def vertex(self) -> torch.Tensor:
+        """(V, 3) view of all positions (``SimModelDO.vertex``)."""
+        return self.x[:, :3]
+
+    def env_vertices(self, i: int) -> torch.Tensor:
+        return self.x[self.env_slice(i), :3]
+
+    def getVertexPositionsTensor(self, vertexId):
+        """``modelDO.py:1483``: positions of global vertex ids."""
+        return self.x[:, :3][vertexId]
+
+    def getVertexPositionsTensorMultiEnvs(self, vertexIds):
+        """``modelDO.py:1489``: env-local vertex ids, one per env."""
+        ids = torch.as_tensor(vertexIds, device=self.device).reshape(-1)
+        base = torch.as_tensor(self.env_vbase[:-1], device=self.device)
+        return self.x[:, :3][ids + base[: ids.numel()]]
+
+    def getLaparoscopePositionsTensor(self):
+        """``modelDO.py:1496``: drag point (index 3) of each env's instrument."""
+        return self.laparoscopeTip[:, 3, :]
+
+    def getLaparoscopeClampVertex(self, vertexId):
+        return self.drag[vertexId]
+
+    # ------------------------------------------------------------------ reset
+    def resetFixedModel(self, envNums):
+        """``modelDO.py:1589-1622``: restore positions, zero predicted positions,
+        velocities and grasp weights, restore the instrument points."""
+        for i in envNums:
+            s = self.env_slice(int(i))
+            self.x[s] = self.initialVertex[s]
+            self.p[s] = 0.0
+            self.p[s, 3] = self.initialVertex[s, 3]
+            self.v[s] = 0.0
+            self.drag[s] = 0.0
+            self.laparoscopeTip[int(i)] = self.initialTips[int(i)]
+
+    def set_grasp(self, env: int, local_ids, weight: float = 1.0):
+        """Force grasp weights on env-local vertices (``forceClampsDragKernel``,
+        ``modelDO.py:281-289``)."""
+        ids = torch.as_tensor(np.asarray(local_ids, dtype=np.int64) + int(self.env_vbase[env]), device=self.device)
+        self.drag[ids] = weight

@@ -1,0 +1,145 @@
+"""Synthetic workloads of the benchmark configurations.
+
+* cube     - 1 env, 10x10x10 vertex cube, 5 tets/cell (3645 tets), XPBD
+             distance (compliance 1e-6) + XPBD volume (compliance 0), bottom
+             face pinned, g = -9.81, dt = 1/60, 10 substeps x 1 iteration,
+             seeded initial velocity noise U[-0.01, 0.01], 128x128 depth.
+* ellipsoids - n envs of a seeded ~20k-tet ellipsoid (1, 0.6, 0.4), Neo-Hookean
+             (mu 1e4, lambda 5e4), bottom 10% pinned, 8 grasped surface
+             vertices pulled toward a tool tip on a random walk (sigma 2 mm),
+             10 substeps, 64x64 depth + RGB.
+* mixed    - n envs (1024) with tets log-uniform in [1k, 40k], ellipsoid or
+             box (50/50), distance + volume + Neo-Hookean, grasp in 50% of
+             envs, 10 substeps, refit + 64x64 depth.
+* large    - n envs (8192) of the 20k-tet ellipsoid, Neo-Hookean + volume +
+             grasp, 20 substeps x 2 iterations, 128x128 RGB-D.
+Every mesh is generated from its own seed; environments of the mixed workload
+each get a distinct mesh.
+"""
+from __future__ import annotations
+
+import dataclasses
+import multiprocessing as mp
+import os
+from typing import List, Optional
+
+import numpy as np
+
+from . import scenes as sc
+
+NH_PARAMS = dict(neoHookean=True, globalVolumeCompliance=1.0 / 5e4, globalDeviatoricCompliance=1.0 / 1e4)
+XPBD_DV = dict(distance="xpbd", volume="xpbd", distanceCompliance=1e-6, volumeComplianceXPBD=0.0)
+BASE = dict(simFrameRate=60, environmentGravity=(0.0, -9.81, 0.0), environmentVelocityDampening=0.1,
+            environmentGroundLevel=0.0, globalKsDrag=1.0)
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    meshes: list
+    env_mesh: list
+    model_kwargs: dict
+    resolution: int
+    camera_pos: tuple
+    look_at: tuple
+    tips: Optional[np.ndarray] = None
+    grasp: Optional[List[Optional[np.ndarray]]] = None
+    initial_velocity: Optional[np.ndarray] = None
+    walk_sigma: float = 0.002
+    rgb: bool = False
+
+    @property
+    def num_envs(self):
+        return len(self.env_mesh)
+
+    def totals(self):
+        T = sum(self.meshes[k].num_tets for k in self.env_mesh)
+        E = sum(self.meshes[k].num_edges for k in self.env_mesh)
+        V = sum(self.meshes[k].num_vertices for k in self.env_mesh)
+        F = sum(self.meshes[k].num_tris for k in self.env_mesh)
+        return dict(tets=T, edges=E, vertices=V, tris=F)
+
+
+def _gen(args):
+    shape, tets, seed = args
+    return sc.solid(shape, tets, seed=seed)
+
+
+def generate(specs, workers: Optional[int] = None):
+    """Generate meshes for (shape, tets, seed) specs, in parallel when safe
+    (a forked pool is used only before the GPU runtime is initialised)."""
+    workers = workers or min(16, os.cpu_count() or 1)
+    fork_ok = True
+    try:
+        import torch
+        fork_ok = not torch.cuda.is_initialized()
+    except Exception:
+        pass
+    if workers > 1 and fork_ok and len(specs) > 8:
+        with mp.get_context("fork").Pool(workers) as pool:
+            return pool.map(_gen, specs, chunksize=4)
+    return [_gen(s) for s in specs]
+
+
+def _tool(mesh, rng_offset=0.02, k=8):
+    tip = sc.top_point(mesh) + np.array([0.0, rng_offset, 0.0], np.float32)
+    return np.repeat(tip[None], 4, axis=0).astype(np.float32), sc.nearest_surface_vertices(mesh, tip, k)
+
+
+def cube(seed: int = 0) -> Workload:
+    mesh = sc.cube_grid(10)
+    rng = np.random.default_rng(seed)
+    v0 = rng.uniform(-0.01, 0.01, size=(mesh.num_vertices, 3)).astype(np.float32)
+    kw = dict(BASE, simSubsteps=10, simConstraintsSteps=1, **XPBD_DV)
+    return Workload("cube-1env-3645tets", [mesh], [0], kw, 128, (2.2, 1.8, 2.6), (0.5, 0.45, 0.5),
+                    tips=np.zeros((1, 4, 3), np.float32), initial_velocity=v0)
+
+
+def ellipsoids(n_env: int = 64, tets: int = 20000, seed: int = 0, substeps=10, iterations=1, resolution=64,
+               volume: bool = False, shared: bool = False) -> Workload:
+    specs = [("ellipsoid", tets, seed + (0 if shared else i)) for i in range(1 if shared else n_env)]
+    meshes = generate(specs)
+    tips, grasp = [], []
+    for i in range(n_env):
+        t, g = _tool(meshes[0 if shared else i])
+        tips.append(t)
+        grasp.append(g)
+    kw = dict(BASE, simSubsteps=substeps, simConstraintsSteps=iterations, distance=None, volume=None, **NH_PARAMS)
+    if volume:
+        kw.update(volume="xpbd", volumeComplianceXPBD=0.0)
+    env_mesh = [0] * n_env if shared else list(range(n_env))
+    return Workload(f"ellipsoid-{n_env}env-{meshes[0].num_tets}tets", meshes, env_mesh, kw, resolution,
+                    (0.0, 1.6, 5.5), (0.0, 0.45, 0.0), tips=np.stack(tips), grasp=grasp, rgb=True)
+
+
+def mixed(n_env: int = 1024, lo: int = 1000, hi: int = 40000, seed: int = 0, substeps=10, iterations=1,
+          resolution=64) -> Workload:
+    sizes, grasp_on = sc.mixed_sizes(n_env, lo, hi, seed)
+    specs = [(shape, t, seed * 100003 + i) for i, (shape, t) in enumerate(sizes)]
+    meshes = generate(specs)
+    tips, grasp = [], []
+    for i, m in enumerate(meshes):
+        t, g = _tool(m)
+        tips.append(t)
+        grasp.append(g if grasp_on[i] else None)
+    kw = dict(BASE, simSubsteps=substeps, simConstraintsSteps=iterations, **XPBD_DV, **NH_PARAMS)
+    return Workload(f"mixed-{n_env}env-logU[{lo},{hi}]tets", meshes, list(range(n_env)), kw, resolution,
+                    (0.0, 1.6, 5.5), (0.0, 0.45, 0.0), tips=np.stack(tips), grasp=grasp)
+
+
+def large(n_env: int = 8192, tets: int = 20000, seed: int = 0) -> Workload:
+    w = ellipsoids(n_env, tets, seed, substeps=20, iterations=2, resolution=128, volume=True, shared=True)
+    w.name = f"ellipsoid-{n_env}env-{w.meshes[0].num_tets}tets-S20I2"
+    return w
+
+
+def build_model(pkg, w: Workload, device="cuda:0"):
+    """SimModelDO for a workload, with the grasp weights set."""
+    model = pkg.SimModelDO(w.meshes, w.env_mesh, device=device, tips=w.tips, initialVelocity=w.initial_velocity,
+                           **w.model_kwargs)
+    if w.grasp is not None:
+        import torch
+        ids = [np.asarray(g, np.int64) + int(model.env_vbase[i]) for i, g in enumerate(w.grasp) if g is not None]
+        if ids:
+            model.drag[torch.as_tensor(np.concatenate(ids), device=model.device)] = 1.0
+    return model

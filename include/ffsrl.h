@@ -1,0 +1,205 @@
+/*
+ * ffsrl.h - C ABI of the MI355X (gfx950) implementation of FF-SRL's per-step
+ * simulation path: XPBD/PBD predict and integrate, distance / volume /
+ * Neo-Hookean tetrahedral constraint projection with the grasp constraint,
+ * triangle-BVH refit and the per-environment ray-cast renderer.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success, a hipError_t value (> 0) for a
+ *     HIP failure, or a negative FFSRL_E_* code; ffsrl_last_error() gives the
+ *     message (thread-local).  No C++ exception crosses this boundary.
+ *   - Device pointers are non-owning; the caller (the Python layer, through
+ *     torch) owns every buffer.  `stream` is a hipStream_t passed as void*;
+ *     all work is stream-ordered on it and nothing synchronises the host,
+ *     so every entry point except ffsrl_bvh_build may be captured into a HIP
+ *     graph.
+ *   - Positions are float4 (x, y, z, inverse mass); velocities float4 (vx, vy,
+ *     vz, 0).  Topology uses mesh-local vertex ids; an environment adds its
+ *     vertex base.  Environments that share a mesh share its topology arrays.
+ */
+#ifndef FFSRL_H
+#define FFSRL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFSRL_ABI_VERSION 1
+
+#define FFSRL_E_ARG (-1)      /* invalid argument */
+#define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
+#define FFSRL_E_STATE (-3)    /* call made in the wrong state */
+
+/* Constraint modes (per constraint family). */
+#define FFSRL_OFF 0
+#define FFSRL_PBD 1   /* stiffness form: distanceConstraints / volumeConstraints */
+#define FFSRL_XPBD 2  /* compliance form: *ConstraintsXPBD */
+
+/* One environment of the CSR environment table (64 bytes, int32 fields).
+ * Ranges index the shared topology arrays of ffsrl_batch; state ranges index
+ * the per-environment state arrays. */
+typedef struct ffsrl_env_desc {
+  int32_t vbase, nv;       /* state vertices [vbase, vbase+nv) */
+  int32_t ebase, ne;       /* edges of the env's mesh */
+  int32_t tbase, nt;       /* tets of the env's mesh */
+  int32_t fbase, nf;       /* render triangles of the mesh, BVH leaf order */
+  int32_t nodebase, nn;    /* BVH topology nodes of the mesh */
+  int32_t boundbase;       /* first BVH bound (state) of this env */
+  int32_t lvlbase, nlvl;   /* level_off[lvlbase .. lvlbase+nlvl] (levels deepest first) */
+  int32_t lam_ebase;       /* first edge lambda of this env (XPBD, iterations > 1) */
+  int32_t lam_tbase;       /* first tet lambda of this env */
+  int32_t flags;           /* reserved */
+} ffsrl_env_desc;
+
+/* Device arrays of a batch of environments. */
+typedef struct ffsrl_batch {
+  int32_t n_env;
+  int32_t max_nv;                 /* largest nv over envs (sizes LDS) */
+  int64_t n_tet_total;            /* length of the tet arrays (plane stride of tet_inv_rest) */
+  const ffsrl_env_desc* envs;     /* [n_env] */
+  /* topology (shared per mesh) */
+  const int32_t* edges;           /* int2 [E]: mesh-local (a, b) */
+  const float* edge_rest;         /* [E] */
+  const int32_t* tets;            /* int4 [T]: mesh-local (a, b, c, d) */
+  const float* tet_rest_volume;   /* [T] */
+  const float* tet_inv_rest;      /* [9][T] planes of the row-major inverse rest shape matrix */
+  const int32_t* tris;            /* int4 [F]: mesh-local (a, b, c, original face id), leaf order */
+  const int32_t* node_left;       /* [N] left child (mesh-local node id) or -1 for a leaf */
+  const int32_t* node_first;      /* [N] first leaf-order triangle of a leaf */
+  const int32_t* node_count;      /* [N] triangle count of a leaf, 0 for inner nodes */
+  const int32_t* level_nodes;     /* [N] node ids grouped by level */
+  const int32_t* level_off;       /* per env lvlbase: offsets into level_nodes (relative to nodebase) */
+  /* state (per environment) */
+  float* x;                       /* float4 [V] positions + inverse mass */
+  float* p;                       /* float4 [V] predicted positions + inverse mass */
+  float* v;                       /* float4 [V] velocities */
+  float* drag;                    /* [V] grasp weight (activeDragConstraint) */
+  float* lam_e;                   /* [sum E] XPBD edge lambdas (NULL when not carried) */
+  float* lam_t;                   /* [sum T] XPBD tet lambdas (NULL when not carried) */
+  float* bvh_lo;                  /* float4 [sum N]: aabb min, .w = int bits (first tri or left child) */
+  float* bvh_hi;                  /* float4 [sum N]: aabb max, .w = int bits (triangle count) */
+  float* tips;                    /* [n_env*4*3] instrument points; grasp pulls to point 2 */
+  /* plain-form scratch (per-kernel launches with global atomics) */
+  float* dp;                      /* [3][V] planes (NULL unless the plain form is used) */
+  int32_t* cnt;                   /* [V] */
+} ffsrl_batch;
+
+/* Scalars of SimModelDO handed to the kernels (modelDO.py:774-830). */
+typedef struct ffsrl_params {
+  int32_t substeps, iterations;
+  float dt;                       /* substep dt = (1/fps)/substeps */
+  float gravity[3];
+  float damping;                  /* velocityDampening */
+  float ground;                   /* groundLevel */
+  int32_t dist_mode;              /* FFSRL_OFF / PBD / XPBD */
+  float dist_k;                   /* kS (PBD) or alpha~ = compliance/dt^2 (XPBD) */
+  int32_t vol_mode;
+  float vol_k;
+  int32_t nh_on;                  /* Neo-Hookean on/off */
+  float nh_vol_c, nh_dev_c;       /* hydrostatic / deviatoric compliance (1/lambda, 1/mu) */
+  float drag_ks;                  /* globalKsDrag */
+  int32_t lambda_legacy;          /* 1: zero lambdas only when iterations > 1 */
+} ffsrl_params;
+
+/* Camera of the ray-cast renderer: one origin and a precomputed direction per
+ * ray of the (2H x 2W) supersampled ray grid (render.py:299-311). */
+typedef struct ffsrl_camera {
+  float origin[3];
+  const float* dirs;              /* float4 [2H*2W] (device), row-major ray order */
+  int32_t height, width;          /* output pixels per env; width % 4 == 0, height % 4 == 0 */
+  int32_t mode;                   /* 4 depth, 3 vertex colour, 2 normal, 0 grayscale, 1 triangle colour */
+  float light_dir[3];
+  float light_intensity;
+} ffsrl_camera;
+
+int ffsrl_abi_version(void);
+const char* ffsrl_last_error(void);
+
+/* ---- batched form ------------------------------------------------------ */
+
+/* One frame of every environment in one launch: S substeps x I iterations of
+ * predict, projection (volume, distance, Neo-Hookean, Jacobi-averaged), grasp,
+ * ground clamp and integrate, followed by the BVH refit when refit != 0.
+ * Replaces SimIntegratorDO.stepModel (integrator.py:1640-1744) followed by
+ * SimBVH.refitBVH (bvh.py:442-451).  `order` (device, may be NULL) gives the
+ * dispatch order of environments (largest first balances mixed sizes).
+ * block = threads per environment (256, 512 or 1024). */
+int ffsrl_frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* order, int32_t block,
+                int32_t refit, void* stream);
+
+/* Largest nv that ffsrl_frame can hold in LDS (dP + counters). */
+int ffsrl_frame_max_vertices(void);
+
+/* Bottom-up refit of every environment's BVH.  Replaces refitBVHLoop
+ * (bvh.py:419-440) / refitBVHLevel (bvh.py:30-104). */
+int ffsrl_bvh_refit(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, void* stream);
+
+/* Ray-cast render of environments [env_begin, env_end) with the 2x2 box
+ * filter.  depth: [n][H][W] (mode 4) ; rgb: [n][H][W][3] (other modes);
+ * hit / thit (optional, may be NULL): per-ray face ids and t, [n][2H][2W].
+ * Replaces drawSimBVHKernelManyEnvs + meshQueryRayManyEnvs + downsample_kernel
+ * (render.py:276-498, 699-724) launched by renderFunctionNew (render.py:975-1022).
+ * Output env index = env - env_begin. */
+int ffsrl_render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t env_begin, int32_t env_end,
+                 float* depth, float* rgb, const float* normals, const float* vcolor,
+                 const float* tcolor, int32_t* hit, float* thit, void* stream);
+
+/* Area-weighted vertex normals (float4 [V]) of the render mesh.  Replaces
+ * zeroVec3Array + vertex_normal_sum_kernel + normalize_kernel (render.py:81-121). */
+int ffsrl_vertex_normals(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, float* normals,
+                         void* stream);
+
+/* Per-env tool update: adds delta[env] (3 floats) to the 4 instrument points
+ * of each env and clamps them to [lo, hi].  Stands in for
+ * applyCartesianActionsInWorkspace (modelDO.py:1385-1404) with the tool tip
+ * driven directly (no remote-centre-of-motion kinematics). */
+int ffsrl_move_tools(const ffsrl_batch* b, const float* delta, const float* lo, const float* hi,
+                     void* stream);
+
+/* Grasp selection: drag[v] = 1 if |x[v] - tip[env*4+3]| < radius for envs with
+ * state[env*3] == 1 and update[env*3] != 0, else 0 (updateClampsDragKernel,
+ * modelDO.py:247-279).  state/update are int32 [n_env*3] device arrays. */
+int ffsrl_grasp_select(const ffsrl_batch* b, float radius, const int32_t* state,
+                       const int32_t* update, void* stream);
+
+/* ---- plain form: one launch per kernel per environment ------------------ */
+/* The reference's per-kernel launch model (integrator.py:1644-1741), restricted
+ * to environments [env_begin, env_end) with one launch per environment. */
+int ffsrl_predict(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end, void* stream);
+int ffsrl_project_volume(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end, void* stream);
+int ffsrl_project_distance(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end, void* stream);
+int ffsrl_project_neohookean(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end, void* stream);
+int ffsrl_apply(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, void* stream);
+int ffsrl_grasp(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end, void* stream);
+int ffsrl_bounds_integrate(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end, void* stream);
+int ffsrl_zero_lambdas(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, void* stream);
+/* One launch per env and BVH level, deepest level first (refitBVHLoop). */
+int ffsrl_bvh_refit_plain(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, void* stream);
+
+/* remapAToBKernel (utils.py:32-42): dst[map[2i]] = src[map[2i+1]] for float4
+ * arrays, i < n. */
+int ffsrl_remap(float* dst, const float* src, const int32_t* map, int64_t n, void* stream);
+
+/* ---- host: BVH construction -------------------------------------------- */
+/* Binned-SAH build of one mesh (SimBVH.buildBVH / subdivide /
+ * findBestSplitPlaneBinned, bvh.py:281-518), host memory.  Outputs hold at
+ * least 2*nf entries (tri_map: nf); the node count and deepest level are
+ * returned through num_nodes and max_level. */
+int ffsrl_bvh_build(const float* verts, int32_t nv, const int32_t* faces, int32_t nf, int32_t bins,
+                    int32_t* left, int32_t* first, int32_t* count, int32_t* level, int32_t* tri_map,
+                    int32_t* num_nodes, int32_t* max_level);
+
+/* ---- HIP graphs --------------------------------------------------------- */
+/* Capture everything issued on `stream` between begin and end into a graph
+ * (replaces wp.capture_begin / capture_end / capture_launch). */
+int ffsrl_graph_begin(void* stream);
+int ffsrl_graph_end(void* stream, void** graph_exec);
+int ffsrl_graph_launch(void* graph_exec, void* stream);
+int ffsrl_graph_destroy(void* graph_exec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,513 @@
+/*
+ * TEST INFRASTRUCTURE ONLY - the oracle.  CPU restatement of the reference's
+ * per-step kernels.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker / CPU baseline.
+ *
+ * This header is included twice by oracle.c, once with REAL=float (SFX=_f32)
+ * and once with REAL=double (SFX=_f64).  Every kernel runs its "threads" in
+ * serial tid order, so the float atomics of the reference accumulate in tid
+ * order (Warp's CPU execution model).  Arrays use the reference's layout:
+ * AoS vec3 (x,y,z per vertex), interleaved int32 index arrays, global ids.
+ *
+ * Builtin arithmetic follows warp-lang 0.10.1 as restated here (Warp is absent
+ * from this machine, so these choices are "parity unpinned" against Warp
+ * itself and pinned instead by closed forms and brute force in tests/):
+ *   length(v) = sqrt((x*x + y*y) + z*z)
+ *   normalize(v) = v / length(v) component-wise if length > 0 else 0
+ *   min(a,b) = a < b ? a : b,  max(a,b) = a > b ? a : b
+ *   mat33(c0,c1,c2) takes COLUMNS; M[i] is row i; mul() sums k = 0,1,2
+ *   determinant(M) = row0 . (row1 x row2)
+ */
+#define CAT_(a, b) a##b
+#define CAT(a, b) CAT_(a, b)
+#define FN(name) CAT(name, SFX)
+#define R(x) ((REAL)(x))
+
+typedef struct { REAL x, y, z; } FN(vec3_t);
+#define V3 FN(vec3_t)
+
+static inline V3 FN(v3)(REAL x, REAL y, REAL z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
+static inline V3 FN(ld)(const REAL* a, long i) { return FN(v3)(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+static inline void FN(st)(REAL* a, long i, V3 v) { a[3 * i] = v.x; a[3 * i + 1] = v.y; a[3 * i + 2] = v.z; }
+static inline V3 FN(add)(V3 a, V3 b) { return FN(v3)(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V3 FN(sub)(V3 a, V3 b) { return FN(v3)(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V3 FN(neg)(V3 a) { return FN(v3)(-a.x, -a.y, -a.z); }
+static inline V3 FN(mul)(V3 a, REAL s) { return FN(v3)(a.x * s, a.y * s, a.z * s); }
+static inline V3 FN(divs)(V3 a, REAL s) { return FN(v3)(a.x / s, a.y / s, a.z / s); }
+static inline REAL FN(dot)(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 FN(cross)(V3 a, V3 b) {
+  return FN(v3)(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline REAL FN(sqrt_)(REAL x) { return (REAL)SQRT(x); }
+static inline REAL FN(length)(V3 a) { return FN(sqrt_)(FN(dot)(a, a)); }
+static inline V3 FN(normalize)(V3 a) {
+  REAL l = FN(length)(a);
+  if (l > R(0)) return FN(divs)(a, l);
+  return FN(v3)(R(0), R(0), R(0));
+}
+static inline REAL FN(fmin_)(REAL a, REAL b) { return a < b ? a : b; }
+static inline REAL FN(fmax_)(REAL a, REAL b) { return a > b ? a : b; }
+static inline V3 FN(vmin)(V3 a, V3 b) { return FN(v3)(FN(fmin_)(a.x, b.x), FN(fmin_)(a.y, b.y), FN(fmin_)(a.z, b.z)); }
+static inline V3 FN(vmax)(V3 a, V3 b) { return FN(v3)(FN(fmax_)(a.x, b.x), FN(fmax_)(a.y, b.y), FN(fmax_)(a.z, b.z)); }
+static inline REAL FN(absr)(REAL a) { return a < R(0) ? -a : a; }
+
+/* FLOAT_EPSILON = sys.float_info.epsilon (integrator.py:8). */
+#define FEPS R(2.220446049250313e-16)
+
+/* gravity (integrator.py:40-63) */
+void FN(oracle_gravity)(long n, const REAL* vertex, REAL* pred, const REAL* vel, const REAL* invMass,
+                        const REAL* g, REAL damp, REAL dt) {
+  V3 G = FN(v3)(g[0], g[1], g[2]);
+  REAL d = R(1.0) - damp;
+  for (long tid = 0; tid < n; ++tid) {
+    V3 x = FN(ld)(vertex, tid), v = FN(ld)(vel, tid);
+    REAL w = invMass[tid];
+    v = FN(add)(v, FN(mul)(FN(mul)(FN(mul)(G, w), dt), d));
+    if (w < FEPS) v = FN(v3)(R(0), R(0), R(0));
+    FN(st)(pred, tid, FN(add)(x, FN(mul)(v, dt)));
+  }
+}
+
+/* distanceConstraints, PBD form (integrator.py:65-109) */
+void FN(oracle_distance_pbd)(long nE, const REAL* pred, REAL* dP, int* cnt, const int* edge,
+                             const REAL* rest, const REAL* invMass, const REAL* active, REAL kS) {
+  for (long tid = 0; tid < nE; ++tid) {
+    REAL act = active ? active[tid] : R(1);
+    if (act == R(0)) continue;
+    int a = edge[2 * tid], b = edge[2 * tid + 1];
+    V3 pa = FN(ld)(pred, a), pb = FN(ld)(pred, b);
+    REAL r = rest[tid];
+    if (r < FEPS) continue;
+    V3 dir = FN(sub)(pb, pa);
+    REAL len = FN(length)(dir);
+    if (len < FEPS) continue;
+    REAL w = invMass[a] + invMass[b];
+    if (w < FEPS) continue;
+    REAL C = len - r;
+    V3 e = FN(divs)(FN(mul)(FN(mul)(FN(mul)(FN(divs)(dir, len), -C), kS), act), w);
+    FN(st)(dP, a, FN(sub)(FN(ld)(dP, a), FN(mul)(e, invMass[a])));
+    FN(st)(dP, b, FN(add)(FN(ld)(dP, b), FN(mul)(e, invMass[b])));
+    cnt[a] += 1; cnt[b] += 1;
+  }
+}
+
+/* distanceConstraintsXPBD (integrator.py:191-240); alpha is compliance/dt^2 */
+void FN(oracle_distance_xpbd)(long nE, const REAL* pred, REAL* dP, REAL* lambdas, int* cnt,
+                              const int* edge, const REAL* rest, const REAL* invMass,
+                              const REAL* active, REAL alpha) {
+  for (long tid = 0; tid < nE; ++tid) {
+    if (active && active[tid] == R(0)) continue;
+    int a = edge[2 * tid], b = edge[2 * tid + 1];
+    V3 pa = FN(ld)(pred, a), pb = FN(ld)(pred, b);
+    REAL r = rest[tid];
+    if (r < FEPS) continue;
+    REAL lam = lambdas[tid];
+    V3 dir = FN(sub)(pb, pa);
+    REAL len = FN(length)(dir);
+    if (len < FEPS) continue;
+    REAL w = invMass[a] + invMass[b];
+    if (w < FEPS) continue;
+    REAL C = len - r;
+    REAL dl = -(C + alpha * lam) / (w + alpha);
+    lambdas[tid] = lam + dl;
+    V3 e = FN(mul)(FN(normalize)(dir), dl);
+    FN(st)(dP, a, FN(sub)(FN(ld)(dP, a), FN(mul)(e, invMass[a])));
+    FN(st)(dP, b, FN(add)(FN(ld)(dP, b), FN(mul)(e, invMass[b])));
+    cnt[a] += 1; cnt[b] += 1;
+  }
+}
+
+/* shared part of the two volume kernels: q vectors and w */
+static inline void FN(vol_terms)(const REAL* pred, const REAL* invMass, const int* t, V3 q[4], REAL m[4],
+                                 REAL* six_vol, REAL* w) {
+  V3 A = FN(ld)(pred, t[0]), B = FN(ld)(pred, t[1]), C = FN(ld)(pred, t[2]), D = FN(ld)(pred, t[3]);
+  V3 p1 = FN(sub)(B, A), p2 = FN(sub)(C, A), p3 = FN(sub)(D, A);
+  q[2] = FN(cross)(p3, p1);
+  q[1] = FN(cross)(p2, p3);
+  q[3] = FN(cross)(p1, p2);
+  q[0] = FN(sub)(FN(sub)(FN(neg)(q[1]), q[2]), q[3]);
+  for (int i = 0; i < 4; ++i) m[i] = invMass[t[i]];
+  *six_vol = FN(dot)(FN(cross)(p1, p2), p3);
+  *w = m[0] * FN(dot)(q[0], q[0]) + m[1] * FN(dot)(q[1], q[1]) + m[2] * FN(dot)(q[2], q[2]) +
+       m[3] * FN(dot)(q[3], q[3]);
+}
+
+static inline void FN(scatter4)(REAL* dP, int* cnt, const int* t, const V3 d[4]) {
+  for (int i = 0; i < 4; ++i) FN(st)(dP, t[i], FN(add)(FN(ld)(dP, t[i]), d[i]));
+  for (int i = 0; i < 4; ++i) cnt[t[i]] += 1;
+}
+
+/* volumeConstraints, PBD form (integrator.py:444-504) */
+void FN(oracle_volume_pbd)(long nT, const REAL* pred, REAL* dP, int* cnt, const int* tet,
+                           const REAL* restVol, const REAL* invMass, const REAL* active, REAL kS) {
+  for (long tid = 0; tid < nT; ++tid) {
+    REAL act = active ? active[tid] : R(1);
+    if (!(act == R(1))) continue;
+    const int* t = tet + 4 * tid;
+    V3 q[4], d[4]; REAL m[4], sv, w;
+    FN(vol_terms)(pred, invMass, t, q, m, &sv, &w);
+    REAL volume = FN(absr)(sv) / R(6.0);
+    if (FN(absr)(w) < FEPS) continue;
+    REAL C = restVol[tid] - volume;
+    REAL dl = -kS * act * C / w;
+    for (int i = 0; i < 4; ++i) d[i] = FN(mul)(FN(mul)(q[i], dl), m[i]);
+    FN(scatter4)(dP, cnt, t, d);
+  }
+}
+
+/* volumeConstraintsXPBD (integrator.py:507-576) */
+void FN(oracle_volume_xpbd)(long nT, const REAL* pred, REAL* dP, REAL* lambdas, int* cnt,
+                            const int* tet, const REAL* restVol, const REAL* invMass,
+                            const REAL* active, REAL alpha) {
+  for (long tid = 0; tid < nT; ++tid) {
+    if (active && !(active[tid] == R(1))) continue;
+    const int* t = tet + 4 * tid;
+    V3 q[4], d[4]; REAL m[4], sv, w;
+    REAL lam = lambdas[tid];
+    FN(vol_terms)(pred, invMass, t, q, m, &sv, &w);
+    REAL volume = sv / R(6.0);
+    if (FN(absr)(w) < FEPS) continue;
+    REAL C = volume - restVol[tid];
+    REAL dl = -(C + alpha * lam) / (w + alpha);
+    lambdas[tid] = lam + dl;
+    for (int i = 0; i < 4; ++i) d[i] = FN(mul)(FN(mul)(q[i], dl), m[i]);
+    FN(scatter4)(dP, cnt, t, d);
+  }
+}
+
+/* 3x3 helpers, row-major m[3*i+j] */
+static inline void FN(mat_cols)(REAL* m, V3 c0, V3 c1, V3 c2) {
+  m[0] = c0.x; m[1] = c1.x; m[2] = c2.x;
+  m[3] = c0.y; m[4] = c1.y; m[5] = c2.y;
+  m[6] = c0.z; m[7] = c1.z; m[8] = c2.z;
+}
+static inline void FN(mat_mul)(REAL* o, const REAL* a, const REAL* b) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      REAL s = R(0);
+      for (int k = 0; k < 3; ++k) s += a[3 * i + k] * b[3 * k + j];
+      o[3 * i + j] = s;
+    }
+}
+static inline void FN(mat_T)(REAL* o, const REAL* a) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) o[3 * j + i] = a[3 * i + j];
+}
+static inline V3 FN(row)(const REAL* m, int i) { return FN(v3)(m[3 * i], m[3 * i + 1], m[3 * i + 2]); }
+
+/* Constraint values and gradient rows of neoHookeanConstraints
+ * (integrator.py:601-683) for the tet positions A..D:
+ *   hydrostatic C_H = det F - 1 - invD/invV with rows of transpose(mat33(F1xF2,
+ *   F2xF0, F0xF1) * M^T); deviatoric C_D = |F|_F with rows of
+ *   transpose(F * M^T * (1/|F|_F)); g0 = -g1 - g2 - g3 in both. */
+typedef struct { REAL CH, CD; V3 gH[4], gD[4]; } FN(nh_terms_t);
+
+static FN(nh_terms_t) FN(nh_terms)(V3 A, V3 B, V3 C3, V3 D, const REAL* M, REAL volC, REAL devC) {
+  FN(nh_terms_t) o;
+  V3 p1 = FN(sub)(B, A), p2 = FN(sub)(C3, A), p3 = FN(sub)(D, A);
+  REAL Ds[9], F[9], dF[9], MT[9], g[9], gt[9];
+  FN(mat_cols)(Ds, p1, p2, p3);
+  FN(mat_mul)(F, Ds, M);
+  V3 F0 = FN(row)(F, 0), F1 = FN(row)(F, 1), F2 = FN(row)(F, 2);
+  V3 dF1 = FN(cross)(F1, F2), dF2 = FN(cross)(F2, F0), dF3 = FN(cross)(F0, F1);
+  FN(mat_cols)(dF, dF1, dF2, dF3);
+  FN(mat_T)(MT, M);
+  FN(mat_mul)(g, dF, MT);
+  FN(mat_T)(gt, g);
+  o.gH[1] = FN(row)(gt, 0); o.gH[2] = FN(row)(gt, 1); o.gH[3] = FN(row)(gt, 2);
+  o.gH[0] = FN(sub)(FN(sub)(FN(neg)(o.gH[1]), o.gH[2]), o.gH[3]);
+  REAL volume = FN(dot)(F0, FN(cross)(F1, F2));
+  REAL invV = R(1.0) / volC, invD = R(1.0) / devC;
+  o.CH = volume - R(1.0) - invD / invV;
+  REAL l0 = FN(length)(F0), l1 = FN(length)(F1), l2 = FN(length)(F2);
+  REAL rs = FN(sqrt_)(l0 * l0 + l1 * l1 + l2 * l2);
+  REAL rsi = R(1.0) / rs;
+  o.CD = rs;
+  FN(mat_mul)(g, F, MT);
+  for (int i = 0; i < 9; ++i) g[i] = g[i] * rsi;
+  FN(mat_T)(gt, g);
+  o.gD[1] = FN(row)(gt, 0); o.gD[2] = FN(row)(gt, 1); o.gD[3] = FN(row)(gt, 2);
+  o.gD[0] = FN(sub)(FN(sub)(FN(neg)(o.gD[1]), o.gD[2]), o.gD[3]);
+  return o;
+}
+
+static inline REAL FN(nh_w)(const V3 g[4], const REAL m[4]) {
+  REAL l, w = R(0);
+  for (int i = 0; i < 4; ++i) { l = FN(length)(g[i]); w += l * l * m[i]; }
+  return w;
+}
+
+/* Debug export of nh_terms: out = [C_H, gH (4x3), C_D, gD (4x3)]. */
+void FN(oracle_nh_terms)(const REAL* pos, const REAL* M, REAL volC, REAL devC, REAL* out) {
+  FN(nh_terms_t) o = FN(nh_terms)(FN(ld)(pos, 0), FN(ld)(pos, 1), FN(ld)(pos, 2), FN(ld)(pos, 3), M, volC, devC);
+  out[0] = o.CH;
+  for (int i = 0; i < 4; ++i) FN(st)(out + 1, i, o.gH[i]);
+  out[13] = o.CD;
+  for (int i = 0; i < 4; ++i) FN(st)(out + 14, i, o.gD[i]);
+}
+
+/* neoHookeanConstraints (integrator.py:578-698).  lambda is read and never
+ * stored, and no alpha*lambda term enters dlambda, exactly as the reference. */
+void FN(oracle_neohookean)(long nT, const REAL* pred, REAL* dP, int* cnt, const int* tet,
+                           const REAL* restVol, const REAL* invMass, const REAL* invRest,
+                           REAL volC, REAL devC, REAL dT) {
+  for (long tid = 0; tid < nT; ++tid) {
+    const int* t = tet + 4 * tid;
+    REAL rv = restVol[tid];
+    REAL m[4] = {invMass[t[0]], invMass[t[1]], invMass[t[2]], invMass[t[3]]};
+    FN(nh_terms_t) o = FN(nh_terms)(FN(ld)(pred, t[0]), FN(ld)(pred, t[1]), FN(ld)(pred, t[2]),
+                                    FN(ld)(pred, t[3]), invRest + 9 * tid, volC, devC);
+    REAL w = FN(nh_w)(o.gH, m);
+    if (w < FEPS) continue;
+    REAL alpha = volC / dT / dT / rv;
+    REAL dl = -o.CH / (w + alpha);
+    V3 d[4];
+    for (int i = 0; i < 4; ++i) d[i] = FN(mul)(FN(mul)(o.gH[i], dl), m[i]);
+    w = FN(nh_w)(o.gD, m);
+    if (w < FEPS) continue;
+    alpha = devC / dT / dT / rv;
+    dl = -o.CD / (w + alpha);
+    for (int i = 0; i < 4; ++i) d[i] = FN(add)(d[i], FN(mul)(FN(mul)(o.gD[i], dl), m[i]));
+    FN(scatter4)(dP, cnt, t, d);
+  }
+}
+
+/* applyConstraints (integrator.py:946-960) */
+void FN(oracle_apply)(long n, REAL* pred, REAL* dP, int* cnt) {
+  for (long tid = 0; tid < n; ++tid) {
+    if (cnt[tid] > 0) {
+      V3 d = FN(ld)(dP, tid);
+      REAL N = (REAL)cnt[tid];
+      FN(st)(pred, tid, FN(add)(FN(ld)(pred, tid), FN(v3)(d.x / N, d.y / N, d.z / N)));
+    }
+    FN(st)(dP, tid, FN(v3)(R(0), R(0), R(0)));
+    cnt[tid] = 0;
+  }
+}
+
+/* dragConstraintsDO (integrator.py:412-442): pull toward tip[env*4+2] */
+void FN(oracle_drag)(long n, REAL* pred, const REAL* drag, const int* vertexToEnv, const REAL* tips,
+                     REAL ks) {
+  for (long tid = 0; tid < n; ++tid) {
+    REAL dr = drag[tid];
+    if (dr == R(0)) continue;
+    int env = vertexToEnv[tid];
+    V3 p = FN(ld)(pred, tid);
+    V3 tip = FN(ld)(tips, (long)env * 4 + 2);
+    V3 dir = FN(sub)(p, tip);
+    REAL len = FN(length)(dir);
+    if (len < FEPS) continue;
+    V3 e = FN(divs)(FN(mul)(FN(mul)(FN(mul)(FN(normalize)(dir), len), dr), ks), R(2.0));
+    FN(st)(pred, tid, FN(sub)(p, e));
+  }
+}
+
+/* bounds (integrator.py:12-21) */
+void FN(oracle_bounds)(long n, REAL* pred, REAL ground) {
+  for (long tid = 0; tid < n; ++tid)
+    if (pred[3 * tid + 1] < ground) pred[3 * tid + 1] = ground;
+}
+
+/* PBDStep (integrator.py:23-38) */
+void FN(oracle_pbd_step)(long n, REAL* vertex, const REAL* pred, REAL* vel, REAL dT) {
+  REAL inv = R(1.0) / dT;
+  for (long tid = 0; tid < n; ++tid) {
+    V3 x = FN(ld)(vertex, tid), p = FN(ld)(pred, tid);
+    FN(st)(vel, tid, FN(mul)(FN(sub)(p, x), inv));
+    FN(st)(vertex, tid, p);
+  }
+}
+
+/* updateClampsDragKernel (modelDO.py:247-279): radius selection around tip[env*4+3] */
+void FN(oracle_grasp_select)(long n, const REAL* pos, const int* vertexToEnv, const REAL* tips,
+                             REAL* drag, REAL radius, const int* state, const int* update) {
+  for (long tid = 0; tid < n; ++tid) {
+    int env = vertexToEnv[tid];
+    if (update[env * 3] == 0) continue;
+    if (state[env * 3] == 1) {
+      REAL len = FN(length)(FN(sub)(FN(ld)(pos, tid), FN(ld)(tips, (long)env * 4 + 3)));
+      drag[tid] = len < radius ? R(1.0) : R(0.0);
+    } else {
+      drag[tid] = R(0.0);
+    }
+  }
+}
+
+/* Constraint set of one step (the arguments SimModelDO hands to the kernels). */
+typedef struct {
+  long nV, nE, nT;
+  int substeps, iterations;
+  REAL dt;            /* substep dt = 1/(fps*substeps) */
+  REAL gravity[3], damping, ground;
+  int dist_mode;      /* 0 off, 1 PBD (kS), 2 XPBD (alpha~) */
+  int vol_mode;       /* 0 off, 1 PBD (kS), 2 XPBD (alpha~) */
+  int nh_on;          /* 1: neoHookeanConstraints */
+  int lambda_legacy;  /* 1: zero lambdas only when iterations > 1 (legacy stepper) */
+  REAL dist_k, vol_k; /* kS or alpha~ depending on mode */
+  REAL nh_vol_c, nh_dev_c;
+  REAL drag_ks;
+} FN(oracle_step_cfg);
+
+/* SimIntegratorDO.stepModel (integrator.py:1640-1744), one frame.  Lambdas are
+ * zeroed at the start of each substep (or as the legacy stepper does,
+ * integrator.py:2154-2173). */
+void FN(oracle_step_frame)(const FN(oracle_step_cfg)* c, REAL* vertex, REAL* pred, REAL* vel,
+                           const REAL* invMass, REAL* dP, int* cnt, const int* edge,
+                           const REAL* edgeRest, const int* tet, const REAL* restVol,
+                           const REAL* invRest, REAL* lamE, REAL* lamT, const REAL* drag,
+                           const int* vertexToEnv, const REAL* tips) {
+  for (int s = 0; s < c->substeps; ++s) {
+    FN(oracle_gravity)(c->nV, vertex, pred, vel, invMass, c->gravity, c->damping, c->dt);
+    if (!c->lambda_legacy || c->iterations > 1) {
+      for (long i = 0; i < c->nE; ++i) lamE[i] = R(0);
+      for (long i = 0; i < c->nT; ++i) lamT[i] = R(0);
+    }
+    for (int it = 0; it < c->iterations; ++it) {
+      if (c->vol_mode == 1) FN(oracle_volume_pbd)(c->nT, pred, dP, cnt, tet, restVol, invMass, 0, c->vol_k);
+      if (c->vol_mode == 2) FN(oracle_volume_xpbd)(c->nT, pred, dP, lamT, cnt, tet, restVol, invMass, 0, c->vol_k);
+      if (c->dist_mode == 1) FN(oracle_distance_pbd)(c->nE, pred, dP, cnt, edge, edgeRest, invMass, 0, c->dist_k);
+      if (c->dist_mode == 2) FN(oracle_distance_xpbd)(c->nE, pred, dP, lamE, cnt, edge, edgeRest, invMass, 0, c->dist_k);
+      if (c->nh_on) FN(oracle_neohookean)(c->nT, pred, dP, cnt, tet, restVol, invMass, invRest, c->nh_vol_c, c->nh_dev_c, c->dt);
+      FN(oracle_apply)(c->nV, pred, dP, cnt);
+      if (drag) FN(oracle_drag)(c->nV, pred, drag, vertexToEnv, tips, c->drag_ks);
+    }
+    FN(oracle_bounds)(c->nV, pred, c->ground);
+    FN(oracle_pbd_step)(c->nV, vertex, pred, vel, c->dt);
+  }
+}
+
+/* refitBVHLevel + refitBVHLoop (bvh.py:30-104, 419-440) for one environment
+ * whose faces index its own vertex array (visFaceShift = 0). */
+void FN(oracle_refit)(int numNodes, int maxLevel, const int* leftNode, const int* firstTri,
+                      const int* triCount, const int* level, const int* triMap, const int* faces,
+                      const REAL* vertex, REAL* aabbMin, REAL* aabbMax) {
+  for (int L = maxLevel; L >= 0; --L) {
+    for (int node = 0; node < numNodes; ++node) {
+      if (level[node] != L) continue;
+      int left = leftNode[node];
+      if (left < 0) {
+        V3 mn = FN(v3)(R(1e6), R(1e6), R(1e6)), mx = FN(v3)(R(-1e6), R(-1e6), R(-1e6));
+        for (int k = 0; k < triCount[node]; ++k) {
+          int tri = triMap[firstTri[node] + k];
+          V3 a = FN(ld)(vertex, faces[3 * tri]), b = FN(ld)(vertex, faces[3 * tri + 1]),
+             c = FN(ld)(vertex, faces[3 * tri + 2]);
+          mn = FN(vmin)(a, FN(vmin)(b, FN(vmin)(c, mn)));
+          mx = FN(vmax)(a, FN(vmax)(b, FN(vmax)(c, mx)));
+        }
+        FN(st)(aabbMin, node, mn); FN(st)(aabbMax, node, mx);
+      } else {
+        int right = left + 1;
+        FN(st)(aabbMin, node, FN(vmin)(FN(ld)(aabbMin, left), FN(ld)(aabbMin, right)));
+        FN(st)(aabbMax, node, FN(vmax)(FN(ld)(aabbMax, left), FN(ld)(aabbMax, right)));
+      }
+    }
+  }
+}
+
+/* intersectAABB (render.py:588-609): returns hit and writes tmin */
+static inline int FN(isect_aabb)(V3 o, V3 d, V3 lo, V3 hi, REAL* tmin_out) {
+  REAL t1 = (lo.x - o.x) / d.x, t2 = (hi.x - o.x) / d.x;
+  REAL t3 = (lo.y - o.y) / d.y, t4 = (hi.y - o.y) / d.y;
+  REAL t5 = (lo.z - o.z) / d.z, t6 = (hi.z - o.z) / d.z;
+  REAL tmin = FN(fmax_)(FN(fmax_)(FN(fmin_)(t1, t2), FN(fmin_)(t3, t4)), FN(fmin_)(t5, t6));
+  REAL tmax = FN(fmin_)(FN(fmin_)(FN(fmax_)(t1, t2), FN(fmax_)(t3, t4)), FN(fmax_)(t5, t6));
+  if (tmax >= R(0) && tmax >= tmin) { *tmin_out = tmin; return 1; }
+  *tmin_out = R(0);
+  return 0;
+}
+
+/* MTTriRayIntersection (render.py:661-697): returns t, or -1 on a miss */
+static inline REAL FN(mt)(V3 a, V3 b, V3 c, V3 o, V3 d) {
+  V3 e1 = FN(sub)(b, a), e2 = FN(sub)(c, a);
+  V3 pv = FN(cross)(d, e2);
+  REAL det = FN(dot)(e1, pv);
+  if (FN(absr)(det) < R(0.0001)) return R(-1.0);
+  REAL inv = R(1.0) / det;
+  V3 tv = FN(sub)(o, a);
+  REAL u = FN(dot)(tv, pv) * inv;
+  if (u < R(0) || u > R(1)) return R(-1.0);
+  V3 qv = FN(cross)(tv, e1);
+  REAL v = FN(dot)(d, qv) * inv;
+  if (v < R(0) || u + v > R(1)) return R(-1.0);
+  return FN(dot)(e2, qv) * inv;
+}
+
+/* meshQueryRayManyEnvs (render.py:411-498) for one ray.  Returns the hit face
+ * (an index into faces) or -1, and its t.  *max_stack reports the deepest
+ * stack use (the reference's stack holds 32 entries). */
+int FN(oracle_ray)(V3 o, V3 d, const REAL* aabbMin, const REAL* aabbMax, const int* leftNode,
+                   const int* firstTri, const int* triCount, const int* triMap, const int* faces,
+                   const REAL* vertex, REAL* t_out, int* max_stack) {
+  int stack[256];
+  int count = 1, hit = -1;
+  stack[0] = 0;
+  REAL tMin = R(1e6);
+  while (count) {
+    int node = stack[count - 1];
+    count -= 1;
+    REAL tb;
+    int h = FN(isect_aabb)(o, d, FN(ld)(aabbMin, node), FN(ld)(aabbMax, node), &tb);
+    if (h && tb < tMin) {
+      int left = leftNode[node];
+      if (left == -1) {
+        for (int k = 0; k < triCount[node]; ++k) {
+          int tri = triMap[firstTri[node] + k];
+          V3 a = FN(ld)(vertex, faces[3 * tri]), b = FN(ld)(vertex, faces[3 * tri + 1]),
+             c = FN(ld)(vertex, faces[3 * tri + 2]);
+          REAL t = FN(mt)(a, b, c, o, d);
+          if (t < tMin && t > R(0)) { tMin = t; hit = tri; }
+        }
+      } else {
+        if (count + 2 > 256) return -2;
+        stack[count] = left;
+        stack[count + 1] = left + 1;
+        count += 2;
+        if (count > *max_stack) *max_stack = count;
+      }
+    }
+  }
+  if (tMin >= R(1e6)) { *t_out = R(0); return -1; }
+  *t_out = tMin;
+  return hit;
+}
+
+/* drawSimBVHKernelManyEnvs (render.py:276-409) depth mode (color = t/50 clamped
+ * to 1, miss = 0) for one environment, plus downsample_kernel (render.py:699-724)
+ * with a 2x2 box.  dirs holds one direction per ray of the (2H x 2W) ray grid in
+ * row-major ray order; hit/tray receive per-ray face ids and t (may be NULL). */
+void FN(oracle_render_depth)(int H, int W, const REAL* origin, const REAL* dirs,
+                             const REAL* aabbMin, const REAL* aabbMax, const int* leftNode,
+                             const int* firstTri, const int* triCount, const int* triMap,
+                             const int* faces, const REAL* vertex, REAL* depth, int* hit, REAL* tray,
+                             int* max_stack) {
+  int RW = 2 * W, RH = 2 * H;
+  V3 o = FN(v3)(origin[0], origin[1], origin[2]);
+  REAL* ray = (REAL*)malloc(sizeof(REAL) * (size_t)RW * RH);
+  for (int r = 0; r < RW * RH; ++r) {
+    REAL t;
+    int h = FN(oracle_ray)(o, FN(ld)(dirs, r), aabbMin, aabbMax, leftNode, firstTri, triCount, triMap,
+                           faces, vertex, &t, max_stack);
+    REAL col = R(0);
+    if (h > -1) {
+      col = t / R(50.0) * R(1.0);
+      if (col > R(1.0)) col = R(1.0);
+    }
+    ray[r] = col;
+    if (hit) hit[r] = h;
+    if (tray) tray[r] = t;
+  }
+  for (int py = 0; py < H; ++py)
+    for (int px = 0; px < W; ++px) {
+      int start = py * 2 * RW + px * 2;
+      REAL c = R(0);
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) c = c + ray[start + i * RW + j];
+      depth[py * W + px] = c / R(4.0);
+    }
+  free(ray);
+}
+
+#undef V3
+#undef FEPS
+#undef R
+#undef FN

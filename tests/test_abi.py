@@ -1,0 +1,111 @@
+"""The C ABI: the library loads, exports every function include/ffsrl.h
+declares, the ctypes structures match the C layout, and argument errors come
+back as error codes with a message (no GPU needed)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ffsrl.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ffsrl_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_every_declared_symbol_is_exported(pkg):
+    h = pkg.lib.load()
+    names = _declared()
+    assert len(names) >= 20
+    nm = subprocess.run(["nm", "-D", "--defined-only", pkg._lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (ffsrl_\w+)", nm))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        getattr(h, n)
+    assert set(pkg._lib.exported_symbols()) == set(names)
+
+
+def test_abi_version(pkg):
+    assert pkg.lib.ffsrl_abi_version() == 1
+    assert pkg.lib.ffsrl_frame_max_vertices() >= 10000
+
+
+_LAYOUT_C = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "ffsrl.h"
+#define F(T, f) printf(#T " " #f " %zu\n", offsetof(T, f));
+int main(void) {
+  printf("ffsrl_env_desc size %zu\n", sizeof(ffsrl_env_desc));
+  printf("ffsrl_batch size %zu\n", sizeof(ffsrl_batch));
+  printf("ffsrl_params size %zu\n", sizeof(ffsrl_params));
+  printf("ffsrl_camera size %zu\n", sizeof(ffsrl_camera));
+  %FIELDS%
+  return 0;
+}
+"""
+
+
+def test_struct_layouts_match_header(pkg, tmp_path):
+    L = pkg._lib
+    structs = {"ffsrl_batch": L.Batch, "ffsrl_params": L.Params, "ffsrl_camera": L.Camera,
+               "ffsrl_env_desc": L.EnvDesc}
+    fields = "\n".join(f"F({n}, {f[0]})" for n, S in structs.items() for f in S._fields_)
+    src = tmp_path / "layout.c"
+    src.write_text(_LAYOUT_C.replace("%FIELDS%", fields))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            a, b, c = line.split()
+            got[(a, b)] = int(c)
+    for n, S in structs.items():
+        assert got[(n, "size")] == C.sizeof(S), n
+        for f in S._fields_:
+            assert got[(n, f[0])] == getattr(S, f[0]).offset, (n, f[0])
+
+
+def test_argument_errors_are_reported(pkg):
+    L = pkg._lib
+    b = L.Batch()
+    prm = L.Params()
+    with pytest.raises(pkg.FFSRLError, match="NULL"):
+        pkg.lib.ffsrl_frame(b, prm, None, 1024, 0, None)
+    with pytest.raises(pkg.FFSRLError, match="params NULL"):
+        pkg.lib.ffsrl_frame(b, None, None, 1024, 0, None)
+    cam = L.Camera()
+    cam.width = cam.height = 6
+    with pytest.raises(pkg.FFSRLError):
+        pkg.lib.ffsrl_render(b, cam, 0, 0, None, None, None, None, None, None, None, None)
+    with pytest.raises(pkg.FFSRLError, match="remap"):
+        pkg.lib.ffsrl_remap(None, None, None, 5, None)
+    assert "remap" in pkg.lib.ffsrl_last_error().decode()
+
+
+def test_frame_rejects_envs_beyond_lds(pkg):
+    L = pkg._lib
+    b = L.Batch()
+    dummy = np.zeros(16, np.float32)
+    for f in ("envs", "x", "p", "v", "drag", "tips"):
+        setattr(b, f, dummy.ctypes.data)
+    b.max_nv = pkg.lib.ffsrl_frame_max_vertices() + 2
+    prm = L.Params()
+    prm.substeps = prm.iterations = 1
+    prm.dt = 0.01
+    with pytest.raises(pkg.FFSRLError, match="LDS"):
+        pkg.lib.ffsrl_frame(b, prm, None, 1024, 0, None)
+
+
+def test_header_has_no_hip_types():
+    text = open(HEADER).read()
+    assert "hipStream_t" not in re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    assert "#include <hip" not in text

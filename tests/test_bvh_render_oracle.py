@@ -1,0 +1,188 @@
+"""BVH build, refit and ray cast on the CPU.
+
+* The library's host SAH builder (C++) must produce exactly the topology of
+  the oracle's NumPy restatement of bvh.py, and both must satisfy the build
+  invariants.
+* The oracle refit is pinned by brute-force min/max over each node's triangles
+  (bit-exact), the oracle ray cast by brute-force nearest hits over all
+  triangles and by an analytic plane.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+
+def _build_lib(pkg, mesh):
+    topo = pkg.model.MeshTopology(mesh)
+    return {"left": topo.left, "first": topo.first, "count": topo.count, "level": topo.level,
+            "tri_map": topo.tri_map, "max_level": topo.max_level}
+
+
+def _surface(mesh):
+    surf = mesh.surface_vertices()
+    local = np.full(mesh.num_vertices, -1, np.int32)
+    local[surf] = np.arange(surf.size)
+    return mesh.verts[surf], local[mesh.tris]
+
+
+@pytest.mark.parametrize("which", ["cube", "ellipsoid", "box"])
+def test_builder_matches_restatement(pkg, orc, which):
+    sc = pkg.scenes
+    mesh = {"cube": lambda: sc.cube_grid(8), "ellipsoid": lambda: sc.solid("ellipsoid", 4000, seed=3),
+            "box": lambda: sc.solid("box", 3000, seed=4)}[which]()
+    a = _build_lib(pkg, mesh)
+    v, f = _surface(mesh)
+    b = orc.bvh_build(v, f)
+    for k in ("left", "first", "count", "level", "tri_map"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["max_level"] == b["max_level"]
+
+
+def _check_invariants(mesh, t):
+    n = len(t["left"])
+    nf = mesh.num_tris
+    assert sorted(t["tri_map"].tolist()) == list(range(nf))        # a permutation
+    seen = np.zeros(nf, np.int32)
+    for i in range(n):
+        if t["count"][i] > 0:                                       # leaf
+            assert t["left"][i] == -1
+            seen[t["tri_map"][t["first"][i]:t["first"][i] + t["count"][i]]] += 1
+        else:                                                       # inner: children are left, left+1
+            l = t["left"][i]
+            assert 0 < l < n - 1 and t["level"][l] == t["level"][i] + 1 == t["level"][l + 1]
+            # children partition the parent's triangle range
+            assert t["first"][l] == t["first"][i]
+            assert t["first"][l + 1] == t["first"][l] + _span(t, l)
+    assert (seen == 1).all()                                        # every tri in exactly one leaf
+    assert t["level"][0] == 0 and t["max_level"] == t["level"].max()
+
+
+def _span(t, node):
+    if t["count"][node] > 0:
+        return t["count"][node]
+    l = t["left"][node]
+    return _span(t, l) + _span(t, l + 1)
+
+
+def test_builder_invariants(pkg):
+    for mesh in (pkg.scenes.cube_grid(6), pkg.scenes.solid("ellipsoid", 2000, seed=9)):
+        _check_invariants(mesh, _build_lib(pkg, mesh))
+
+
+def test_builder_rejects_bad_input(pkg):
+    z = np.zeros(8, np.int32)
+    v = np.zeros((3, 3), np.float32)
+    f = np.array([0, 1, 5], np.int32)   # index 5 out of range
+    nn, ml = C.c_int32(), C.c_int32()
+    with pytest.raises(pkg.FFSRLError, match="out of range"):
+        pkg.lib.ffsrl_bvh_build(v.ctypes.data, 3, f.ctypes.data, 1, 8, z.ctypes.data, z.ctypes.data,
+                                z.ctypes.data, z.ctypes.data, z.ctypes.data, C.byref(nn), C.byref(ml))
+
+
+def _deformed(mesh, seed=0, amp=0.05):
+    rng = np.random.default_rng(seed)
+    return (mesh.verts + rng.uniform(-amp, amp, mesh.verts.shape)).astype(np.float32)
+
+
+def test_refit_bit_exact_vs_brute_force(pkg, orc):
+    mesh = pkg.scenes.solid("ellipsoid", 3000, seed=1)
+    t = _build_lib(pkg, mesh)
+    x = _deformed(mesh)
+    mn, mx = orc.refit(t, mesh.tris, x)
+    tv = x[mesh.tris]                                   # (F, 3, 3)
+
+    def bounds(node):
+        if t["count"][node] > 0:
+            ids = t["tri_map"][t["first"][node]:t["first"][node] + t["count"][node]]
+            return tv[ids].reshape(-1, 3).min(0), tv[ids].reshape(-1, 3).max(0)
+        l = t["left"][node]
+        a, b = bounds(l), bounds(l + 1)
+        return np.minimum(a[0], b[0]), np.maximum(a[1], b[1])
+
+    for node in range(len(t["left"])):
+        lo, hi = bounds(node)
+        assert np.array_equal(mn[node], lo) and np.array_equal(mx[node], hi), node
+
+
+def _camera_dirs(pkg, H, W, pos, at):
+    r = pkg.render
+    d = np.asarray(at, np.float64) - np.asarray(pos, np.float32).astype(np.float64)
+    d /= np.linalg.norm(d)
+    import math
+    q = r.quat_rpy(math.asin(d[1]), math.atan2(d[0], d[2]) - math.pi, 0.0)
+    return r.ray_directions(H, W, q)
+
+
+def _mt64(a, b, c, o, d):
+    """Moller-Trumbore in float64 over all triangles (vectorised)."""
+    e1, e2 = b - a, c - a
+    pv = np.cross(d[None], e2)
+    det = (e1 * pv).sum(1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / det
+        tv = o[None] - a
+        u = (tv * pv).sum(1) * inv
+        qv = np.cross(tv, e1)
+        v = (d[None] * qv).sum(1) * inv
+        t = (e2 * qv).sum(1) * inv
+    ok = (np.abs(det) >= 1e-4) & (u >= 0) & (u <= 1) & (v >= 0) & (u + v <= 1) & (t > 0)
+    return np.where(ok, t, np.inf)
+
+
+def test_raycast_matches_brute_force(pkg, orc):
+    mesh = pkg.scenes.solid("ellipsoid", 3000, seed=2)
+    x = _deformed(mesh, 1, 0.01)
+    t = _build_lib(pkg, mesh)
+    mn, mx = orc.refit(t, mesh.tris, x)
+    H = W = 12
+    pos = np.array([0.0, 1.2, 4.0], np.float32)
+    dirs = _camera_dirs(pkg, H, W, pos, (0.0, 0.4, 0.0))
+    depth, hit, tr, ms = orc.render_depth(H, W, pos, dirs, t, mn, mx, mesh.tris, x)
+    assert ms <= t["max_level"] + 1 <= 32                      # stack depth bound
+    tv = x[mesh.tris].astype(np.float64)
+    n_hit = agree = 0
+    for r in range(4 * H * W):
+        tt = _mt64(tv[:, 0], tv[:, 1], tv[:, 2], pos.astype(np.float64), dirs[r, :3].astype(np.float64))
+        best = np.argmin(tt)
+        if not np.isfinite(tt[best]):
+            assert hit[r] == -1
+            continue
+        n_hit += 1
+        assert hit[r] >= 0
+        # the same triangle, or one at the same distance to fp32 precision (ties)
+        if hit[r] == best:
+            agree += 1
+        else:
+            assert abs(tt[hit[r]] - tt[best]) <= 1e-5 * tt[best]
+        assert abs(tr[r] - tt[best]) <= 1e-5 * tt[best]
+    assert n_hit > 100 and agree >= 0.98 * n_hit
+
+
+def test_raycast_plane_analytic(orc):
+    """Two triangles spanning z = -2 seen from the origin along -z: t = 2 / |dz|,
+    depth = t / 50 (render.py:401-407), misses outside the square are 0."""
+    x = np.array([[-1, -1, -2], [1, -1, -2], [1, 1, -2], [-1, 1, -2]], np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    t = {"left": np.array([1, -1, -1], np.int32), "first": np.array([0, 0, 1], np.int32),
+         "count": np.array([0, 1, 1], np.int32), "level": np.array([0, 1, 1], np.int32),
+         "tri_map": np.array([0, 1], np.int32)}
+    mn, mx = orc.refit(t, f, x)
+    H = W = 8
+    RW = RH = 16
+    dirs = np.zeros((RW * RH, 4), np.float32)
+    r = np.arange(RW * RH)
+    sx = 2.0 * (r % RW) / RW - 1.0
+    sy = 2.0 * (RH - r // RW) / RH - 1.0
+    d = np.stack([sx * 0.6, sy * 0.6, -np.ones(RW * RH)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dirs[:, :3] = d
+    depth, hit, tr, _ = orc.render_depth(H, W, np.zeros(3, np.float32), dirs, t, mn, mx, f, x)
+    inside = (np.abs(d[:, 0] * 2 / d[:, 2]) < 0.999) & (np.abs(d[:, 1] * 2 / d[:, 2]) < 0.999)
+    assert (hit[inside] >= 0).all()
+    assert np.allclose(tr[inside], 2.0 / np.abs(d[inside, 2]), rtol=1e-6)
+    far = (np.abs(d[:, 0] * 2 / d[:, 2]) > 1.001) | (np.abs(d[:, 1] * 2 / d[:, 2]) > 1.001)
+    assert (hit[far] == -1).all() and (tr[far] == 0).all()
+    per_ray = np.where(hit >= 0, tr / 50.0, 0.0).reshape(RH, RW)
+    box = per_ray.reshape(H, 2, W, 2).sum(axis=(1, 3)) / 4.0
+    assert np.allclose(depth, box, rtol=1e-6)

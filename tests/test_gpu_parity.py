@@ -1,0 +1,248 @@
+"""GPU parity: the HIP path through the C ABI against the oracle.
+
+Tolerances (from the path's definition of done):
+  * vertex positions after N frames within 1e-5 x the mesh bounding-box
+    diagonal of the fp32 oracle; the errors of both against the fp64 oracle
+    are printed;
+  * BVH node bounds and ray-hit triangle ids bit-exact on the same positions;
+  * depth bit-exact on the same positions (it is a function of the hit t).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+C1 = dict(simSubsteps=10, simFrameRate=60, simConstraintsSteps=1, environmentGravity=(0.0, -9.81, 0.0),
+          environmentVelocityDampening=0.1, distance="xpbd", volume="xpbd", distanceCompliance=1e-6,
+          volumeComplianceXPBD=0.0)
+C1_ORACLE = dict(substeps=10, iterations=1, fps=60, gravity=(0.0, -9.81, 0.0), damping=0.1, dist_mode=2, vol_mode=2,
+                 dist_compliance=1e-6, vol_compliance=0.0)
+NH = dict(simSubsteps=10, simFrameRate=60, simConstraintsSteps=1, environmentGravity=(0.0, -9.81, 0.0),
+          environmentVelocityDampening=0.1, distance=None, volume=None, neoHookean=True,
+          globalVolumeCompliance=1.0 / 5e4, globalDeviatoricCompliance=1.0 / 1e4, globalKsDrag=1.0)
+NH_ORACLE = dict(substeps=10, iterations=1, fps=60, gravity=(0.0, -9.81, 0.0), damping=0.1, nh_on=1,
+                 nh_vol_c=1.0 / 5e4, nh_dev_c=1.0 / 1e4, drag_ks=1.0)
+
+
+def _report(name, gpu_x, o32, o64, diag):
+    e32 = float(np.abs(gpu_x - o32).max())
+    eg64 = float(np.abs(gpu_x - o64).max())
+    e3264 = float(np.abs(o32 - o64).max())
+    print(f"\n[{name}] diag={diag:.4f} |gpu-o32|={e32:.3e} ({e32 / diag:.2e} diag) "
+          f"|gpu-o64|={eg64:.3e} |o32-o64|={e3264:.3e}")
+    return e32
+
+
+def _c1_model(pkg, n=10, frames_velocity_seed=7):
+    mesh = pkg.scenes.cube_grid(n)
+    rng = np.random.default_rng(frames_velocity_seed)
+    v0 = rng.uniform(-0.01, 0.01, size=(mesh.num_vertices, 3)).astype(np.float32)
+    return mesh, v0
+
+
+@pytest.mark.parametrize("form", ["batched", "plain"])
+def test_c1_cube_100_frames(pkg, orc, gpu, form):
+    mesh, v0 = _c1_model(pkg)
+    model = pkg.SimModelDO([mesh], [0], device=gpu, initialVelocity=v0, **C1)
+    integ = pkg.SimIntegratorDO(gpu)
+    frames = 100
+    for _ in range(frames):
+        if form == "batched":
+            integ.stepModel(model)
+        else:
+            integ.stepModelPlain(model)
+    torch.cuda.synchronize()
+    x = model.vertex().cpu().numpy()
+    cfg = orc.StepConfig(**C1_ORACLE)
+    o32 = orc.OracleSim([mesh], [0], np.float32, initial_velocity=v0)
+    o64 = orc.OracleSim([mesh], [0], np.float64, initial_velocity=v0)
+    o32.step(cfg, frames)
+    o64.step(cfg, frames)
+    diag = mesh.bbox_diagonal()
+    e = _report(f"C1 {form}", x, o32.x, o64.x, diag)
+    assert np.isfinite(x).all()
+    assert e <= 1e-5 * diag
+
+
+def _bvh_dict(topo):
+    return {"left": topo.left, "first": topo.first, "count": topo.count, "level": topo.level,
+            "tri_map": topo.tri_map}
+
+
+def test_refit_and_depth_bit_exact(pkg, orc, gpu):
+    """C1 camera: 128x128 depth (256x256 rays) after 20 frames with the refit
+    fused into the frame; bounds, per-ray hits and depth equal the oracle's."""
+    mesh, v0 = _c1_model(pkg)
+    model = pkg.SimModelDO([mesh], [0], device=gpu, initialVelocity=v0, **C1)
+    integ = pkg.SimIntegratorDO(gpu)
+    for _ in range(20):
+        integ.stepModel(model, refit=True)
+    ren = pkg.RaycastRendererDO(gpu, model, cameraPos=(2.2, 1.8, 2.6), resolution=128)
+    ren.lookAt((0.5, 0.45, 0.5))
+    hit = torch.full((4 * 128 * 128,), -7, dtype=torch.int32, device=gpu)
+    thit = torch.zeros(4 * 128 * 128, dtype=torch.float32, device=gpu)
+    depth = ren.render_depth(hit=hit, thit=thit)
+    torch.cuda.synchronize()
+    x = model.vertex().cpu().numpy()
+    bvh = _bvh_dict(model.topo[0])
+    mn, mx = orc.refit(bvh, mesh.tris, x)
+    assert np.array_equal(model.bvh_lo[:, :3].cpu().numpy(), mn)
+    assert np.array_equal(model.bvh_hi[:, :3].cpu().numpy(), mx)
+    d_ref, h_ref, t_ref, ms = orc.render_depth(128, 128, np.asarray(ren.camera.origin[:], np.float32),
+                                               ren.dirs.cpu().numpy(), bvh, mn, mx, mesh.tris, x)
+    h = hit.cpu().numpy()
+    print(f"\n[render C1] hits {(h_ref >= 0).sum()} / {h_ref.size}, max stack {ms}")
+    assert (h_ref >= 0).sum() > 0.2 * h_ref.size
+    assert np.array_equal(h, h_ref)
+    assert np.array_equal(thit.cpu().numpy(), t_ref)
+    assert np.array_equal(depth[0].cpu().numpy(), d_ref)
+
+
+def test_refit_fused_equals_standalone(pkg, gpu):
+    mesh, v0 = _c1_model(pkg, 8)
+    model = pkg.SimModelDO([mesh], [0], device=gpu, initialVelocity=v0, **C1)
+    integ = pkg.SimIntegratorDO(gpu)
+    integ.stepModel(model, refit=True)
+    lo, hi = model.bvh_lo.clone(), model.bvh_hi.clone()
+    model.bvh_lo.zero_()
+    model.bvh_hi.zero_()
+    pkg.SimBVH(gpu, model)                     # refit on construction
+    assert torch.equal(lo, model.bvh_lo) and torch.equal(hi, model.bvh_hi)
+    model.bvh_lo.zero_()
+    model.bvh_hi.zero_()
+    pkg.SimBVH(gpu, model).refitBVH(plain=True)
+    assert torch.equal(lo, model.bvh_lo) and torch.equal(hi, model.bvh_hi)
+
+
+def _grasp_scene(pkg, n_env, tets, seed):
+    sc = pkg.scenes
+    meshes, tips, grasp = [], [], []
+    for i in range(n_env):
+        m = sc.solid("ellipsoid", tets, seed=seed + i)
+        meshes.append(m)
+        tip = sc.top_point(m) + np.array([0.0, 0.02, 0.0], np.float32)
+        tips.append(np.repeat(tip[None], 4, axis=0))
+        grasp.append(sc.nearest_surface_vertices(m, tip, 8))
+    return meshes, np.stack(tips), grasp
+
+
+@pytest.mark.parametrize("form", ["batched", "plain"])
+def test_c2_neohookean_grasp_random_walk(pkg, orc, gpu, form):
+    """Config-2 scheme at reduced size: NH (mu 1e4, lambda 5e4), 8 grasped
+    surface vertices pulled toward a tool tip on a seeded random walk."""
+    n_env, frames = 3, 30
+    meshes, tips, grasp = _grasp_scene(pkg, n_env, 2500, 11)
+    model = pkg.SimModelDO(meshes, list(range(n_env)), device=gpu, tips=tips, **NH)
+    o32 = orc.OracleSim(meshes, list(range(n_env)), np.float32, tips=tips)
+    o64 = orc.OracleSim(meshes, list(range(n_env)), np.float64, tips=tips)
+    for i, g in enumerate(grasp):
+        model.set_grasp(i, g)
+        o32.drag[o32.vbase[i] + g] = 1.0
+        o64.drag[o64.vbase[i] + g] = 1.0
+    rng = np.random.default_rng(5)
+    integ = pkg.SimIntegratorDO(gpu)
+    cfg = orc.StepConfig(**NH_ORACLE)
+    for _ in range(frames):
+        delta = rng.normal(0.0, 0.002, size=(n_env, 3)).astype(np.float32)
+        dt = torch.from_numpy(delta).to(gpu)
+        pkg.lib.ffsrl_move_tools(model.batch, dt.data_ptr(), None, None, model.stream())
+        if form == "batched":
+            integ.stepModel(model)
+        else:
+            integ.stepModelPlain(model)
+        step = np.repeat(delta, 4, axis=0)
+        o32.tips = (o32.tips + step).astype(np.float32)
+        o64.tips = o64.tips + step.astype(np.float64)
+        o32.step(cfg, 1)
+        o64.step(cfg, 1)
+    torch.cuda.synchronize()
+    assert np.array_equal(model.laparoscopeTip.cpu().numpy().reshape(-1, 3), o32.tips)
+    x = model.vertex().cpu().numpy()
+    for i in range(n_env):
+        s = slice(o32.vbase[i], o32.vbase[i + 1])
+        diag = meshes[i].bbox_diagonal()
+        e = _report(f"C2 {form} env{i}", x[s], o32.x[s], o64.x[s], diag)
+        assert e <= 1e-5 * diag
+    # the grasped vertices follow the tool
+    g0 = model.vertex()[model.env_slice(0)][torch.as_tensor(grasp[0], device=gpu)].mean(0).cpu().numpy()
+    assert np.linalg.norm(g0 - model.laparoscopeTip[0, 2].cpu().numpy()) < 0.1
+
+
+def test_mixed_sizes_alone_equals_in_mix(pkg, gpu):
+    """Each env's result does not depend on the envs batched with it."""
+    sc = pkg.scenes
+    specs = [("ellipsoid", 1200), ("box", 5000), ("ellipsoid", 9000), ("box", 2000), ("ellipsoid", 3000)]
+    meshes = [sc.solid(s, t, seed=i) for i, (s, t) in enumerate(specs)]
+    kw = dict(NH, distance="xpbd", volume="xpbd", distanceCompliance=1e-6, volumeComplianceXPBD=0.0)
+    mix = pkg.SimModelDO(meshes, [4, 0, 3, 1, 2, 0], device=gpu, **kw)
+    integ = pkg.SimIntegratorDO(gpu)
+    for _ in range(10):
+        integ.stepModel(mix, refit=True)
+    for e, k in enumerate([4, 0, 3, 1, 2, 0]):
+        alone = pkg.SimModelDO([meshes[k]], [0], device=gpu, **kw)
+        for _ in range(10):
+            integ.stepModel(alone, refit=True)
+        a = alone.vertex().cpu().numpy()
+        b = mix.env_vertices(e).cpu().numpy()
+        diag = meshes[k].bbox_diagonal()
+        err = float(np.abs(a - b).max())
+        print(f"\n[alone vs mix] env {e} mesh {k}: {err:.3e} ({err / diag:.2e} diag)")
+        assert err <= 1e-5 * diag
+
+
+def test_xpbd_two_iterations_carry_lambda(pkg, orc, gpu):
+    mesh, v0 = _c1_model(pkg, 7)
+    kw = dict(C1, simConstraintsSteps=2)
+    for form in ("batched", "plain"):
+        model = pkg.SimModelDO([mesh], [0], device=gpu, initialVelocity=v0, **kw)
+        assert model.carries_lambda
+        integ = pkg.SimIntegratorDO(gpu)
+        for _ in range(30):
+            integ.stepModel(model) if form == "batched" else integ.stepModelPlain(model)
+        torch.cuda.synchronize()
+        cfg = orc.StepConfig(**dict(C1_ORACLE, iterations=2))
+        o32 = orc.OracleSim([mesh], [0], np.float32, initial_velocity=v0)
+        o64 = orc.OracleSim([mesh], [0], np.float64, initial_velocity=v0)
+        o32.step(cfg, 30)
+        o64.step(cfg, 30)
+        diag = mesh.bbox_diagonal()
+        e = _report(f"XPBD I=2 {form}", model.vertex().cpu().numpy(), o32.x, o64.x, diag)
+        assert e <= 1e-5 * diag
+
+
+def test_pbd_reference_do_path(pkg, orc, gpu):
+    """The reference DO step's own forms: PBD volume + PBD distance with kS
+    (integrator.py:1658-1680) on a negatively oriented mesh (Q4)."""
+    mesh = pkg.scenes.cube_grid(7, orientation=-1)
+    kw = dict(simSubsteps=8, simFrameRate=30, simConstraintsSteps=1, environmentGravity=(0.0, -9.8, 0.0),
+              environmentVelocityDampening=0.1, distance="pbd", volume="pbd", globalKsDistance=1.0,
+              globalKsVolume=1.0)
+    model = pkg.SimModelDO([mesh], [0], device=gpu, **kw)
+    integ = pkg.SimIntegratorDO(gpu)
+    for _ in range(40):
+        integ.stepModel(model)
+    torch.cuda.synchronize()
+    cfg = orc.StepConfig(substeps=8, iterations=1, fps=30, gravity=(0.0, -9.8, 0.0), damping=0.1, dist_mode=1,
+                         dist_k=1.0, vol_mode=1, vol_k=1.0)
+    o32 = orc.OracleSim([mesh], [0], np.float32)
+    o64 = orc.OracleSim([mesh], [0], np.float64)
+    o32.step(cfg, 40)
+    o64.step(cfg, 40)
+    diag = mesh.bbox_diagonal()
+    e = _report("PBD DO path", model.vertex().cpu().numpy(), o32.x, o64.x, diag)
+    assert e <= 1e-5 * diag
+
+
+def test_reset_restores_initial_state(pkg, gpu):
+    mesh, v0 = _c1_model(pkg, 6)
+    model = pkg.SimModelDO([mesh, mesh], [0, 1], device=gpu, **C1)
+    integ = pkg.SimIntegratorDO(gpu)
+    for _ in range(5):
+        integ.stepModel(model)
+    before = model.env_vertices(1).clone()
+    model.resetFixedModel([0])
+    torch.cuda.synchronize()
+    assert torch.equal(model.env_vertices(0), model.initialVertex[model.env_slice(0), :3])
+    assert torch.equal(model.env_vertices(1), before)
+    assert not model.v[model.env_slice(0)].any()
