@@ -26,6 +26,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 256 MiB device-to-device copy before timing (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -135,6 +141,7 @@ def main():
     wl = importlib.import_module("ff-srl_amd.workloads")
     t_setup = time.perf_counter()
     w = make_workload(wl, args)                       # meshes generated before the GPU runtime starts
+    log(f"workload {w.name}: {w.totals()} in {time.perf_counter() - t_setup:.1f} s")
     import torch
     pkg = importlib.import_module("ff-srl_amd")
     dev = torch.device("cuda:0")
@@ -158,8 +165,15 @@ def main():
         lib.ffsrl_render(model.batch, ren.camera, 0, n, depth_out.data_ptr(), None, None, None, None, None, None, sh)
 
     pkg.SimBVH(dev, model)
+    if args.calibrate_copy:
+        a = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t_setup
+    log(f"model on device after {setup_s:.1f} s; warmup")
     for f in range(args.warmup):
         step(f, s)
     torch.cuda.synchronize()
@@ -179,6 +193,7 @@ def main():
         e3.record(stream)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {elapsed:.3f} s")
     step_ms = np.array([ev[i][0].elapsed_time(ev[i][3]) for i in range(args.steps)])
     frame_ms = np.array([ev[i][1].elapsed_time(ev[i][2]) for i in range(args.steps)])
     render_ms = np.array([ev[i][2].elapsed_time(ev[i][3]) for i in range(args.steps)])
@@ -206,6 +221,7 @@ def main():
                        "algorithmic_bytes_per_launch": alg}
 
     if not args.no_graph:
+        log("graph replay")
         g = C.c_void_p()
         cs = torch.cuda.Stream()
         cs.wait_stream(stream)
@@ -222,6 +238,7 @@ def main():
         lib.ffsrl_graph_destroy(g)
 
     if not args.no_plain and args.plain_frames > 0:
+        log("plain form")
         model.enable_plain()
 
         def plain_step(f):
@@ -247,6 +264,7 @@ def main():
         out["batched_over_plain"] = value / out["plain_env_steps_per_s"]
 
     if not args.no_cpu_baseline:
+        log("cpu baseline")
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
         out["cpu_baseline"] = cpu_baseline(pkg, orc, w, model, ren, args.cpu_seconds, args.seed)

@@ -18,7 +18,7 @@ E_ARG, E_LIMIT, E_STATE = -1, -2, -3
 class EnvDesc(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "vbase", "nv", "ebase", "ne", "tbase", "nt", "fbase", "nf", "nodebase", "nn",
-        "boundbase", "lvlbase", "nlvl", "lam_ebase", "lam_tbase", "flags")]
+        "boundbase", "lvlbase", "nlvl", "lam_ebase", "lam_tbase", "dp_exp")]
 
 
 ENV_DESC_FIELDS = [f[0] for f in EnvDesc._fields_]
@@ -36,7 +36,7 @@ class Batch(C.Structure):
         ("node_count", _vp), ("level_nodes", _vp), ("level_off", _vp),
         ("x", _vp), ("p", _vp), ("v", _vp), ("drag", _vp), ("lam_e", _vp), ("lam_t", _vp),
         ("bvh_lo", _vp), ("bvh_hi", _vp), ("tips", _vp),
-        ("dp", _vp), ("cnt", _vp),
+        ("dp", _vp), ("cnt", _vp), ("host_envs", _vp), ("host_level_off", _vp),
     ]
 
 
@@ -80,6 +80,7 @@ SIGNATURES = {
     "ffsrl_bvh_refit_plain": [_P(Batch), _i32, _i32, _vp],
     "ffsrl_remap": [_vp, _vp, _vp, _i64, _vp],
     "ffsrl_bvh_build": [_vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _P(_i32), _P(_i32)],
+    "ffsrl_color_constraints": [_vp, _i64, _i32, _i32, _vp, _P(_i32)],
     "ffsrl_graph_begin": [_vp],
     "ffsrl_graph_end": [_vp, _P(_vp)],
     "ffsrl_graph_launch": [_vp, _vp],

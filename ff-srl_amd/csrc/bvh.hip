@@ -181,22 +181,11 @@ int ffsrl_bvh_refit_plain(const ffsrl_batch* b, int32_t eb, int32_t ee, void* st
   if (!b || !b->envs || !b->level_off) return fail(FFSRL_E_ARG, "refit needs the env table and BVH arrays");
   if (eb < 0 || ee > b->n_env || eb > ee) return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", eb, ee, b->n_env);
   hipStream_t st = as_stream(stream);
-  // The per-level launch needs host copies of the env table and level offsets.
-  static thread_local std::vector<ffsrl_env_desc> envs;
-  static thread_local std::vector<int32_t> lvl;
-  static thread_local const void* key_envs = nullptr;
-  static thread_local const void* key_lvl = nullptr;
-  if (key_envs != b->envs || key_lvl != b->level_off || (int)envs.size() != b->n_env) {
-    envs.resize(b->n_env);
-    FFSRL_HIP(hipMemcpyAsync(envs.data(), b->envs, sizeof(ffsrl_env_desc) * b->n_env, hipMemcpyDeviceToHost, st));
-    int32_t nlo = 0;
-    for (auto& e : envs) nlo = std::max(nlo, e.lvlbase + e.nlvl + 1);
-    lvl.resize(nlo);
-    if (nlo) FFSRL_HIP(hipMemcpyAsync(lvl.data(), b->level_off, sizeof(int32_t) * nlo, hipMemcpyDeviceToHost, st));
-    FFSRL_HIP(hipStreamSynchronize(st));
-    key_envs = b->envs;
-    key_lvl = b->level_off;
-  }
+  // The per-level launches are issued from the host: use the host mirrors.
+  if (!b->host_envs || !b->host_level_off)
+    return fail(FFSRL_E_ARG, "the plain refit needs host_envs and host_level_off");
+  const ffsrl_env_desc* envs = b->host_envs;
+  const int32_t* lvl = b->host_level_off;
   for (int i = eb; i < ee; ++i) {
     const ffsrl_env_desc& e = envs[i];
     for (int L = 0; L < e.nlvl; ++L) {

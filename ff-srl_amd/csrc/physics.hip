@@ -47,22 +47,36 @@ constexpr int lds_bytes_for(int nv) {
 }
 constexpr int kMaxFrameVertices = (kLdsBytes / 14) & ~1;
 
+// Jacobi sums in LDS are fixed point: each delta component is rounded to the
+// nearest multiple of the env's quantum q = 2^dp_exp (about 2^-27 of the mesh
+// diagonal D) and added with an integer LDS atomic.  Integer atomics run at
+// the LDS's full rate on gfx950 (float LDS atomics are ~30x slower), and
+// integer sums do not depend on the order of the additions, so the frame is
+// deterministic.  Each term is clamped to +-2^24 q (D/16 .. D/8), so sums of
+// up to 128 terms cannot overflow; the rounding error per term is <= q/2.
 struct FrameCtx {
-  float* dpx;
-  float* dpy;
-  float* dpz;
+  int32_t* dpx;
+  int32_t* dpy;
+  int32_t* dpz;
   uint32_t* cnt;  // two 16-bit counters per word
+  float inv_q;    // 2^-dp_exp
+  float q;        // 2^dp_exp
 };
 
+__device__ __forceinline__ int32_t to_fix(float x, float inv_q) {
+  float v = rintf(x * inv_q);  // scaling by a power of two is exact
+  v = fminf(fmaxf(v, -16777216.0f), 16777216.0f);
+  return (int32_t)v;
+}
 __device__ __forceinline__ void lds_add(const FrameCtx& c, int i, f3 d) {
-  atomicAdd(&c.dpx[i], d.x);
-  atomicAdd(&c.dpy[i], d.y);
-  atomicAdd(&c.dpz[i], d.z);
+  atomicAdd(&c.dpx[i], to_fix(d.x, c.inv_q));
+  atomicAdd(&c.dpy[i], to_fix(d.y, c.inv_q));
+  atomicAdd(&c.dpz[i], to_fix(d.z, c.inv_q));
 }
 __device__ __forceinline__ void lds_sub(const FrameCtx& c, int i, f3 d) {
-  atomicAdd(&c.dpx[i], -d.x);
-  atomicAdd(&c.dpy[i], -d.y);
-  atomicAdd(&c.dpz[i], -d.z);
+  atomicAdd(&c.dpx[i], -to_fix(d.x, c.inv_q));
+  atomicAdd(&c.dpy[i], -to_fix(d.y, c.inv_q));
+  atomicAdd(&c.dpz[i], -to_fix(d.z, c.inv_q));
 }
 __device__ __forceinline__ void lds_count(const FrameCtx& c, int i, uint32_t n) {
   atomicAdd(&c.cnt[i >> 1], n << (16 * (i & 1)));
@@ -79,12 +93,13 @@ struct VertexStep {
 template <int BLOCK, bool CARRY>
 __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm, const int32_t* order,
                                                  int refit) {
-  extern __shared__ float lds[];
+  extern __shared__ int32_t lds[];
   const int env = order ? order[blockIdx.x] : (int)blockIdx.x;
   const ffsrl_env_desc e = b.envs[env];
   const int nv = e.nv;
   const int cap = (nv + 1) & ~1;
-  FrameCtx c{lds, lds + cap, lds + 2 * cap, reinterpret_cast<uint32_t*>(lds + 3 * cap)};
+  FrameCtx c{lds, lds + cap, lds + 2 * cap, reinterpret_cast<uint32_t*>(lds + 3 * cap), ldexpf(1.0f, -e.dp_exp),
+             ldexpf(1.0f, e.dp_exp)};
   const int tid = threadIdx.x;
   const int64_t vb = e.vbase;
   float* X = b.x;
@@ -103,9 +118,9 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
   const int64_t T = b.n_tet_total;
 
   for (int i = tid; i < cap; i += BLOCK) {
-    c.dpx[i] = 0.0f;
-    c.dpy[i] = 0.0f;
-    c.dpz[i] = 0.0f;
+    c.dpx[i] = 0;
+    c.dpy[i] = 0;
+    c.dpz[i] = 0;
   }
   for (int i = tid; i < cap / 2; i += BLOCK) c.cnt[i] = 0u;
   // predict of substep 0
@@ -204,11 +219,12 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
           f3 p = xyz(p4);
           if (n > 0) {
             const float N = (float)n;
-            p = p + mk(c.dpx[i] / N, c.dpy[i] / N, c.dpz[i] / N);
+            const f3 d = mk((float)c.dpx[i] * c.q, (float)c.dpy[i] * c.q, (float)c.dpz[i] * c.q);
+            p = p + mk(d.x / N, d.y / N, d.z / N);
           }
-          c.dpx[i] = 0.0f;
-          c.dpy[i] = 0.0f;
-          c.dpz[i] = 0.0f;
+          c.dpx[i] = 0;
+          c.dpy[i] = 0;
+          c.dpz[i] = 0;
           const float dr = b.drag[vb + i];
           if (dr != 0.0f) p = grasp_pull(p, tip, dr, vs.ks);
           if (!last_it) {
@@ -422,57 +438,21 @@ __global__ void k_remap(float* dst, const float* src, const int32_t* map, int64_
   st4(dst, map[2 * i], ld4(src, map[2 * i + 1]));
 }
 
-static int host_envs(const ffsrl_batch* b, int begin, int end, ffsrl_env_desc* out, hipStream_t st) {
-  if (begin < 0 || end > b->n_env || begin > end) return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", begin, end, b->n_env);
-  if (end == begin) return 0;
-  FFSRL_HIP(hipMemcpyAsync(out, b->envs + begin, sizeof(ffsrl_env_desc) * (end - begin), hipMemcpyDeviceToHost, st));
-  FFSRL_HIP(hipStreamSynchronize(st));
-  return 0;
-}
-
-static int64_t total_vertices(const ffsrl_env_desc* e, int n) {
-  int64_t v = 0;
-  for (int i = 0; i < n; ++i) v = std::max<int64_t>(v, (int64_t)e[i].vbase + e[i].nv);
-  return v;
-}
-
 }  // namespace ffsrl
 
 using namespace ffsrl;
 
-// Env descriptors of the plain form are cached on the host per batch pointer
-// (they are read once per call otherwise, which would synchronise the stream).
-#include <unordered_map>
-#include <vector>
-#include <mutex>
-
 namespace {
-struct PlainCache {
-  const ffsrl_env_desc* dev = nullptr;
-  int n = 0;
-  std::vector<ffsrl_env_desc> host;
-  int64_t vtotal = 0;
-};
-std::mutex g_cache_mu;
-PlainCache g_cache;
-
-int plain_envs(const ffsrl_batch* b, int begin, int end, hipStream_t st, const ffsrl_env_desc** out,
-               int64_t* vtotal) {
+// The plain form launches per environment from the host, so it reads the
+// caller's host mirror of the env table (ffsrl_batch.host_envs).
+int plain_envs(const ffsrl_batch* b, int begin, int end, const ffsrl_env_desc** out, int64_t* vtotal) {
   if (begin < 0 || end > b->n_env || begin > end)
     return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", begin, end, b->n_env);
-  std::lock_guard<std::mutex> lk(g_cache_mu);
-  if (g_cache.dev != b->envs || g_cache.n != b->n_env) {
-    g_cache.host.resize(b->n_env);
-    if (b->n_env) {
-      int rc = host_envs(b, 0, b->n_env, g_cache.host.data(), st);
-      if (rc) return rc;
-    }
-    g_cache.dev = b->envs;
-    g_cache.n = b->n_env;
-    g_cache.vtotal = total_vertices(g_cache.host.data(), b->n_env);
-  }
-  *out = g_cache.host.data();
-  *vtotal = g_cache.vtotal;
+  if (!b->host_envs) return fail(FFSRL_E_ARG, "the plain form needs host_envs (host mirror of the env table)");
+  int64_t v = 0;
+  for (int i = 0; i < b->n_env; ++i) v = std::max<int64_t>(v, (int64_t)b->host_envs[i].vbase + b->host_envs[i].nv);
+  *out = b->host_envs;
+  *vtotal = v;
   return 0;
 }
 
@@ -515,7 +495,7 @@ int ffsrl_predict(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int
   const ffsrl_env_desc* h;
   int64_t V;
   hipStream_t st = as_stream(stream);
-  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nv) continue;
     hipLaunchKernelGGL(k_predict, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, *prm, (int64_t)h[i].vbase, h[i].nv);
@@ -532,7 +512,7 @@ int ffsrl_project_volume(const ffsrl_batch* b, const ffsrl_params* prm, int32_t 
   const ffsrl_env_desc* h;
   int64_t V;
   hipStream_t st = as_stream(stream);
-  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nt) continue;
     hipLaunchKernelGGL(k_volume, dim3(nblk(h[i].nt, 256)), dim3(256), 0, st, *b, *prm, h[i], V);
@@ -549,7 +529,7 @@ int ffsrl_project_neohookean(const ffsrl_batch* b, const ffsrl_params* prm, int3
   const ffsrl_env_desc* h;
   int64_t V;
   hipStream_t st = as_stream(stream);
-  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nt) continue;
     hipLaunchKernelGGL(k_neohookean, dim3(nblk(h[i].nt, 256)), dim3(256), 0, st, *b, *prm, h[i], V);
@@ -566,7 +546,7 @@ int ffsrl_project_distance(const ffsrl_batch* b, const ffsrl_params* prm, int32_
   const ffsrl_env_desc* h;
   int64_t V;
   hipStream_t st = as_stream(stream);
-  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].ne) continue;
     hipLaunchKernelGGL(k_distance, dim3(nblk(h[i].ne, 256)), dim3(256), 0, st, *b, *prm, h[i], V);
@@ -582,7 +562,7 @@ int ffsrl_apply(const ffsrl_batch* b, int32_t eb, int32_t ee, void* stream) {
   const ffsrl_env_desc* h;
   int64_t V;
   hipStream_t st = as_stream(stream);
-  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nv) continue;
     hipLaunchKernelGGL(k_apply, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, (int64_t)h[i].vbase, h[i].nv, V);
@@ -597,7 +577,7 @@ int ffsrl_grasp(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32
   const ffsrl_env_desc* h;
   int64_t V;
   hipStream_t st = as_stream(stream);
-  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nv) continue;
     hipLaunchKernelGGL(k_grasp, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, *prm, i, (int64_t)h[i].vbase, h[i].nv);
@@ -612,7 +592,7 @@ int ffsrl_bounds_integrate(const ffsrl_batch* b, const ffsrl_params* prm, int32_
   const ffsrl_env_desc* h;
   int64_t V;
   hipStream_t st = as_stream(stream);
-  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nv) continue;
     hipLaunchKernelGGL(k_bounds_integrate, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, *prm, (int64_t)h[i].vbase,
@@ -628,7 +608,7 @@ int ffsrl_zero_lambdas(const ffsrl_batch* b, int32_t eb, int32_t ee, void* strea
   const ffsrl_env_desc* h;
   int64_t V;
   hipStream_t st = as_stream(stream);
-  if ((rc = plain_envs(b, eb, ee, st, &h, &V))) return rc;
+  if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (b->lam_e && h[i].ne)
       hipLaunchKernelGGL(k_zero, dim3(nblk(h[i].ne, 256)), dim3(256), 0, st, b->lam_e, (int64_t)h[i].lam_ebase, h[i].ne);
@@ -655,7 +635,7 @@ int ffsrl_grasp_select(const ffsrl_batch* b, float radius, const int32_t* state,
   const ffsrl_env_desc* h;
   int64_t V;
   hipStream_t st = as_stream(stream);
-  if ((rc = plain_envs(b, 0, b->n_env, st, &h, &V))) return rc;
+  if ((rc = plain_envs(b, 0, b->n_env, &h, &V))) return rc;
   for (int i = 0; i < b->n_env; ++i) {
     if (!h[i].nv) continue;
     hipLaunchKernelGGL(k_grasp_select, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, i, radius, state, update);

@@ -77,6 +77,26 @@ class MeshTopology:
         self.level_nodes = np.concatenate(order).astype(np.int32)
         self.level_off = np.asarray(self.level_off, dtype=np.int32)
         self.tri_colors_leaf = mesh.tri_colors[tri_map] if mesh.tri_colors is not None else None
+        # Constraint order on the device: sorted by colour (stable), so a
+        # wavefront's lanes touch disjoint vertices.
+        self.tet_perm, self.num_tet_colors = self._color_order(mesh.tets, mesh.num_vertices)
+        self.edge_perm, self.num_edge_colors = self._color_order(mesh.edges, mesh.num_vertices)
+        self.tets = np.ascontiguousarray(mesh.tets[self.tet_perm]).astype(np.int32)
+        self.tet_rest_volume = np.ascontiguousarray(mesh.tet_rest_volume[self.tet_perm])
+        self.tet_inv_rest = np.ascontiguousarray(mesh.tet_inv_rest[self.tet_perm].reshape(-1, 9))
+        self.edges = np.ascontiguousarray(mesh.edges[self.edge_perm]).astype(np.int32)
+        self.edge_rest = np.ascontiguousarray(mesh.edge_rest[self.edge_perm])
+
+    @staticmethod
+    def _color_order(elems: np.ndarray, nv: int):
+        elems = np.ascontiguousarray(elems, dtype=np.int32)
+        n = elems.shape[0]
+        if n == 0:
+            return np.zeros(0, np.int64), 0
+        color = np.empty(n, np.int32)
+        nc = C.c_int32(0)
+        lib.ffsrl_color_constraints(elems.ctypes.data, n, elems.shape[1], nv, color.ctypes.data, C.byref(nc))
+        return np.argsort(color, kind="stable"), int(nc.value)
 
 
 class SimModelDO:
@@ -167,18 +187,19 @@ class SimModelDO:
             off["l"].append(off["l"][-1] + t.level_off.size)
         self.mesh_off = {k: np.asarray(v, dtype=np.int64) for k, v in off.items()}
         cat = np.concatenate
-        self.t_edges = torch.from_numpy(cat([t.mesh.edges for t in self.topo]).astype(np.int32)).to(dev)
-        self.t_edge_rest = torch.from_numpy(cat([t.mesh.edge_rest for t in self.topo])).to(dev)
-        self.t_tets = torch.from_numpy(cat([t.mesh.tets for t in self.topo]).astype(np.int32)).to(dev)
-        self.t_tet_rest_volume = torch.from_numpy(cat([t.mesh.tet_rest_volume for t in self.topo])).to(dev)
-        inv_rest = cat([t.mesh.tet_inv_rest.reshape(-1, 9) for t in self.topo]).astype(np.float32)
+        self.t_edges = torch.from_numpy(cat([t.edges for t in self.topo])).to(dev)
+        self.t_edge_rest = torch.from_numpy(cat([t.edge_rest for t in self.topo])).to(dev)
+        self.t_tets = torch.from_numpy(cat([t.tets for t in self.topo])).to(dev)
+        self.t_tet_rest_volume = torch.from_numpy(cat([t.tet_rest_volume for t in self.topo])).to(dev)
+        inv_rest = cat([t.tet_inv_rest for t in self.topo]).astype(np.float32)
         self.t_tet_inv_rest = torch.from_numpy(np.ascontiguousarray(inv_rest.T)).to(dev)
         self.t_tris = torch.from_numpy(cat([t.tris_leaf for t in self.topo])).to(dev)
         self.t_node_left = torch.from_numpy(cat([t.left for t in self.topo])).to(dev)
         self.t_node_first = torch.from_numpy(cat([t.first for t in self.topo])).to(dev)
         self.t_node_count = torch.from_numpy(cat([t.count for t in self.topo])).to(dev)
         self.t_level_nodes = torch.from_numpy(cat([t.level_nodes for t in self.topo])).to(dev)
-        self.t_level_off = torch.from_numpy(cat([t.level_off for t in self.topo])).to(dev)
+        self.level_off_host = np.ascontiguousarray(cat([t.level_off for t in self.topo]).astype(np.int32))
+        self.t_level_off = torch.from_numpy(self.level_off_host).to(dev)
         tc = [t.tri_colors_leaf if t.tri_colors_leaf is not None else np.ones((t.mesh.num_tris, 3), np.float32)
               for t in self.topo]
         self.t_tri_colors = torch.from_numpy(cat(tc).astype(np.float32)).to(dev)
@@ -205,6 +226,10 @@ class SimModelDO:
             row[F["boundbase"]] = nb
             row[F["lvlbase"]], row[F["nlvl"]] = self.mesh_off["l"][k], t.max_level + 1
             row[F["lam_ebase"]], row[F["lam_tbase"]] = (le, lt) if carry else (0, 0)
+            # fixed-point quantum of the batched frame's Jacobi sums: 2^-27 of the
+            # mesh diagonal rounded down to a power of two (range 8..16 diagonals)
+            diag = max(m.bbox_diagonal(), 1e-6)
+            row[F["dp_exp"]] = int(math.floor(math.log2(diag))) - 27
             vb += m.num_vertices
             nb += t.num_nodes
             if carry:
@@ -282,6 +307,8 @@ class SimModelDO:
         b.tips = _ptr(self.laparoscopeTip)
         b.dp = _ptr(self.dp)
         b.cnt = _ptr(self.cnt)
+        b.host_envs = self.env_desc_host.ctypes.data
+        b.host_level_off = self.level_off_host.ctypes.data
         return b
 
     def enable_plain(self):

@@ -162,7 +162,8 @@ def _cells_for_target(shape, semi_axes, target_tets):
     a = np.asarray(semi_axes, dtype=np.float64)
     vol = (4.0 / 3.0) * math.pi * a.prod() if shape == "ellipsoid" else 8.0 * a.prod()
     h = (5.0 * vol / float(target_tets)) ** (1.0 / 3.0)
-    # Refine h so the clipped cell count lands near the target.
+    # Refine h so the clipped cell count lands near the target, then keep the
+    # closest count among nearby cell sizes (counts are quantised by the grid).
     for _ in range(6):
         cells, origin = _clip_cells(shape, a, h)
         got = 5 * len(cells)
@@ -173,7 +174,14 @@ def _cells_for_target(shape, semi_axes, target_tets):
         if abs(ratio - 1.0) < 0.02:
             break
         h *= ratio ** (1.0 / 3.0)
-    return cells, origin, h
+    best = (abs(5 * len(cells) - target_tets), cells, origin, h)
+    if best[0] > 0.02 * target_tets:
+        for f in np.linspace(0.9, 1.1, 21):
+            c2, o2 = _clip_cells(shape, a, h * f)
+            err = abs(5 * len(c2) - target_tets)
+            if err < best[0]:
+                best = (err, c2, o2, h * f)
+    return best[1], best[2], best[3]
 
 
 def solid(shape: str = "ellipsoid", target_tets: int = 20000, semi_axes=(1.0, 0.6, 0.4),

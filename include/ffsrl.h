@@ -50,7 +50,8 @@ typedef struct ffsrl_env_desc {
   int32_t lvlbase, nlvl;   /* level_off[lvlbase .. lvlbase+nlvl] (levels deepest first) */
   int32_t lam_ebase;       /* first edge lambda of this env (XPBD, iterations > 1) */
   int32_t lam_tbase;       /* first tet lambda of this env */
-  int32_t flags;           /* reserved */
+  int32_t dp_exp;          /* quantum of the fixed-point Jacobi sums of the batched
+                              frame: 2^dp_exp (about 2^-27 x bbox diagonal) */
 } ffsrl_env_desc;
 
 /* Device arrays of a batch of environments. */
@@ -84,6 +85,10 @@ typedef struct ffsrl_batch {
   /* plain-form scratch (per-kernel launches with global atomics) */
   float* dp;                      /* [3][V] planes (NULL unless the plain form is used) */
   int32_t* cnt;                   /* [V] */
+  /* host mirrors needed by the plain form, which launches per env (and per
+   * BVH level) from the host: the env table and the level offsets */
+  const ffsrl_env_desc* host_envs;   /* [n_env] (host memory) */
+  const int32_t* host_level_off;     /* same content as level_off (host memory) */
 } ffsrl_batch;
 
 /* Scalars of SimModelDO handed to the kernels (modelDO.py:774-830). */
@@ -190,6 +195,14 @@ int ffsrl_remap(float* dst, const float* src, const int32_t* map, int64_t n, voi
 int ffsrl_bvh_build(const float* verts, int32_t nv, const int32_t* faces, int32_t nf, int32_t bins,
                     int32_t* left, int32_t* first, int32_t* count, int32_t* level, int32_t* tri_map,
                     int32_t* num_nodes, int32_t* max_level);
+
+/* Greedy colouring of n constraints of `arity` vertices (host memory): two
+ * constraints that share a vertex get different colours.  Sorting a mesh's
+ * tets and edges by colour before upload keeps the lanes of a wavefront on
+ * disjoint vertices.  No reference counterpart (the reference accumulates
+ * with global atomics in element order, integrator.py:105-109). */
+int ffsrl_color_constraints(const int32_t* elems, int64_t n, int32_t arity, int32_t nv, int32_t* color,
+                            int32_t* num_colors);
 
 /* ---- HIP graphs --------------------------------------------------------- */
 /* Capture everything issued on `stream` between begin and end into a graph
