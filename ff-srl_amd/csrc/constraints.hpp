@@ -1,40 +1,81 @@
 // Per-constraint projection math, shared by the batched (LDS) and plain
 // (global-atomic) kernels.  Each function follows the reference kernel's
-// expression order; positions arrive as float4 (x, y, z, inverse mass).
+// expression; positions arrive as float4 (x, y, z, inverse mass).
+//
+// Arithmetic policy Ar<FAST>:
+//   FAST = false: IEEE division and square root, the reference's operation
+//                 order (the plain form);
+//   FAST = true:  hardware reciprocal / square root / reciprocal square root
+//                 (<= 1 ulp), multiplications by reciprocals, squared lengths
+//                 taken as dot products (the batched frame).
+// Both are within the positional tolerance of the reference; the choice only
+// moves the last bits.  This file is compiled with multiply-add contraction
+// (no bit-exact result depends on it: the refit is min/max only).
 #pragma once
 
 #include "common.hpp"
 
 namespace ffsrl {
 
+template <bool FAST>
+struct Ar;
+
+template <>
+struct Ar<false> {
+  static __device__ __forceinline__ float div(float a, float b) { return a / b; }
+  static __device__ __forceinline__ float sqrt(float x) { return sqrtf(x); }
+  static __device__ __forceinline__ f3 vdiv(f3 a, float s) { return a / s; }
+  static __device__ __forceinline__ float sq_len(f3 a) {  // length(a) * length(a)
+    float l = length(a);
+    return l * l;
+  }
+};
+
+template <>
+struct Ar<true> {
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float div(float a, float b) { return a * rcp(b); }
+  static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ f3 vdiv(f3 a, float s) { return a * rcp(s); }
+  static __device__ __forceinline__ float sq_len(f3 a) { return dot(a, a); }
+};
+
 // distanceConstraints, PBD form (integrator.py:65-109), all edges active.
 // On success returns 1 and e = edgeDP; the caller applies dP[a] -= e*wa and
 // dP[b] += e*wb and counts both ends.
+template <bool FAST>
 __device__ __forceinline__ int distance_pbd(float4 A, float4 B, float rest, float kS, f3* e) {
+  using M = Ar<FAST>;
   if (rest < kFeps) return 0;
   f3 dir = xyz(B) - xyz(A);
-  float len = length(dir);
+  float len = M::sqrt(dot(dir, dir));
   if (len < kFeps) return 0;
   float w = A.w + B.w;
   if (w < kFeps) return 0;
   float C = len - rest;
-  *e = ((dir / len) * (-C)) * kS / w;
+  if (FAST) {
+    *e = dir * (M::div(-C * kS, w) * M::div(1.0f, len));
+  } else {
+    *e = ((dir / len) * (-C)) * kS / w;
+  }
   return 1;
 }
 
 // distanceConstraintsXPBD (integrator.py:191-240); alpha = compliance/dt^2.
 // *lam is read and updated in place.
+template <bool FAST>
 __device__ __forceinline__ int distance_xpbd(float4 A, float4 B, float rest, float alpha, float* lam, f3* e) {
+  using M = Ar<FAST>;
   if (rest < kFeps) return 0;
   f3 dir = xyz(B) - xyz(A);
-  float len = length(dir);
+  float len = M::sqrt(dot(dir, dir));
   if (len < kFeps) return 0;
   float w = A.w + B.w;
   if (w < kFeps) return 0;
   float C = len - rest;
-  float dl = -(C + alpha * (*lam)) / (w + alpha);
+  float dl = M::div(-(C + alpha * (*lam)), w + alpha);
   *lam = *lam + dl;
-  *e = normalize(dir) * dl;
+  *e = M::vdiv(dir, len) * dl;  // normalize(dir) * dlambda
   return 1;
 }
 
@@ -60,24 +101,26 @@ __device__ __forceinline__ VolTerms vol_terms(const float4 P[4]) {
 }
 
 // volumeConstraints, PBD form (integrator.py:444-504): |V|, C = V0 - |V|.
+template <bool FAST>
 __device__ __forceinline__ int volume_pbd(const float4 P[4], float rest, float kS, f3 d[4]) {
   VolTerms t = vol_terms(P);
-  float volume = fabsf(t.six_vol) / 6.0f;
+  float volume = FAST ? fabsf(t.six_vol) * (1.0f / 6.0f) : fabsf(t.six_vol) / 6.0f;
   if (fabsf(t.w) < kFeps) return 0;
   float C = rest - volume;
-  float dl = -kS * C / t.w;
+  float dl = Ar<FAST>::div(-kS * C, t.w);
 #pragma unroll
   for (int i = 0; i < 4; ++i) d[i] = (t.q[i] * dl) * P[i].w;
   return 1;
 }
 
 // volumeConstraintsXPBD (integrator.py:507-576): signed V, C = V - V0.
+template <bool FAST>
 __device__ __forceinline__ int volume_xpbd(const float4 P[4], float rest, float alpha, float* lam, f3 d[4]) {
   VolTerms t = vol_terms(P);
-  float volume = t.six_vol / 6.0f;
+  float volume = FAST ? t.six_vol * (1.0f / 6.0f) : t.six_vol / 6.0f;
   if (fabsf(t.w) < kFeps) return 0;
   float C = volume - rest;
-  float dl = -(C + alpha * (*lam)) / (t.w + alpha);
+  float dl = Ar<FAST>::div(-(C + alpha * (*lam)), t.w + alpha);
   *lam = *lam + dl;
 #pragma unroll
   for (int i = 0; i < 4; ++i) d[i] = (t.q[i] * dl) * P[i].w;
@@ -105,8 +148,10 @@ __host__ __device__ inline NHConst nh_const(float volC, float devC, float dT) {
 // with gradient rows of transpose(mat33(F1xF2, F2xF0, F0xF1) * M^T), then
 // deviatoric |F|_F with gradient rows of transpose(F * M^T / |F|); both
 // deltas from the same positions; lambda is not carried.
+template <bool FAST>
 __device__ __forceinline__ int neohookean(const float4 P[4], float rest, const float M[9], const NHConst& k,
                                           f3 d[4]) {
+  using A = Ar<FAST>;
   f3 a = xyz(P[0]);
   f3 p1 = xyz(P[1]) - a, p2 = xyz(P[2]) - a, p3 = xyz(P[3]) - a;
   const float mA = P[0].w, mB = P[1].w, mC = P[2].w, mD = P[3].w;
@@ -142,19 +187,27 @@ __device__ __forceinline__ int neohookean(const float4 P[4], float rest, const f
   f3 g0 = ((-g1) - g2) - g3;
   float volume = dot(F0, cross(F1, F2));
   float C = volume - 1.0f - k.gamma;
-  float l, w = 0.0f;
-  l = length(g0); w = w + l * l * mA;
-  l = length(g1); w = w + l * l * mB;
-  l = length(g2); w = w + l * l * mC;
-  l = length(g3); w = w + l * l * mD;
+  float w = 0.0f;
+  w = w + A::sq_len(g0) * mA;
+  w = w + A::sq_len(g1) * mB;
+  w = w + A::sq_len(g2) * mC;
+  w = w + A::sq_len(g3) * mD;
   if (w < kFeps) return 0;
-  float alpha = k.alpha_h / rest;
-  float dl = -C / (w + alpha);
+  const float inv_rest = FAST ? __builtin_amdgcn_rcpf(rest) : 0.0f;
+  float alpha = FAST ? k.alpha_h * inv_rest : k.alpha_h / rest;
+  float dl = A::div(-C, w + alpha);
   f3 dA = (g0 * dl) * mA, dB = (g1 * dl) * mB, dC = (g2 * dl) * mC, dD = (g3 * dl) * mD;
-  float l0 = length(F0), l1 = length(F1), l2 = length(F2);
-  float rs = sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
-  float rsi = 1.0f / rs;
-  C = rs;
+  float rsi;
+  if (FAST) {
+    const float r2 = (dot(F0, F0) + dot(F1, F1)) + dot(F2, F2);
+    rsi = __builtin_amdgcn_rsqf(r2);
+    C = r2 * rsi;
+  } else {
+    float l0 = length(F0), l1 = length(F1), l2 = length(F2);
+    float rs = sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
+    rsi = 1.0f / rs;
+    C = rs;
+  }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -170,13 +223,13 @@ __device__ __forceinline__ int neohookean(const float4 P[4], float rest, const f
   g3 = mk(g[2], g[5], g[8]);
   g0 = ((-g1) - g2) - g3;
   w = 0.0f;
-  l = length(g0); w = w + l * l * mA;
-  l = length(g1); w = w + l * l * mB;
-  l = length(g2); w = w + l * l * mC;
-  l = length(g3); w = w + l * l * mD;
+  w = w + A::sq_len(g0) * mA;
+  w = w + A::sq_len(g1) * mB;
+  w = w + A::sq_len(g2) * mC;
+  w = w + A::sq_len(g3) * mD;
   if (w < kFeps) return 0;
-  alpha = k.alpha_d / rest;
-  dl = -C / (w + alpha);
+  alpha = FAST ? k.alpha_d * inv_rest : k.alpha_d / rest;
+  dl = A::div(-C, w + alpha);
   d[0] = dA + (g0 * dl) * mA;
   d[1] = dB + (g1 * dl) * mB;
   d[2] = dC + (g2 * dl) * mC;
@@ -193,10 +246,12 @@ __device__ __forceinline__ f3 predict(f3 x, f3 v, float w, f3 g, float dt, float
 }
 
 // dragConstraintsDO (integrator.py:412-442): p -= |p-tip| * n * drag * ks / 2.
+template <bool FAST>
 __device__ __forceinline__ f3 grasp_pull(f3 p, f3 tip, float dr, float ks) {
   f3 dir = p - tip;
-  float len = length(dir);
+  float len = Ar<FAST>::sqrt(dot(dir, dir));
   if (len < kFeps) return p;
+  if (FAST) return p - dir * ((dr * ks) * 0.5f);  // |dir| * dir/|dir| = dir
   f3 e = (((normalize(dir) * len) * dr) * ks) / 2.0f;
   return p - e;
 }

@@ -150,10 +150,10 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
           f3 d[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
           uint32_t n = 0;
           if (prm.vol_mode == FFSRL_PBD) {
-            n += volume_pbd(Pq, rest, prm.vol_k, d);
+            n += volume_pbd<true>(Pq, rest, prm.vol_k, d);
           } else if (prm.vol_mode == FFSRL_XPBD) {
             float lam = CARRY ? b.lam_t[e.lam_tbase + t] : 0.0f;
-            n += volume_xpbd(Pq, rest, prm.vol_k, &lam, d);
+            n += volume_xpbd<true>(Pq, rest, prm.vol_k, &lam, d);
             if (CARRY) b.lam_t[e.lam_tbase + t] = lam;
           }
           if (prm.nh_on) {
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
 #pragma unroll
             for (int k = 0; k < 9; ++k) M[k] = b.tet_inv_rest[k * T + ti];
             f3 dn[4];
-            if (neohookean(Pq, rest, M, nhk, dn)) {
+            if (neohookean<true>(Pq, rest, M, nhk, dn)) {
               n += 1;
 #pragma unroll
               for (int k = 0; k < 4; ++k) d[k] = d[k] + dn[k];
@@ -189,10 +189,10 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
           f3 d;
           int ok;
           if (prm.dist_mode == FFSRL_PBD) {
-            ok = distance_pbd(A, B, rest, prm.dist_k, &d);
+            ok = distance_pbd<true>(A, B, rest, prm.dist_k, &d);
           } else {
             float lam = CARRY ? b.lam_e[e.lam_ebase + k] : 0.0f;
-            ok = distance_xpbd(A, B, rest, prm.dist_k, &lam, &d);
+            ok = distance_xpbd<true>(A, B, rest, prm.dist_k, &lam, &d);
             if (CARRY && ok) b.lam_e[e.lam_ebase + k] = lam;
           }
           if (ok) {
@@ -220,13 +220,13 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
           if (n > 0) {
             const float N = (float)n;
             const f3 d = mk((float)c.dpx[i] * c.q, (float)c.dpy[i] * c.q, (float)c.dpz[i] * c.q);
-            p = p + mk(d.x / N, d.y / N, d.z / N);
+            p = p + d * __builtin_amdgcn_rcpf(N);
           }
           c.dpx[i] = 0;
           c.dpy[i] = 0;
           c.dpz[i] = 0;
           const float dr = b.drag[vb + i];
-          if (dr != 0.0f) p = grasp_pull(p, tip, dr, vs.ks);
+          if (dr != 0.0f) p = grasp_pull<true>(p, tip, dr, vs.ks);
           if (!last_it) {
             st4(P, vb + i, with_w(p, p4.w));
             continue;
@@ -304,10 +304,10 @@ __global__ void k_volume(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, int6
   f3 d[4];
   int ok;
   if (prm.vol_mode == FFSRL_PBD) {
-    ok = volume_pbd(Pq, b.tet_rest_volume[ti], prm.vol_k, d);
+    ok = volume_pbd<false>(Pq, b.tet_rest_volume[ti], prm.vol_k, d);
   } else {
     float lam = b.lam_t ? b.lam_t[e.lam_tbase + t] : 0.0f;
-    ok = volume_xpbd(Pq, b.tet_rest_volume[ti], prm.vol_k, &lam, d);
+    ok = volume_xpbd<false>(Pq, b.tet_rest_volume[ti], prm.vol_k, &lam, d);
     if (b.lam_t && ok) b.lam_t[e.lam_tbase + t] = lam;
   }
   if (!ok) return;
@@ -329,7 +329,7 @@ __global__ void k_neohookean(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, 
 #pragma unroll
   for (int k = 0; k < 9; ++k) M[k] = b.tet_inv_rest[k * T + ti];
   f3 d[4];
-  if (!neohookean(Pq, b.tet_rest_volume[ti], M, nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt), d)) return;
+  if (!neohookean<false>(Pq, b.tet_rest_volume[ti], M, nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt), d)) return;
   const int ids[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -347,10 +347,10 @@ __global__ void k_distance(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, in
   f3 d;
   int ok;
   if (prm.dist_mode == FFSRL_PBD) {
-    ok = distance_pbd(A, B, b.edge_rest[ei], prm.dist_k, &d);
+    ok = distance_pbd<false>(A, B, b.edge_rest[ei], prm.dist_k, &d);
   } else {
     float lam = b.lam_e ? b.lam_e[e.lam_ebase + k] : 0.0f;
-    ok = distance_xpbd(A, B, b.edge_rest[ei], prm.dist_k, &lam, &d);
+    ok = distance_xpbd<false>(A, B, b.edge_rest[ei], prm.dist_k, &lam, &d);
     if (b.lam_e && ok) b.lam_e[e.lam_ebase + k] = lam;
   }
   if (!ok) return;
@@ -384,7 +384,7 @@ __global__ void k_grasp(ffsrl_batch b, ffsrl_params prm, int env, int64_t vb, in
   if (dr == 0.0f) return;
   const float* t = b.tips + (env * 4 + 2) * 3;
   float4 p = ld4(b.p, vb + i);
-  st4(b.p, vb + i, with_w(grasp_pull(xyz(p), mk(t[0], t[1], t[2]), dr, prm.drag_ks), p.w));
+  st4(b.p, vb + i, with_w(grasp_pull<false>(xyz(p), mk(t[0], t[1], t[2]), dr, prm.drag_ks), p.w));
 }
 
 __global__ void k_bounds_integrate(ffsrl_batch b, ffsrl_params prm, int64_t vb, int nv) {

@@ -9,12 +9,14 @@
 // order, so no ray buffer goes to HBM.  Traversal order, slab test, the
 // Moller-Trumbore test and the strict "0 < t < tMin" acceptance follow the
 // reference expression by expression, so hit triangles are bit-exact.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace ffsrl {
 
-constexpr int kStack = 32;
-constexpr int kWaves = 4;
+constexpr int kStack = 32;  // deepest supported BVH (levels); the reference's stack size
+constexpr int kWaves = 2;   // wavefronts per block: small blocks release LDS early
 
 __device__ __forceinline__ int isect_aabb(f3 o, f3 d, f3 lo, f3 hi, float* tmin_out) {
   const float t1 = (lo.x - o.x) / d.x, t2 = (hi.x - o.x) / d.x;
@@ -24,6 +26,29 @@ __device__ __forceinline__ int isect_aabb(f3 o, f3 d, f3 lo, f3 hi, float* tmin_
   const float tmax = fmin_(fmin_(fmax_(t1, t2), fmax_(t3, t4)), fmax_(t5, t6));
   *tmin_out = tmin;
   return (tmax >= 0.0f) && (tmax >= tmin);
+}
+
+// The traversal only needs the decision "hit && tmin < tMin" of intersectAABB.
+// It is taken from products with per-ray reciprocals, (lo - o) * (1/d), which
+// are within 3 ulp of the quotients (lo - o) / d; special values (d = 0) give
+// the same infinities and NaNs either way.  When a comparison of the decision
+// is closer than 1e-6 relative, or a slab value is not finite, the decision is
+// recomputed with the exact quotients.  Either way the decision equals the
+// one of the division form, so the visited nodes and hits are unchanged.
+__device__ __forceinline__ bool close_rel(float a, float b) { return fabsf(a - b) <= 1e-6f * (fabsf(a) + fabsf(b)); }
+
+__device__ __forceinline__ bool visit_node(f3 o, f3 d, f3 inv, f3 lo, f3 hi, float tMin) {
+  const float t1 = (lo.x - o.x) * inv.x, t2 = (hi.x - o.x) * inv.x;
+  const float t3 = (lo.y - o.y) * inv.y, t4 = (hi.y - o.y) * inv.y;
+  const float t5 = (lo.z - o.z) * inv.z, t6 = (hi.z - o.z) * inv.z;
+  const float tmin = fmax_(fmax_(fmin_(t1, t2), fmin_(t3, t4)), fmin_(t5, t6));
+  const float tmax = fmin_(fmin_(fmax_(t1, t2), fmax_(t3, t4)), fmax_(t5, t6));
+  const bool finite = __builtin_isfinite(t1) && __builtin_isfinite(t2) && __builtin_isfinite(t3) &&
+                      __builtin_isfinite(t4) && __builtin_isfinite(t5) && __builtin_isfinite(t6);
+  const bool robust = finite && fabsf(tmax) > 1e-30f && !close_rel(tmax, tmin) && !close_rel(tmin, tMin);
+  if (robust) return (tmax >= 0.0f) && (tmax >= tmin) && (tmin < tMin);
+  float tb;
+  return isect_aabb(o, d, lo, hi, &tb) && tb < tMin;
 }
 
 __device__ __forceinline__ float mt_isect(f3 a, f3 b, f3 c, f3 o, f3 d) {
@@ -51,8 +76,9 @@ struct RenderOut {
   float* thit;
 };
 
-__global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_camera cam, int env_begin, RenderOut out) {
-  __shared__ int stack[kWaves][kStack][64];
+__global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_camera cam, int env_begin, RenderOut out,
+                                                        int depth) {
+  extern __shared__ int stack[];  // [kWaves][depth][64]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int RW = 2 * cam.width, RH = 2 * cam.height;
   const int tiles_x = RW >> 3, tiles = tiles_x * (RH >> 3);
@@ -65,8 +91,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
   const int64_t vb = e.vbase, sb = e.boundbase, fb = e.fbase;
   const f3 o = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
   const f3 d = xyz(ld4(cam.dirs, r));
+  const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
 
-  int* stk = &stack[wave][0][lane];
+  int* stk = stack + wave * depth * 64 + lane;
   stk[0] = 0;
   int count = 1, best = -1;
   float tMin = 1e6f;
@@ -74,8 +101,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
     const int node = stk[(count - 1) * 64];
     count -= 1;
     const float4 lo = ld4(b.bvh_lo, sb + node), hi = ld4(b.bvh_hi, sb + node);
-    float tb;
-    if (isect_aabb(o, d, xyz(lo), xyz(hi), &tb) && tb < tMin) {
+    if (visit_node(o, d, inv, xyz(lo), xyz(hi), tMin)) {
       const int n = __float_as_int(hi.w);
       if (n > 0) {
         const int first = __float_as_int(lo.w);
@@ -87,7 +113,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
             best = first + k;
           }
         }
-      } else if (count + 2 <= kStack) {  // depth checked at admission: never false
+      } else if (count + 2 <= depth) {  // depth = deepest BVH level + 1: never false
         const int l = __float_as_int(lo.w);
         stk[count * 64] = l;
         stk[(count + 1) * 64] = l + 1;
@@ -212,8 +238,18 @@ int ffsrl_render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t eb, int3
   if (ee - eb > 65535) return fail(FFSRL_E_LIMIT, "at most 65535 environments per render call");
   const int tiles = (2 * cam->width / 8) * (2 * cam->height / 8);
   RenderOut out{depth, rgb, normals, vcolor, tcolor, hit, thit};
-  hipLaunchKernelGGL(k_render, dim3((tiles + kWaves - 1) / kWaves, ee - eb), dim3(64 * kWaves), 0, as_stream(stream),
-                     *b, *cam, eb, out);
+  // Stack entries per ray: the deepest BVH of the range (levels = max level + 1
+  // bounds the stack of this right-first DFS); kStack without a host mirror.
+  int stack_depth = kStack;
+  if (b->host_envs) {
+    stack_depth = 2;
+    for (int i = eb; i < ee; ++i) stack_depth = std::max(stack_depth, (int)b->host_envs[i].nlvl + 1);
+    if (stack_depth > kStack) return fail(FFSRL_E_LIMIT, "BVH with %d levels exceeds the %d-entry ray stack",
+                                          stack_depth - 1, kStack);
+  }
+  const int lds = kWaves * stack_depth * 64 * (int)sizeof(int);
+  hipLaunchKernelGGL(k_render, dim3((tiles + kWaves - 1) / kWaves, ee - eb), dim3(64 * kWaves), lds, as_stream(stream),
+                     *b, *cam, eb, out, stack_depth);
   FFSRL_HIP(hipGetLastError());
   return 0;
 }
