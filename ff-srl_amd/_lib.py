@@ -9,7 +9,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libffsrl.so")
+# FFSRL_LIB selects another build of the same library (e.g. a variant built
+# with different tuning flags); the default is the in-tree build.
+LIB_PATH = os.environ.get("FFSRL_LIB") or os.path.join(HERE, "_build", "libffsrl.so")
 
 OFF, PBD, XPBD = 0, 1, 2
 E_ARG, E_LIMIT, E_STATE = -1, -2, -3

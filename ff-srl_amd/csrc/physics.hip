@@ -90,6 +90,184 @@ struct VertexStep {
   float dt, damp1, inv_dt, ground, ks;
 };
 
+// Topology of one tet / edge of the projection loop.
+struct TetIn {
+  int4 q;
+  float rest;
+};
+struct EdgeIn {
+  int2 q;
+  float rest;
+};
+
+__device__ __forceinline__ TetIn load_tet(const ffsrl_batch& b, int64_t ti) {
+  TetIn t;
+  t.q = ldi4(b.tets, ti);
+  t.rest = b.tet_rest_volume[ti];
+  return t;
+}
+__device__ __forceinline__ EdgeIn load_edge(const ffsrl_batch& b, int64_t ei) {
+  EdgeIn t;
+  t.q = ldi2(b.edges, ei);
+  t.rest = b.edge_rest[ei];
+  return t;
+}
+
+// Positions and inverse rest matrix of one tet, gathered ahead of use.
+struct TetG {
+  float4 P[4];
+  float M[9];
+};
+
+__device__ __forceinline__ TetG gather_tet(const ffsrl_batch& b, const float* P, int64_t vb, const TetIn& t,
+                                           int64_t ti, int64_t T, int nh) {
+  TetG g;
+  if (nh) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g.M[k] = b.tet_inv_rest[k * T + ti];
+  }
+  g.P[0] = ld4(P, vb + t.q.x);
+  g.P[1] = ld4(P, vb + t.q.y);
+  g.P[2] = ld4(P, vb + t.q.z);
+  g.P[3] = ld4(P, vb + t.q.w);
+  return g;
+}
+
+// Volume + Neo-Hookean deltas of one tet, summed into LDS.
+template <bool CARRY>
+__device__ __forceinline__ void process_tet(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
+                                            const FrameCtx& c, const NHConst& nhk, const TetIn& tc, const TetG& g,
+                                            int j) {
+  f3 d[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
+  uint32_t n = 0;
+  if (prm.vol_mode == FFSRL_PBD) {
+    n += volume_pbd<true>(g.P, tc.rest, prm.vol_k, d);
+  } else if (prm.vol_mode == FFSRL_XPBD) {
+    float lam = CARRY ? b.lam_t[e.lam_tbase + j] : 0.0f;
+    n += volume_xpbd<true>(g.P, tc.rest, prm.vol_k, &lam, d);
+    if (CARRY) b.lam_t[e.lam_tbase + j] = lam;
+  }
+  if (prm.nh_on) {
+    f3 dn[4];
+    if (neohookean<true>(g.P, tc.rest, g.M, nhk, dn)) {
+      n += 1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) d[k] = d[k] + dn[k];
+    }
+  }
+  if (n) {
+    lds_add(c, tc.q.x, d[0]);
+    lds_add(c, tc.q.y, d[1]);
+    lds_add(c, tc.q.z, d[2]);
+    lds_add(c, tc.q.w, d[3]);
+    lds_count(c, tc.q.x, n);
+    lds_count(c, tc.q.y, n);
+    lds_count(c, tc.q.z, n);
+    lds_count(c, tc.q.w, n);
+  }
+}
+
+// Distance delta of one edge, summed into LDS.
+template <bool CARRY>
+__device__ __forceinline__ void process_edge(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
+                                             const FrameCtx& c, const EdgeIn& ec, float4 A, float4 B, int j) {
+  f3 d;
+  int ok;
+  if (prm.dist_mode == FFSRL_PBD) {
+    ok = distance_pbd<true>(A, B, ec.rest, prm.dist_k, &d);
+  } else {
+    float lam = CARRY ? b.lam_e[e.lam_ebase + j] : 0.0f;
+    ok = distance_xpbd<true>(A, B, ec.rest, prm.dist_k, &lam, &d);
+    if (CARRY && ok) b.lam_e[e.lam_ebase + j] = lam;
+  }
+  if (ok) {
+    lds_sub(c, ec.q.x, d * A.w);
+    lds_add(c, ec.q.y, d * B.w);
+    lds_count(c, ec.q.x, 1u);
+    lds_count(c, ec.q.y, 1u);
+  }
+}
+
+// Work of one stride for one thread: tet j and edge j of the env.
+struct Slot {
+  TetIn t;
+  TetG g;
+  EdgeIn e;
+  float4 A, B;
+};
+
+// One projection pass over the env's tets and edges.  Thread j handles tet j
+// and edge j of each stride of BLOCK.  The pass is software-pipelined over
+// rotating slots (the loop is unrolled over the slots, so "next" and
+// "current" are different registers and no load result is copied before it
+// is consumed):
+//   BLOCK <= 512 (256 VGPRs per lane): three slots; indices are loaded two
+//     strides ahead and positions / inverse rest matrices one stride ahead;
+//   BLOCK == 1024 (128 VGPRs per lane): two slots; indices are loaded one
+//     stride ahead and positions at use.
+template <int BLOCK, bool CARRY>
+__device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
+                                        const FrameCtx& c, const NHConst& nhk, const float* P, int64_t vb, int64_t T,
+                                        int nt, int ne) {
+  const int nmax = nt > ne ? nt : ne;
+  const int j0 = threadIdx.x;
+  auto ids = [&](Slot& s, int jj) {
+    if (jj < nt) s.t = load_tet(b, e.tbase + jj);
+    if (jj < ne) s.e = load_edge(b, e.ebase + jj);
+  };
+  auto gat = [&](Slot& s, int jj) {
+    if (jj < nt) s.g = gather_tet(b, P, vb, s.t, e.tbase + jj, T, prm.nh_on);
+    if (jj < ne) {
+      s.A = ld4(P, vb + s.e.q.x);
+      s.B = ld4(P, vb + s.e.q.y);
+    }
+  };
+  auto run = [&](const Slot& s, int jj) {
+    if (jj < nt) process_tet<CARRY>(b, prm, e, c, nhk, s.t, s.g, jj);
+    if (jj < ne) process_edge<CARRY>(b, prm, e, c, s.e, s.A, s.B, jj);
+  };
+  if constexpr (BLOCK <= 512) {
+    Slot s0, s1, s2;
+    ids(s0, j0);
+    ids(s1, j0 + BLOCK);
+    gat(s0, j0);
+    for (int j = j0; j < nmax; j += 3 * BLOCK) {
+      ids(s2, j + 2 * BLOCK);
+      gat(s1, j + BLOCK);
+      run(s0, j);
+      if (j + BLOCK >= nmax) break;
+      ids(s0, j + 3 * BLOCK);
+      gat(s2, j + 2 * BLOCK);
+      run(s1, j + BLOCK);
+      if (j + 2 * BLOCK >= nmax) break;
+      ids(s1, j + 4 * BLOCK);
+      gat(s0, j + 3 * BLOCK);
+      run(s2, j + 2 * BLOCK);
+    }
+  } else {
+    // Gather and compute one item at a time to stay within 128 VGPRs.
+    auto gr = [&](const Slot& s, int jj) {
+      if (jj < nt) {
+        TetG g = gather_tet(b, P, vb, s.t, e.tbase + jj, T, prm.nh_on);
+        process_tet<CARRY>(b, prm, e, c, nhk, s.t, g, jj);
+      }
+      if (jj < ne) {
+        float4 A = ld4(P, vb + s.e.q.x), B = ld4(P, vb + s.e.q.y);
+        process_edge<CARRY>(b, prm, e, c, s.e, A, B, jj);
+      }
+    };
+    Slot s0, s1;
+    ids(s0, j0);
+    for (int j = j0; j < nmax; j += 2 * BLOCK) {
+      ids(s1, j + BLOCK);
+      gr(s0, j);
+      if (j + BLOCK >= nmax) break;
+      ids(s0, j + 2 * BLOCK);
+      gr(s1, j + BLOCK);
+    }
+  }
+}
+
 template <int BLOCK, bool CARRY>
 __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm, const int32_t* order,
                                                  int refit) {
@@ -131,6 +309,8 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
   __syncthreads();
 
   const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
+  const bool do_tets = prm.vol_mode != FFSRL_OFF || prm.nh_on;
+  const bool do_edges = prm.dist_mode != FFSRL_OFF;
   for (int s = 0; s < prm.substeps; ++s) {
     if (CARRY && reset_lambda) {
       // Each thread zeroes the lambdas of exactly the constraints it projects.
@@ -140,69 +320,8 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
         for (int k = tid; k < e.ne; k += BLOCK) b.lam_e[e.lam_ebase + k] = 0.0f;
     }
     for (int it = 0; it < prm.iterations; ++it) {
-      // ---- tets: volume (PBD / XPBD) and Neo-Hookean ----
-      if (prm.vol_mode != FFSRL_OFF || prm.nh_on) {
-        for (int t = tid; t < e.nt; t += BLOCK) {
-          const int64_t ti = e.tbase + t;
-          const int4 q = ldi4(b.tets, ti);
-          float4 Pq[4] = {ld4(P, vb + q.x), ld4(P, vb + q.y), ld4(P, vb + q.z), ld4(P, vb + q.w)};
-          const float rest = b.tet_rest_volume[ti];
-          f3 d[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
-          uint32_t n = 0;
-          if (prm.vol_mode == FFSRL_PBD) {
-            n += volume_pbd<true>(Pq, rest, prm.vol_k, d);
-          } else if (prm.vol_mode == FFSRL_XPBD) {
-            float lam = CARRY ? b.lam_t[e.lam_tbase + t] : 0.0f;
-            n += volume_xpbd<true>(Pq, rest, prm.vol_k, &lam, d);
-            if (CARRY) b.lam_t[e.lam_tbase + t] = lam;
-          }
-          if (prm.nh_on) {
-            float M[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) M[k] = b.tet_inv_rest[k * T + ti];
-            f3 dn[4];
-            if (neohookean<true>(Pq, rest, M, nhk, dn)) {
-              n += 1;
-#pragma unroll
-              for (int k = 0; k < 4; ++k) d[k] = d[k] + dn[k];
-            }
-          }
-          if (n) {
-            lds_add(c, q.x, d[0]);
-            lds_add(c, q.y, d[1]);
-            lds_add(c, q.z, d[2]);
-            lds_add(c, q.w, d[3]);
-            lds_count(c, q.x, n);
-            lds_count(c, q.y, n);
-            lds_count(c, q.z, n);
-            lds_count(c, q.w, n);
-          }
-        }
-      }
-      // ---- edges: distance (PBD / XPBD) ----
-      if (prm.dist_mode != FFSRL_OFF) {
-        for (int k = tid; k < e.ne; k += BLOCK) {
-          const int64_t ei = e.ebase + k;
-          const int2 q = ldi2(b.edges, ei);
-          const float4 A = ld4(P, vb + q.x), B = ld4(P, vb + q.y);
-          const float rest = b.edge_rest[ei];
-          f3 d;
-          int ok;
-          if (prm.dist_mode == FFSRL_PBD) {
-            ok = distance_pbd<true>(A, B, rest, prm.dist_k, &d);
-          } else {
-            float lam = CARRY ? b.lam_e[e.lam_ebase + k] : 0.0f;
-            ok = distance_xpbd<true>(A, B, rest, prm.dist_k, &lam, &d);
-            if (CARRY && ok) b.lam_e[e.lam_ebase + k] = lam;
-          }
-          if (ok) {
-            lds_sub(c, q.x, d * A.w);
-            lds_add(c, q.y, d * B.w);
-            lds_count(c, q.x, 1u);
-            lds_count(c, q.y, 1u);
-          }
-        }
-      }
+      // ---- projection: tets (volume + Neo-Hookean) and edges (distance) ----
+      project<BLOCK, CARRY>(b, prm, e, c, nhk, P, vb, T, do_tets ? e.nt : 0, do_edges ? e.ne : 0);
       __syncthreads();
       // ---- apply (+ grasp; + ground, integrate and next predict) ----
       const bool last_it = (it == prm.iterations - 1);

@@ -41,14 +41,14 @@ def _c1_model(pkg, n=10, frames_velocity_seed=7):
     return mesh, v0
 
 
-@pytest.mark.parametrize("form", ["batched", "plain"])
+@pytest.mark.parametrize("form", ["batched", "batched512", "batched256", "plain"])
 def test_c1_cube_100_frames(pkg, orc, gpu, form):
     mesh, v0 = _c1_model(pkg)
     model = pkg.SimModelDO([mesh], [0], device=gpu, initialVelocity=v0, **C1)
-    integ = pkg.SimIntegratorDO(gpu)
+    integ = pkg.SimIntegratorDO(gpu, block={"batched512": 512, "batched256": 256}.get(form, 1024))
     frames = 100
     for _ in range(frames):
-        if form == "batched":
+        if form.startswith("batched"):
             integ.stepModel(model)
         else:
             integ.stepModelPlain(model)
@@ -245,14 +245,15 @@ def test_single_substep_kernel_parity(pkg, orc, gpu, form, iters, cset):
     assert moved > 1e-4                                   # the step did something
 
 
-def test_mixed_sizes_alone_equals_in_mix(pkg, gpu):
+@pytest.mark.parametrize("block", [1024, 512])
+def test_mixed_sizes_alone_equals_in_mix(pkg, gpu, block):
     """Each env's result does not depend on the envs batched with it."""
     sc = pkg.scenes
     specs = [("ellipsoid", 1200), ("box", 5000), ("ellipsoid", 9000), ("box", 2000), ("ellipsoid", 3000)]
     meshes = [sc.solid(s, t, seed=i) for i, (s, t) in enumerate(specs)]
     kw = dict(NH, distance="xpbd", volume="xpbd", distanceCompliance=1e-6, volumeComplianceXPBD=0.0)
     mix = pkg.SimModelDO(meshes, [4, 0, 3, 1, 2, 0], device=gpu, **kw)
-    integ = pkg.SimIntegratorDO(gpu)
+    integ = pkg.SimIntegratorDO(gpu, block=block)
     for _ in range(10):
         integ.stepModel(mix, refit=True)
     for e, k in enumerate([4, 0, 3, 1, 2, 0]):
