@@ -2,9 +2,10 @@
 //
 // Batched form (ffsrl_frame): one workgroup owns one environment for a whole
 // frame.  The Jacobi deltas and constraint counts of the env's vertices are
-// reduced in LDS with ds_add (12 B of dP + a 16-bit counter per vertex), so
-// no global atomics are issued; phases (predict, project, apply) are separated
-// by workgroup barriers instead of kernel boundaries, and the integrate of
+// reduced in LDS with 64-bit integer ds_add (16 B per vertex: fixed-point dP
+// and the count), so no global atomics are issued; phases (predict, project,
+// apply) are separated by workgroup barriers instead of kernel boundaries, and
+// the integrate of
 // substep s is fused with the predict of substep s+1.  The BVH refit of the
 // env runs at the end of the same workgroup.  Environments of any mix of sizes
 // run in one launch; dispatching them largest-first balances the CUs.
@@ -41,11 +42,8 @@ int check_hip(hipError_t e, const char* what) {
 
 // LDS budget of one workgroup on gfx950.
 constexpr int kLdsBytes = 160 * 1024;
-constexpr int lds_bytes_for(int nv) {
-  int cap = (nv + 1) & ~1;
-  return cap * 12 + (cap / 2) * 4;
-}
-constexpr int kMaxFrameVertices = (kLdsBytes / 14) & ~1;
+constexpr int lds_bytes_for(int nv) { return nv * 16; }
+constexpr int kMaxFrameVertices = kLdsBytes / 16;
 
 // Jacobi sums in LDS are fixed point: each delta component is rounded to the
 // nearest multiple of the env's quantum q = 2^dp_exp (about 2^-27 of the mesh
@@ -53,14 +51,19 @@ constexpr int kMaxFrameVertices = (kLdsBytes / 14) & ~1;
 // the LDS's full rate on gfx950 (float LDS atomics are ~30x slower), and
 // integer sums do not depend on the order of the additions, so the frame is
 // deterministic.  Each term is clamped to +-2^24 q (D/16 .. D/8), so sums of
-// up to 128 terms cannot overflow; the rounding error per term is <= q/2.
+// up to 128 terms stay inside int32; the rounding error per term is <= q/2.
+//
+// A vertex owns 16 B of LDS, two 64-bit words updated with ds_add_u64:
+//   w0 = X + Y * 2^32,  w1 = Z + count * 2^32   (two's complement, mod 2^64)
+// where X, Y, Z are the int32 fixed-point sums.  Integer addition mod 2^64 is
+// exact and associative, so w0 and w1 hold exactly these values whatever the
+// order; since |X| < 2^31, X is the low half of w0 read as int32 and
+// Y = high half - (X < 0 ? -1 : 0) (likewise Z and the count).  One 64-bit
+// atomic thus carries two sums at one address.
 struct FrameCtx {
-  int32_t* dpx;
-  int32_t* dpy;
-  int32_t* dpz;
-  uint32_t* cnt;  // two 16-bit counters per word
-  float inv_q;    // 2^-dp_exp
-  float q;        // 2^dp_exp
+  unsigned long long* acc;  // 2 words per vertex
+  float inv_q;              // 2^-dp_exp
+  float q;                  // 2^dp_exp
 };
 
 __device__ __forceinline__ int32_t to_fix(float x, float inv_q) {
@@ -68,18 +71,22 @@ __device__ __forceinline__ int32_t to_fix(float x, float inv_q) {
   v = fminf(fmaxf(v, -16777216.0f), 16777216.0f);
   return (int32_t)v;
 }
-__device__ __forceinline__ void lds_add(const FrameCtx& c, int i, f3 d) {
-  atomicAdd(&c.dpx[i], to_fix(d.x, c.inv_q));
-  atomicAdd(&c.dpy[i], to_fix(d.y, c.inv_q));
-  atomicAdd(&c.dpz[i], to_fix(d.z, c.inv_q));
+// (lo + hi * 2^32) as a 64-bit two's complement addend, lo sign-extended.
+__device__ __forceinline__ unsigned long long pack2(int32_t lo, int32_t hi) {
+  const uint32_t h = (uint32_t)hi + (uint32_t)(lo >> 31);
+  return ((unsigned long long)h << 32) | (uint32_t)lo;
 }
-__device__ __forceinline__ void lds_sub(const FrameCtx& c, int i, f3 d) {
-  atomicAdd(&c.dpx[i], -to_fix(d.x, c.inv_q));
-  atomicAdd(&c.dpy[i], -to_fix(d.y, c.inv_q));
-  atomicAdd(&c.dpz[i], -to_fix(d.z, c.inv_q));
+__device__ __forceinline__ void unpack2(unsigned long long w, int32_t* lo, int32_t* hi) {
+  const int32_t l = (int32_t)(uint32_t)w;
+  *lo = l;
+  *hi = (int32_t)((uint32_t)(w >> 32) - (uint32_t)(l >> 31));
 }
-__device__ __forceinline__ void lds_count(const FrameCtx& c, int i, uint32_t n) {
-  atomicAdd(&c.cnt[i >> 1], n << (16 * (i & 1)));
+// Adds sign * d (fixed point) and n to the count of vertex i.
+template <int SIGN>
+__device__ __forceinline__ void lds_acc(const FrameCtx& c, int i, f3 d, uint32_t n) {
+  const int32_t x = to_fix(d.x, c.inv_q), y = to_fix(d.y, c.inv_q), z = to_fix(d.z, c.inv_q);
+  atomicAdd(&c.acc[2 * i], pack2(SIGN * x, SIGN * y));
+  atomicAdd(&c.acc[2 * i + 1], pack2(SIGN * z, (int32_t)n));
 }
 
 __device__ __forceinline__ float4 with_w(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
@@ -156,14 +163,10 @@ __device__ __forceinline__ void process_tet(const ffsrl_batch& b, const ffsrl_pa
     }
   }
   if (n) {
-    lds_add(c, tc.q.x, d[0]);
-    lds_add(c, tc.q.y, d[1]);
-    lds_add(c, tc.q.z, d[2]);
-    lds_add(c, tc.q.w, d[3]);
-    lds_count(c, tc.q.x, n);
-    lds_count(c, tc.q.y, n);
-    lds_count(c, tc.q.z, n);
-    lds_count(c, tc.q.w, n);
+    lds_acc<1>(c, tc.q.x, d[0], n);
+    lds_acc<1>(c, tc.q.y, d[1], n);
+    lds_acc<1>(c, tc.q.z, d[2], n);
+    lds_acc<1>(c, tc.q.w, d[3], n);
   }
 }
 
@@ -181,10 +184,8 @@ __device__ __forceinline__ void process_edge(const ffsrl_batch& b, const ffsrl_p
     if (CARRY && ok) b.lam_e[e.lam_ebase + j] = lam;
   }
   if (ok) {
-    lds_sub(c, ec.q.x, d * A.w);
-    lds_add(c, ec.q.y, d * B.w);
-    lds_count(c, ec.q.x, 1u);
-    lds_count(c, ec.q.y, 1u);
+    lds_acc<-1>(c, ec.q.x, d * A.w, 1u);
+    lds_acc<1>(c, ec.q.y, d * B.w, 1u);
   }
 }
 
@@ -271,13 +272,11 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
 template <int BLOCK, bool CARRY>
 __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm, const int32_t* order,
                                                  int refit) {
-  extern __shared__ int32_t lds[];
+  extern __shared__ unsigned long long lds[];
   const int env = order ? order[blockIdx.x] : (int)blockIdx.x;
   const ffsrl_env_desc e = b.envs[env];
   const int nv = e.nv;
-  const int cap = (nv + 1) & ~1;
-  FrameCtx c{lds, lds + cap, lds + 2 * cap, reinterpret_cast<uint32_t*>(lds + 3 * cap), ldexpf(1.0f, -e.dp_exp),
-             ldexpf(1.0f, e.dp_exp)};
+  FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
   const int tid = threadIdx.x;
   const int64_t vb = e.vbase;
   float* X = b.x;
@@ -295,12 +294,7 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
   const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
   const int64_t T = b.n_tet_total;
 
-  for (int i = tid; i < cap; i += BLOCK) {
-    c.dpx[i] = 0;
-    c.dpy[i] = 0;
-    c.dpz[i] = 0;
-  }
-  for (int i = tid; i < cap / 2; i += BLOCK) c.cnt[i] = 0u;
+  for (int i = tid; i < 2 * nv; i += BLOCK) c.acc[i] = 0ull;
   // predict of substep 0
   for (int i = tid; i < nv; i += BLOCK) {
     float4 x = ld4(X, vb + i), v = ld4(Vv, vb + i);
@@ -326,38 +320,32 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
       // ---- apply (+ grasp; + ground, integrate and next predict) ----
       const bool last_it = (it == prm.iterations - 1);
       const bool next_sub = (s + 1 < prm.substeps);
-      for (int j = tid; j < cap / 2; j += BLOCK) {
-        const uint32_t word = c.cnt[j];
-        c.cnt[j] = 0u;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int i = 2 * j + h;
-          if (i >= nv) break;
-          const uint32_t n = (word >> (16 * h)) & 0xffffu;
-          float4 p4 = ld4(P, vb + i);
-          f3 p = xyz(p4);
-          if (n > 0) {
-            const float N = (float)n;
-            const f3 d = mk((float)c.dpx[i] * c.q, (float)c.dpy[i] * c.q, (float)c.dpz[i] * c.q);
-            p = p + d * __builtin_amdgcn_rcpf(N);
-          }
-          c.dpx[i] = 0;
-          c.dpy[i] = 0;
-          c.dpz[i] = 0;
-          const float dr = b.drag[vb + i];
-          if (dr != 0.0f) p = grasp_pull<true>(p, tip, dr, vs.ks);
-          if (!last_it) {
-            st4(P, vb + i, with_w(p, p4.w));
-            continue;
-          }
-          if (p.y < vs.ground) p.y = vs.ground;
-          const float4 x4 = ld4(X, vb + i);
-          const f3 v = (p - xyz(x4)) * vs.inv_dt;
-          st4(X, vb + i, with_w(p, x4.w));
-          st4(Vv, vb + i, with_w(v, 0.0f));
-          if (next_sub) p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
-          st4(P, vb + i, with_w(p, p4.w));
+      for (int i = tid; i < nv; i += BLOCK) {
+        const unsigned long long w0 = c.acc[2 * i], w1 = c.acc[2 * i + 1];
+        c.acc[2 * i] = 0ull;
+        c.acc[2 * i + 1] = 0ull;
+        int32_t fx, fy, fz, n;
+        unpack2(w0, &fx, &fy);
+        unpack2(w1, &fz, &n);
+        float4 p4 = ld4(P, vb + i);
+        f3 p = xyz(p4);
+        if (n > 0) {
+          const float s = c.q * __builtin_amdgcn_rcpf((float)n);
+          p = p + mk((float)fx, (float)fy, (float)fz) * s;
         }
+        const float dr = b.drag[vb + i];
+        if (dr != 0.0f) p = grasp_pull<true>(p, tip, dr, vs.ks);
+        if (!last_it) {
+          st4(P, vb + i, with_w(p, p4.w));
+          continue;
+        }
+        if (p.y < vs.ground) p.y = vs.ground;
+        const float4 x4 = ld4(X, vb + i);
+        const f3 v = (p - xyz(x4)) * vs.inv_dt;
+        st4(X, vb + i, with_w(p, x4.w));
+        st4(Vv, vb + i, with_w(v, 0.0f));
+        if (next_sub) p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
+        st4(P, vb + i, with_w(p, p4.w));
       }
       __syncthreads();
     }
@@ -598,7 +586,7 @@ int ffsrl_frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* or
   if (refit && (!b->tris || !b->node_left || !b->level_nodes || !b->level_off || !b->bvh_lo || !b->bvh_hi))
     return fail(FFSRL_E_ARG, "refit requested without BVH arrays");
   if (b->n_env == 0) return 0;
-  const int lds = lds_bytes_for(b->max_nv < 2 ? 2 : b->max_nv);
+  const int lds = lds_bytes_for(b->max_nv < 1 ? 1 : b->max_nv);
   hipStream_t st = as_stream(stream);
   switch (block) {
     case 256: return launch_frame<256>(*b, *prm, order, refit, carry, lds, st);
