@@ -83,6 +83,7 @@ SIGNATURES = {
     "ffsrl_remap": [_vp, _vp, _vp, _i64, _vp],
     "ffsrl_bvh_build": [_vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _P(_i32), _P(_i32)],
     "ffsrl_color_constraints": [_vp, _i64, _i32, _i32, _vp, _P(_i32)],
+    "ffsrl_order_constraints": [_vp, _i64, _i32, _vp, _i32, _i32, _vp],
     "ffsrl_graph_begin": [_vp],
     "ffsrl_graph_end": [_vp, _P(_vp)],
     "ffsrl_graph_launch": [_vp, _vp],

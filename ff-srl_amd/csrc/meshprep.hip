@@ -1,14 +1,22 @@
-// Host-side mesh preparation: greedy graph colouring of constraints.
+// Host-side mesh preparation: the order in which the frame kernel visits a
+// mesh's constraints.
 //
-// Two constraints conflict when they share a vertex.  Sorting each mesh's
-// tets and edges by colour (stable within a colour) makes the 64 lanes of a
-// wavefront touch 64 disjoint vertex sets, so the LDS reductions of the
-// Jacobi deltas in the frame kernel never hit the same address twice within
-// one instruction.  (The reference accumulates with global atomics in
-// element order, integrator.py:105-109; the colouring only reorders the
-// summation.)
+// The kernel's Jacobi sums are integers, so the order changes neither the
+// result nor its reproducibility (the reference accumulates with global
+// atomics in element order, integrator.py:105-109); it only changes speed:
+//   * spatial locality: the position gathers of a wavefront should hit few
+//     cache lines, so constraints are sorted along a Morton curve of their
+//     centroids;
+//   * LDS address conflicts: two lanes of one ds_add to the same vertex are
+//     serialised, so within each chunk of consecutive constraints the order is
+//     a stable sort by a greedy colour (two constraints conflict when they
+//     share a vertex).
+// ffsrl_color_constraints colours a whole list; ffsrl_order_constraints
+// builds the frame kernel's order (Morton, then colour within chunks).
 #include <stdint.h>
 
+#include <algorithm>
+#include <numeric>
 #include <vector>
 
 #include "common.hpp"
@@ -40,6 +48,78 @@ int ffsrl_color_constraints(const int32_t* elems, int64_t n, int32_t arity, int3
     for (int k = 0; k < arity; ++k) used[(size_t)elems[i * arity + k] * kWords + c / 64] |= 1ull << (c % 64);
   }
   *num_colors = ncol;
+  return 0;
+}
+
+static uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+int ffsrl_order_constraints(const int32_t* elems, int64_t n, int32_t arity, const float* verts, int32_t nv,
+                            int32_t chunk, int64_t* perm) {
+  if (!elems || !verts || !perm || n < 0 || arity < 1 || arity > 8 || nv < 0 || chunk < 1)
+    return fail(FFSRL_E_ARG, "order_constraints: bad arguments");
+  float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int32_t v = 0; v < nv; ++v)
+    for (int a = 0; a < 3; ++a) {
+      const float x = verts[3 * v + a];
+      if (v == 0 || x < lo[a]) lo[a] = x;
+      if (v == 0 || x > hi[a]) hi[a] = x;
+    }
+  std::vector<uint32_t> key((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t k = 0;
+    for (int a = 0; a < 3; ++a) {
+      double c = 0.0;
+      for (int j = 0; j < arity; ++j) {
+        const int32_t v = elems[i * arity + j];
+        if (v < 0 || v >= nv) return fail(FFSRL_E_ARG, "order_constraints: vertex %d out of range", v);
+        c += verts[3 * v + a];
+      }
+      c /= arity;
+      const double ext = (double)hi[a] - lo[a];
+      const double t = ext > 0 ? (c - lo[a]) / ext : 0.0;
+      k |= spread10((uint32_t)std::min(1023.0, std::max(0.0, t * 1023.0))) << a;
+    }
+    key[(size_t)i] = k;
+  }
+  std::vector<int64_t> order((size_t)n);
+  std::iota(order.begin(), order.end(), int64_t{0});
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[(size_t)a] < key[(size_t)b]; });
+  // greedy colour within each chunk, then a stable sort of the chunk by colour
+  constexpr int kWords = 4;
+  std::vector<uint64_t> used((size_t)nv * kWords, 0);
+  std::vector<int32_t> col((size_t)std::min<int64_t>(n, chunk));
+  std::vector<int64_t> idx(col.size());
+  for (int64_t a = 0; a < n; a += chunk) {
+    const int64_t m = std::min<int64_t>(chunk, n - a);
+    for (int64_t i = 0; i < m; ++i) {
+      const int32_t* e = elems + order[(size_t)(a + i)] * arity;
+      uint64_t mask[kWords] = {0, 0, 0, 0};
+      for (int j = 0; j < arity; ++j)
+        for (int w = 0; w < kWords; ++w) mask[w] |= used[(size_t)e[j] * kWords + w];
+      int c = -1;
+      for (int w = 0; w < kWords && c < 0; ++w)
+        if (~mask[w]) c = w * 64 + __builtin_ctzll(~mask[w]);
+      if (c < 0) c = 64 * kWords;  // more colours than tracked: keep the Morton position
+      col[(size_t)i] = c;
+      if (c < 64 * kWords)
+        for (int j = 0; j < arity; ++j) used[(size_t)e[j] * kWords + c / 64] |= 1ull << (c % 64);
+    }
+    for (int64_t i = 0; i < m; ++i) {
+      const int32_t* e = elems + order[(size_t)(a + i)] * arity;
+      for (int j = 0; j < arity; ++j)
+        for (int w = 0; w < kWords; ++w) used[(size_t)e[j] * kWords + w] = 0;
+      idx[(size_t)i] = i;
+    }
+    std::stable_sort(idx.begin(), idx.begin() + m, [&](int64_t x, int64_t y) { return col[(size_t)x] < col[(size_t)y]; });
+    for (int64_t i = 0; i < m; ++i) perm[a + i] = order[(size_t)(a + idx[(size_t)i])];
+  }
   return 0;
 }
 

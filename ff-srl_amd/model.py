@@ -27,6 +27,7 @@ from ._lib import Batch, EnvDesc, ENV_DESC_FIELDS, OFF, PBD, XPBD, Params, lib
 from .scenes import TetMesh
 
 MAX_STACK = 32  # traversal stack entries per ray (render.py:424, 924)
+LOCAL_CHUNK = 128  # constraints per colour-sorted run of the "local" order
 
 _MODES = {None: OFF, "off": OFF, "pbd": PBD, "xpbd": XPBD}
 
@@ -38,7 +39,7 @@ def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
 class MeshTopology:
     """Host-side topology of one mesh plus its BVH (built once)."""
 
-    def __init__(self, mesh: TetMesh, bins: int = 8):
+    def __init__(self, mesh: TetMesh, bins: int = 8, constraint_order: str = "local"):
         self.mesh = mesh
         surf = mesh.surface_vertices()
         local = np.full(mesh.num_vertices, -1, dtype=np.int32)
@@ -77,10 +78,10 @@ class MeshTopology:
         self.level_nodes = np.concatenate(order).astype(np.int32)
         self.level_off = np.asarray(self.level_off, dtype=np.int32)
         self.tri_colors_leaf = mesh.tri_colors[tri_map] if mesh.tri_colors is not None else None
-        # Constraint order on the device: sorted by colour (stable), so a
-        # wavefront's lanes touch disjoint vertices.
-        self.tet_perm, self.num_tet_colors = self._color_order(mesh.tets, mesh.num_vertices)
-        self.edge_perm, self.num_edge_colors = self._color_order(mesh.edges, mesh.num_vertices)
+        # Constraint order on the device (see ``constraint_order``); the
+        # frame's integer Jacobi sums make results independent of it.
+        self.tet_perm = self._order(mesh.tets, mesh, constraint_order)
+        self.edge_perm = self._order(mesh.edges, mesh, constraint_order)
         self.tets = np.ascontiguousarray(mesh.tets[self.tet_perm]).astype(np.int32)
         self.tet_rest_volume = np.ascontiguousarray(mesh.tet_rest_volume[self.tet_perm])
         self.tet_inv_rest = np.ascontiguousarray(mesh.tet_inv_rest[self.tet_perm].reshape(-1, 9))
@@ -88,15 +89,30 @@ class MeshTopology:
         self.edge_rest = np.ascontiguousarray(mesh.edge_rest[self.edge_perm])
 
     @staticmethod
-    def _color_order(elems: np.ndarray, nv: int):
+    def _order(elems: np.ndarray, mesh: TetMesh, order: str) -> np.ndarray:
+        """Visit order of the constraints in the frame kernel:
+        "local" - Morton order of the centroids, greedy-coloured within runs of
+                  128 (position gathers of a wavefront hit few cache lines and
+                  its LDS atomics rarely collide);
+        "color" - stable sort by a greedy colour of the whole mesh;
+        "mesh"  - the mesh's own order."""
         elems = np.ascontiguousarray(elems, dtype=np.int32)
         n = elems.shape[0]
-        if n == 0:
-            return np.zeros(0, np.int64), 0
-        color = np.empty(n, np.int32)
-        nc = C.c_int32(0)
-        lib.ffsrl_color_constraints(elems.ctypes.data, n, elems.shape[1], nv, color.ctypes.data, C.byref(nc))
-        return np.argsort(color, kind="stable"), int(nc.value)
+        if n == 0 or order == "mesh":
+            return np.arange(n, dtype=np.int64)
+        if order == "color":
+            color = np.empty(n, np.int32)
+            nc = C.c_int32(0)
+            lib.ffsrl_color_constraints(elems.ctypes.data, n, elems.shape[1], mesh.num_vertices, color.ctypes.data,
+                                        C.byref(nc))
+            return np.argsort(color, kind="stable")
+        if order != "local":
+            raise ValueError(f"unknown constraint order {order!r}")
+        verts = np.ascontiguousarray(mesh.verts, dtype=np.float32)
+        perm = np.empty(n, np.int64)
+        lib.ffsrl_order_constraints(elems.ctypes.data, n, elems.shape[1], verts.ctypes.data, mesh.num_vertices,
+                                    LOCAL_CHUNK, perm.ctypes.data)
+        return perm
 
 
 class SimModelDO:
@@ -116,7 +132,8 @@ class SimModelDO:
                  distance="pbd", volume="pbd", neoHookean=False, distanceCompliance=0.0,
                  volumeComplianceXPBD=0.0, globalVolumeCompliance=1.0, globalDeviatoricCompliance=1.0,
                  globalLaparoscopeDragLookupRadius=0.15, lambdaLegacy=False, bins=8,
-                 initialVelocity: Optional[np.ndarray] = None, tips: Optional[np.ndarray] = None):
+                 initialVelocity: Optional[np.ndarray] = None, tips: Optional[np.ndarray] = None,
+                 constraint_order: str = "local"):
         self.device = torch.device(device)
         self.meshes = list(meshes)
         env_mesh = list(range(len(self.meshes))) if env_mesh is None else list(env_mesh)
@@ -143,7 +160,8 @@ class SimModelDO:
         self.lambda_legacy = bool(lambdaLegacy)
         self.params = self._make_params()
 
-        self.topo = [MeshTopology(m, bins) for m in self.meshes]
+        self.constraint_order = constraint_order
+        self.topo = [MeshTopology(m, bins, constraint_order) for m in self.meshes]
         self._build_topology()
         self._build_envs()
         self._build_state(initialVelocity, tips)

@@ -197,12 +197,20 @@ int ffsrl_bvh_build(const float* verts, int32_t nv, const int32_t* faces, int32_
                     int32_t* num_nodes, int32_t* max_level);
 
 /* Greedy colouring of n constraints of `arity` vertices (host memory): two
- * constraints that share a vertex get different colours.  Sorting a mesh's
- * tets and edges by colour before upload keeps the lanes of a wavefront on
- * disjoint vertices.  No reference counterpart (the reference accumulates
- * with global atomics in element order, integrator.py:105-109). */
+ * constraints that share a vertex get different colours.  No reference
+ * counterpart (the reference accumulates with global atomics in element
+ * order, integrator.py:105-109). */
 int ffsrl_color_constraints(const int32_t* elems, int64_t n, int32_t arity, int32_t nv, int32_t* color,
                             int32_t* num_colors);
+
+/* Order in which the frame kernel visits a mesh's constraints (host memory):
+ * sorted along a Morton curve of the constraint centroids (verts: nv x 3
+ * floats), then, within each run of `chunk` consecutive constraints, stably
+ * by a greedy colour so that the lanes of one LDS atomic rarely share a
+ * vertex.  perm[i] = element visited i-th.  The frame's integer Jacobi sums
+ * make the result independent of this order.  No reference counterpart. */
+int ffsrl_order_constraints(const int32_t* elems, int64_t n, int32_t arity, const float* verts, int32_t nv,
+                            int32_t chunk, int64_t* perm);
 
 /* ---- HIP graphs --------------------------------------------------------- */
 /* Capture everything issued on `stream` between begin and end into a graph

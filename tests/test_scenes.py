@@ -68,7 +68,7 @@ def test_nearest_surface_vertices(pkg):
 @pytest.mark.parametrize("which", ["tets", "edges"])
 def test_coloring_is_conflict_free(pkg, which):
     m = pkg.scenes.solid("box", 5000, seed=2)
-    topo = pkg.model.MeshTopology(m)
+    topo = pkg.model.MeshTopology(m, constraint_order="color")
     elems = getattr(m, which)
     color = np.empty(len(elems), np.int32)
     nc = C.c_int32()
@@ -84,3 +84,54 @@ def test_coloring_is_conflict_free(pkg, which):
     assert (np.diff(color[perm]) >= 0).all()
     dev = topo.tets if which == "tets" else topo.edges
     assert np.array_equal(dev, elems[perm])
+
+
+def _lane_collisions(elems, lanes=64):
+    """Vertex repeats inside groups of `lanes` consecutive constraints (the
+    lanes of one wavefront's LDS atomic)."""
+    n = 0
+    for a in range(0, len(elems), lanes):
+        v = elems[a:a + lanes].reshape(-1)
+        n += len(v) - len(np.unique(v))
+    return n
+
+
+def _gather_lines(elems, lanes=64, per_line=8):
+    """Distinct 128-byte lines of float4 positions touched per group of
+    `lanes` consecutive constraints (the gathers of one wavefront)."""
+    n = 0
+    for a in range(0, len(elems), lanes):
+        n += len(np.unique(elems[a:a + lanes].reshape(-1) // per_line))
+    return n
+
+
+@pytest.mark.parametrize("which", ["tets", "edges"])
+def test_local_order(pkg, which):
+    rng = np.random.default_rng(5)
+    m = pkg.scenes.solid("ellipsoid", 6000, seed=4)
+    elems = np.ascontiguousarray(getattr(m, which), np.int32)
+    # scramble the element order so that locality must come from the ordering
+    elems = elems[rng.permutation(len(elems))]
+    verts = np.ascontiguousarray(m.verts, np.float32)
+
+    def order(chunk):
+        perm = np.empty(len(elems), np.int64)
+        pkg.lib.ffsrl_order_constraints(elems.ctypes.data, len(elems), elems.shape[1], verts.ctypes.data,
+                                        m.num_vertices, chunk, perm.ctypes.data)
+        assert np.array_equal(np.sort(perm), np.arange(len(elems)))   # a permutation
+        return perm
+
+    local, morton = elems[order(128)], elems[order(1)]               # chunk 1: Morton order only
+    # spatially local: far fewer distinct cache lines per wavefront than the scrambled order
+    assert _gather_lines(local) < 0.5 * _gather_lines(elems)
+    assert _gather_lines(morton) < 0.5 * _gather_lines(elems)
+    # colour-sorted within runs of 128: fewer lanes of one atomic share a vertex
+    assert _lane_collisions(local) < _lane_collisions(morton)
+    # each run of 128 holds the same elements as the Morton order's run
+    for a in range(0, len(elems), 128):
+        assert sorted(map(tuple, local[a:a + 128])) == sorted(map(tuple, morton[a:a + 128]))
+    perm = np.empty(len(elems), np.int64)
+    # bad arguments are rejected
+    with pytest.raises(pkg.FFSRLError):
+        pkg.lib.ffsrl_order_constraints(elems.ctypes.data, len(elems), elems.shape[1], verts.ctypes.data,
+                                        m.num_vertices, 0, perm.ctypes.data)
