@@ -10,13 +10,15 @@
 // Moller-Trumbore test and the strict "0 < t < tMin" acceptance follow the
 // reference expression by expression, so hit triangles are bit-exact.
 #include <algorithm>
+#include <climits>
 
 #include "common.hpp"
 
 namespace ffsrl {
 
 constexpr int kStack = 32;  // deepest supported BVH (levels); the reference's stack size
-constexpr int kWaves = 2;   // wavefronts per block: small blocks release LDS early
+constexpr int kWaves = 1;   // wavefronts per block (one 8x8 ray tile each)
+constexpr int kXcds = 8;    // gfx950 dispatches workgroups round-robin over 8 XCDs
 
 __device__ __forceinline__ int isect_aabb(f3 o, f3 d, f3 lo, f3 hi, float* tmin_out) {
   const float t1 = (lo.x - o.x) / d.x, t2 = (hi.x - o.x) / d.x;
@@ -76,15 +78,22 @@ struct RenderOut {
   float* thit;
 };
 
-__global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_camera cam, int env_begin, RenderOut out,
-                                                        int depth) {
+__global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_camera cam, int env_begin, int n_env,
+                                                        RenderOut out, int depth) {
   extern __shared__ int stack[];  // [kWaves][depth][64]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int RW = 2 * cam.width, RH = 2 * cam.height;
   const int tiles_x = RW >> 3, tiles = tiles_x * (RH >> 3);
-  const int tile = blockIdx.x * kWaves + wave;
-  if (tile >= tiles) return;  // whole wavefront exits; no block barrier below
-  const int el = blockIdx.y, env = env_begin + el;
+  // XCD-aware mapping of the 1D grid: block B runs on XCD B % 8, so the
+  // blocks of env el are B = 8 * (bpe * (el / 8) + j) + el % 8, j < bpe: all
+  // tiles of an env share one XCD's L2 (its BVH, triangles and positions are
+  // fetched into one L2 instead of eight).
+  const int bpe = (tiles + kWaves - 1) / kWaves;
+  const int xcd = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
+  const int el = (k / bpe) * kXcds + xcd;
+  const int tile = (k % bpe) * kWaves + wave;
+  if (el >= n_env || tile >= tiles) return;  // whole wavefront exits; no block barrier below
+  const int env = env_begin + el;
   const int col = (tile % tiles_x) * 8 + (lane & 7), row = (tile / tiles_x) * 8 + (lane >> 3);
   const int r = row * RW + col;
   const ffsrl_env_desc e = b.envs[env];
@@ -235,7 +244,6 @@ int ffsrl_render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t eb, int3
   if (m == 3 && !vcolor) return fail(FFSRL_E_ARG, "vertex colour mode needs vertex colours");
   if (m == 1 && !tcolor) return fail(FFSRL_E_ARG, "triangle colour mode needs triangle colours");
   if (ee == eb) return 0;
-  if (ee - eb > 65535) return fail(FFSRL_E_LIMIT, "at most 65535 environments per render call");
   const int tiles = (2 * cam->width / 8) * (2 * cam->height / 8);
   RenderOut out{depth, rgb, normals, vcolor, tcolor, hit, thit};
   // Stack entries per ray: the deepest BVH of the range (levels = max level + 1
@@ -248,8 +256,10 @@ int ffsrl_render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t eb, int3
                                           stack_depth - 1, kStack);
   }
   const int lds = kWaves * stack_depth * 64 * (int)sizeof(int);
-  hipLaunchKernelGGL(k_render, dim3((tiles + kWaves - 1) / kWaves, ee - eb), dim3(64 * kWaves), lds, as_stream(stream),
-                     *b, *cam, eb, out, stack_depth);
+  const int64_t bpe = (tiles + kWaves - 1) / kWaves, groups = (ee - eb + kXcds - 1) / kXcds;
+  if (groups * kXcds * bpe > INT32_MAX) return fail(FFSRL_E_LIMIT, "render grid too large");
+  hipLaunchKernelGGL(k_render, dim3((unsigned)(groups * kXcds * bpe)), dim3(64 * kWaves), lds, as_stream(stream), *b,
+                     *cam, eb, ee - eb, out, stack_depth);
   FFSRL_HIP(hipGetLastError());
   return 0;
 }
