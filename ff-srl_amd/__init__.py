@@ -10,8 +10,9 @@ from .integrator import SimIntegratorDO
 from .model import SimModelDO
 from .render import RaycastRendererDO, RenderMode, WarpRaycastRendererDO
 from . import scenes
+from . import serve
 
 __all__ = [
     "FFSRLError", "lib", "SimBVH", "SimIntegratorDO", "SimModelDO", "RaycastRendererDO", "RenderMode",
-    "WarpRaycastRendererDO", "scenes",
+    "WarpRaycastRendererDO", "scenes", "serve",
 ]

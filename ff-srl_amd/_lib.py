@@ -60,12 +60,25 @@ class Camera(C.Structure):
     ]
 
 
+class Requests(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("envs", _vp), ("kind", _vp), ("action", _vp), ("x0", _vp), ("tips0", _vp),
+        ("lo", _vp), ("hi", _vp),
+    ]
+
+
+REQ_STEP, REQ_RESET = 0, 1
+
 _i32, _f32, _i64 = C.c_int32, C.c_float, C.c_int64
 _P = C.POINTER
 
 # name -> (argtypes).  Every function returns int (0 = ok).
 SIGNATURES = {
     "ffsrl_frame": [_P(Batch), _P(Params), _vp, _i32, _i32, _vp],
+    "ffsrl_frame_envs": [_P(Batch), _P(Params), _vp, _i32, _i32, _i32, _vp],
+    "ffsrl_render_envs": [_P(Batch), _P(Camera), _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "ffsrl_apply_requests": [_P(Batch), _P(Requests), _f32, _vp],
+    "ffsrl_observe_envs": [_P(Batch), _vp, _i32, _vp, _f32, _vp, _vp],
     "ffsrl_bvh_refit": [_P(Batch), _i32, _i32, _vp],
     "ffsrl_render": [_P(Batch), _P(Camera), _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ffsrl_vertex_normals": [_P(Batch), _i32, _i32, _vp, _vp],

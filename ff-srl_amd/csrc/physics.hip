@@ -354,12 +354,12 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
 }
 
 template <int BLOCK>
-static int launch_frame(const ffsrl_batch& b, const ffsrl_params& prm, const int32_t* order, int refit,
+static int launch_frame(const ffsrl_batch& b, const ffsrl_params& prm, const int32_t* order, int n, int refit,
                         bool carry, int lds, hipStream_t st) {
   auto kern = carry ? k_frame<BLOCK, true> : k_frame<BLOCK, false>;
   FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  hipLaunchKernelGGL(kern, dim3(b.n_env), dim3(BLOCK), lds, st, b, prm, order, refit);
+  hipLaunchKernelGGL(kern, dim3(n), dim3(BLOCK), lds, st, b, prm, order, refit);
   FFSRL_HIP(hipGetLastError());
   return 0;
 }
@@ -572,8 +572,8 @@ int ffsrl_abi_version(void) { return FFSRL_ABI_VERSION; }
 const char* ffsrl_last_error(void) { return g_err.c_str(); }
 int ffsrl_frame_max_vertices(void) { return kMaxFrameVertices; }
 
-int ffsrl_frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* order, int32_t block,
-                int32_t refit, void* stream) {
+static int frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
+                 int32_t refit, void* stream) {
   if (!prm) return fail(FFSRL_E_ARG, "params NULL");
   int rc = check_batch(b, prm);
   if (rc) return rc;
@@ -585,15 +585,28 @@ int ffsrl_frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* or
     return fail(FFSRL_E_ARG, "XPBD with iterations > 1 needs lambda arrays");
   if (refit && (!b->tris || !b->node_left || !b->level_nodes || !b->level_off || !b->bvh_lo || !b->bvh_hi))
     return fail(FFSRL_E_ARG, "refit requested without BVH arrays");
-  if (b->n_env == 0) return 0;
+  if (n < 0 || n > b->n_env) return fail(FFSRL_E_ARG, "%d environments listed, batch has %d", n, b->n_env);
+  if (n == 0) return 0;
   const int lds = lds_bytes_for(b->max_nv < 1 ? 1 : b->max_nv);
   hipStream_t st = as_stream(stream);
   switch (block) {
-    case 256: return launch_frame<256>(*b, *prm, order, refit, carry, lds, st);
-    case 512: return launch_frame<512>(*b, *prm, order, refit, carry, lds, st);
-    case 1024: return launch_frame<1024>(*b, *prm, order, refit, carry, lds, st);
+    case 256: return launch_frame<256>(*b, *prm, envs, n, refit, carry, lds, st);
+    case 512: return launch_frame<512>(*b, *prm, envs, n, refit, carry, lds, st);
+    case 1024: return launch_frame<1024>(*b, *prm, envs, n, refit, carry, lds, st);
     default: return fail(FFSRL_E_ARG, "block must be 256, 512 or 1024 (got %d)", block);
   }
+}
+
+int ffsrl_frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* order, int32_t block,
+                int32_t refit, void* stream) {
+  if (!b) return fail(FFSRL_E_ARG, "batch NULL");
+  return frame(b, prm, order, b->n_env, block, refit, stream);
+}
+
+int ffsrl_frame_envs(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
+                     int32_t refit, void* stream) {
+  if (n > 0 && !envs) return fail(FFSRL_E_ARG, "env list NULL");
+  return frame(b, prm, envs, n, block, refit, stream);
 }
 
 int ffsrl_predict(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {

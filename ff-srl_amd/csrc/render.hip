@@ -78,8 +78,10 @@ struct RenderOut {
   float* thit;
 };
 
-__global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_camera cam, int env_begin, int n_env,
-                                                        RenderOut out, int depth) {
+// Environments: env_begin + el for el < n_env, or list[el] when list is not
+// NULL; outputs are indexed by el for a range and by the env id for a list.
+__global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_camera cam, int env_begin,
+                                                        const int32_t* list, int n_env, RenderOut out, int depth) {
   extern __shared__ int stack[];  // [kWaves][depth][64]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int RW = 2 * cam.width, RH = 2 * cam.height;
@@ -93,7 +95,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
   const int el = (k / bpe) * kXcds + xcd;
   const int tile = (k % bpe) * kWaves + wave;
   if (el >= n_env || tile >= tiles) return;  // whole wavefront exits; no block barrier below
-  const int env = env_begin + el;
+  const int env = list ? list[el] : env_begin + el;
+  const int64_t oi = list ? env : el;  // output slot
   const int col = (tile % tiles_x) * 8 + (lane & 7), row = (tile / tiles_x) * 8 + (lane >> 3);
   const int r = row * RW + col;
   const ffsrl_env_desc e = b.envs[env];
@@ -134,8 +137,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
   int4 tri = make_int4(0, 0, 0, -1);
   if (hitp) tri = ldi4(b.tris, fb + best);
   const int64_t rays_env = (int64_t)RW * RH;
-  if (out.hit) out.hit[el * rays_env + r] = hitp ? tri.w : -1;
-  if (out.thit) out.thit[el * rays_env + r] = hitp ? tMin : 0.0f;
+  if (out.hit) out.hit[oi * rays_env + r] = hitp ? tri.w : -1;
+  if (out.thit) out.thit[oi * rays_env + r] = hitp ? tMin : 0.0f;
 
   f3 color = mk(0.0f, 0.0f, 0.0f);
   if (hitp) {
@@ -178,7 +181,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
   }
   // 2x2 box: rays (2py+i, 2px+j) summed in (i, j) = (0,0), (0,1), (1,0), (1,1) order.
   const bool owner = ((lane & 1) == 0) && (((lane >> 3) & 1) == 0);
-  const int64_t pix = (int64_t)el * cam.height * cam.width + (row >> 1) * cam.width + (col >> 1);
+  const int64_t pix = oi * cam.height * cam.width + (row >> 1) * cam.width + (col >> 1);
   if (cam.mode == 4) {
     const float c01 = __shfl_down(color.x, 1), c10 = __shfl_down(color.x, 8), c11 = __shfl_down(color.x, 9);
     if (owner && out.depth) out.depth[pix] = (((color.x + c01) + c10) + c11) / 4.0f;
@@ -228,40 +231,57 @@ using namespace ffsrl;
 
 extern "C" {
 
-int ffsrl_render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t eb, int32_t ee, float* depth, float* rgb,
-                 const float* normals, const float* vcolor, const float* tcolor, int32_t* hit, float* thit,
-                 void* stream) {
+static int render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t eb, const int32_t* list, int32_t n,
+                  const RenderOut& out, hipStream_t st) {
   if (!b || !cam || !b->envs || !b->x || !b->tris || !b->bvh_lo || !b->bvh_hi || !cam->dirs)
     return fail(FFSRL_E_ARG, "render needs positions, triangles, BVH bounds and ray directions");
   if (cam->width <= 0 || cam->height <= 0 || (cam->width & 3) || (cam->height & 3))
     return fail(FFSRL_E_ARG, "render size %dx%d: width and height must be positive multiples of 4", cam->height,
                 cam->width);
-  if (eb < 0 || ee > b->n_env || eb > ee) return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", eb, ee, b->n_env);
   const int m = cam->mode;
   if (m != 0 && m != 1 && m != 2 && m != 3 && m != 4) return fail(FFSRL_E_ARG, "render mode %d", m);
-  if (m == 4 && !depth && !hit) return fail(FFSRL_E_ARG, "depth mode needs a depth (or hit) output");
-  if (m != 4 && (!rgb || !normals)) return fail(FFSRL_E_ARG, "colour modes need rgb output and normals");
-  if (m == 3 && !vcolor) return fail(FFSRL_E_ARG, "vertex colour mode needs vertex colours");
-  if (m == 1 && !tcolor) return fail(FFSRL_E_ARG, "triangle colour mode needs triangle colours");
-  if (ee == eb) return 0;
+  if (m == 4 && !out.depth && !out.hit) return fail(FFSRL_E_ARG, "depth mode needs a depth (or hit) output");
+  if (m != 4 && (!out.rgb || !out.normals)) return fail(FFSRL_E_ARG, "colour modes need rgb output and normals");
+  if (m == 3 && !out.vcolor) return fail(FFSRL_E_ARG, "vertex colour mode needs vertex colours");
+  if (m == 1 && !out.tcolor) return fail(FFSRL_E_ARG, "triangle colour mode needs triangle colours");
+  if (n == 0) return 0;
   const int tiles = (2 * cam->width / 8) * (2 * cam->height / 8);
-  RenderOut out{depth, rgb, normals, vcolor, tcolor, hit, thit};
-  // Stack entries per ray: the deepest BVH of the range (levels = max level + 1
-  // bounds the stack of this right-first DFS); kStack without a host mirror.
+  // Stack entries per ray: the deepest BVH of the envs (levels = max level + 1
+  // bounds the stack of this right-first DFS; a list is bounded by the whole
+  // table); kStack without a host mirror.
   int stack_depth = kStack;
   if (b->host_envs) {
     stack_depth = 2;
-    for (int i = eb; i < ee; ++i) stack_depth = std::max(stack_depth, (int)b->host_envs[i].nlvl + 1);
+    const int lo = list ? 0 : eb, hi = list ? b->n_env : eb + n;
+    for (int i = lo; i < hi; ++i) stack_depth = std::max(stack_depth, (int)b->host_envs[i].nlvl + 1);
     if (stack_depth > kStack) return fail(FFSRL_E_LIMIT, "BVH with %d levels exceeds the %d-entry ray stack",
                                           stack_depth - 1, kStack);
   }
   const int lds = kWaves * stack_depth * 64 * (int)sizeof(int);
-  const int64_t bpe = (tiles + kWaves - 1) / kWaves, groups = (ee - eb + kXcds - 1) / kXcds;
+  const int64_t bpe = (tiles + kWaves - 1) / kWaves, groups = (n + kXcds - 1) / kXcds;
   if (groups * kXcds * bpe > INT32_MAX) return fail(FFSRL_E_LIMIT, "render grid too large");
-  hipLaunchKernelGGL(k_render, dim3((unsigned)(groups * kXcds * bpe)), dim3(64 * kWaves), lds, as_stream(stream), *b,
-                     *cam, eb, ee - eb, out, stack_depth);
+  hipLaunchKernelGGL(k_render, dim3((unsigned)(groups * kXcds * bpe)), dim3(64 * kWaves), lds, st, *b, *cam, eb, list,
+                     n, out, stack_depth);
   FFSRL_HIP(hipGetLastError());
   return 0;
+}
+
+int ffsrl_render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t eb, int32_t ee, float* depth, float* rgb,
+                 const float* normals, const float* vcolor, const float* tcolor, int32_t* hit, float* thit,
+                 void* stream) {
+  if (!b) return fail(FFSRL_E_ARG, "batch NULL");
+  if (eb < 0 || ee > b->n_env || eb > ee) return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", eb, ee, b->n_env);
+  return render(b, cam, eb, nullptr, ee - eb, RenderOut{depth, rgb, normals, vcolor, tcolor, hit, thit},
+                as_stream(stream));
+}
+
+int ffsrl_render_envs(const ffsrl_batch* b, const ffsrl_camera* cam, const int32_t* envs, int32_t n, float* depth,
+                      float* rgb, const float* normals, const float* vcolor, const float* tcolor, int32_t* hit,
+                      float* thit, void* stream) {
+  if (!b) return fail(FFSRL_E_ARG, "batch NULL");
+  if (n < 0 || n > b->n_env || (n > 0 && !envs))
+    return fail(FFSRL_E_ARG, "env list of %d entries (batch has %d envs)", n, b->n_env);
+  return render(b, cam, 0, envs, n, RenderOut{depth, rgb, normals, vcolor, tcolor, hit, thit}, as_stream(stream));
 }
 
 int ffsrl_vertex_normals(const ffsrl_batch* b, int32_t eb, int32_t ee, float* normals, void* stream) {

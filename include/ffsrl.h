@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 1
+#define FFSRL_ABI_VERSION 2
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -119,6 +119,20 @@ typedef struct ffsrl_camera {
   float light_intensity;
 } ffsrl_camera;
 
+/* One tick of serving requests (device arrays of n entries, env ids distinct). */
+#define FFSRL_REQ_STEP 0
+#define FFSRL_REQ_RESET 1
+typedef struct ffsrl_requests {
+  int32_t n;
+  const int32_t* envs;            /* env id of each request */
+  const int32_t* kind;            /* FFSRL_REQ_STEP / FFSRL_REQ_RESET */
+  const float* action;            /* float4 per request: tool delta xyz, clamp (-1 keep, 0 open, 1 close) */
+  const float* x0;                /* float4 [V]: rest positions (xyz) and inverse masses (w) */
+  const float* tips0;             /* float [n_env*4*3]: initial instrument points */
+  const float* lo;                /* float[3] workspace lower corner, or NULL */
+  const float* hi;                /* float[3] workspace upper corner, or NULL */
+} ffsrl_requests;
+
 int ffsrl_abi_version(void);
 const char* ffsrl_last_error(void);
 
@@ -133,6 +147,12 @@ const char* ffsrl_last_error(void);
  * block = threads per environment (256, 512 or 1024). */
 int ffsrl_frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* order, int32_t block,
                 int32_t refit, void* stream);
+
+/* ffsrl_frame restricted to the n environments listed in envs (device, ids
+ * distinct, in dispatch order): the serving loop steps only the scenes with a
+ * pending step request.  Same reference counterpart as ffsrl_frame. */
+int ffsrl_frame_envs(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
+                     int32_t refit, void* stream);
 
 /* Largest nv that ffsrl_frame can hold in LDS (dP + counters). */
 int ffsrl_frame_max_vertices(void);
@@ -151,6 +171,14 @@ int ffsrl_render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t env_begi
                  float* depth, float* rgb, const float* normals, const float* vcolor,
                  const float* tcolor, int32_t* hit, float* thit, void* stream);
 
+/* ffsrl_render of the n environments listed in envs (device, ids distinct).
+ * Outputs are indexed by env id (they hold n_env images), so each scene of
+ * the serving loop keeps its latest observation in place.  Same reference
+ * counterpart as ffsrl_render. */
+int ffsrl_render_envs(const ffsrl_batch* b, const ffsrl_camera* cam, const int32_t* envs, int32_t n,
+                      float* depth, float* rgb, const float* normals, const float* vcolor,
+                      const float* tcolor, int32_t* hit, float* thit, void* stream);
+
 /* Area-weighted vertex normals (float4 [V]) of the render mesh.  Replaces
  * zeroVec3Array + vertex_normal_sum_kernel + normalize_kernel (render.py:81-121). */
 int ffsrl_vertex_normals(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, float* normals,
@@ -168,6 +196,29 @@ int ffsrl_move_tools(const ffsrl_batch* b, const float* delta, const float* lo, 
  * modelDO.py:247-279).  state/update are int32 [n_env*3] device arrays. */
 int ffsrl_grasp_select(const ffsrl_batch* b, float radius, const int32_t* state,
                        const int32_t* update, void* stream);
+
+/* ---- serving ------------------------------------------------------------ */
+
+/* Applies one tick of requests: a reset restores the env like resetFixedModel
+ * (modelDO.py:1589-1622: positions from x0, velocities and grasp weights 0,
+ * instrument points from tips0) and refits its BVH; a step adds the action's
+ * delta to the env's 4 instrument points (clamped to [lo, hi] when given, as
+ * applyCartesianActionsInWorkspace clamps, modelDO.py:1385-1404) and, when
+ * the clamp command is 0 or 1, reselects the grasp like updateClampsDragKernel
+ * (modelDO.py:247-279): drag = |x - tip[env*4+3]| < radius when closed, 0 when
+ * open. */
+int ffsrl_apply_requests(const ffsrl_batch* b, const ffsrl_requests* r, float radius, void* stream);
+
+/* 32 observation scalars per listed env, written to out[env*32 + k]:
+ *   0-11 the 4 instrument points, 12-14 centroid, 15-17 / 18-20 bounding box
+ *   min / max, 21-23 mean velocity, 24 max speed, 25 grasped vertices,
+ *   26-28 centroid of the grasped vertices (0 if none), 29 vertices at or
+ *   below the ground level, 30 / 31 max / mean displacement from x0.
+ * Sums use a fixed reduction tree (deterministic).  No reference counterpart
+ * (the reference's observations are read on the host from torch views,
+ * modelDO.py:1483-1518). */
+int ffsrl_observe_envs(const ffsrl_batch* b, const int32_t* envs, int32_t n, const float* x0, float ground,
+                       float* out, void* stream);
 
 /* ---- plain form: one launch per kernel per environment ------------------ */
 /* The reference's per-kernel launch model (integrator.py:1644-1741), restricted

@@ -33,7 +33,7 @@ def test_every_declared_symbol_is_exported(pkg):
 
 
 def test_abi_version(pkg):
-    assert pkg.lib.ffsrl_abi_version() == 1
+    assert pkg.lib.ffsrl_abi_version() == 2
     assert pkg.lib.ffsrl_frame_max_vertices() >= 10000
 
 
@@ -47,6 +47,7 @@ int main(void) {
   printf("ffsrl_batch size %zu\n", sizeof(ffsrl_batch));
   printf("ffsrl_params size %zu\n", sizeof(ffsrl_params));
   printf("ffsrl_camera size %zu\n", sizeof(ffsrl_camera));
+  printf("ffsrl_requests size %zu\n", sizeof(ffsrl_requests));
   %FIELDS%
   return 0;
 }
@@ -56,7 +57,7 @@ int main(void) {
 def test_struct_layouts_match_header(pkg, tmp_path):
     L = pkg._lib
     structs = {"ffsrl_batch": L.Batch, "ffsrl_params": L.Params, "ffsrl_camera": L.Camera,
-               "ffsrl_env_desc": L.EnvDesc}
+               "ffsrl_env_desc": L.EnvDesc, "ffsrl_requests": L.Requests}
     fields = "\n".join(f"F({n}, {f[0]})" for n, S in structs.items() for f in S._fields_)
     src = tmp_path / "layout.c"
     src.write_text(_LAYOUT_C.replace("%FIELDS%", fields))
@@ -89,6 +90,28 @@ def test_argument_errors_are_reported(pkg):
     with pytest.raises(pkg.FFSRLError, match="remap"):
         pkg.lib.ffsrl_remap(None, None, None, 5, None)
     assert "remap" in pkg.lib.ffsrl_last_error().decode()
+    # list forms of the serving loop
+    dummy = np.zeros(16, np.float32)
+    for f in ("envs", "x", "p", "v", "drag", "tips"):
+        setattr(b, f, dummy.ctypes.data)
+    b.n_env = 4
+    prm.substeps = prm.iterations = 1
+    prm.dt = 0.01
+    with pytest.raises(pkg.FFSRLError, match="env list NULL"):
+        pkg.lib.ffsrl_frame_envs(b, prm, None, 2, 1024, 0, None)
+    with pytest.raises(pkg.FFSRLError, match="5 environments listed"):
+        pkg.lib.ffsrl_frame_envs(b, prm, dummy.ctypes.data, 5, 1024, 0, None)
+    with pytest.raises(pkg.FFSRLError, match="env list"):
+        pkg.lib.ffsrl_render_envs(b, cam, None, 2, None, None, None, None, None, None, None, None)
+    r = L.Requests()
+    r.n = 2
+    with pytest.raises(pkg.FFSRLError, match="NULL request array"):
+        pkg.lib.ffsrl_apply_requests(b, r, 0.1, None)
+    r.n = 9
+    with pytest.raises(pkg.FFSRLError, match="9 requests"):
+        pkg.lib.ffsrl_apply_requests(b, r, 0.1, None)
+    with pytest.raises(pkg.FFSRLError, match="NULL argument"):
+        pkg.lib.ffsrl_observe_envs(b, None, 2, None, 0.0, None, None)
 
 
 def test_frame_rejects_envs_beyond_lds(pkg):

@@ -1,0 +1,162 @@
+"""Serving loop on the GPU: each scene's observations are the same whether it
+is served alone or continuously batched with other scenes (bit-exact: the
+frame's Jacobi sums are integers and every reduction has a fixed order),
+admission under a memory budget, and the real-time Poisson driver."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+PHYS = dict(simSubsteps=10, simFrameRate=60, simConstraintsSteps=1, environmentGravity=(0.0, -9.81, 0.0),
+            environmentVelocityDampening=0.1, distance="xpbd", volume="xpbd", distanceCompliance=1e-6,
+            volumeComplianceXPBD=0.0, neoHookean=True, globalVolumeCompliance=1.0 / 5e4,
+            globalDeviatoricCompliance=1.0 / 1e4, globalKsDrag=1.0)
+
+
+def _classes(pkg):
+    sc = pkg.scenes
+    meshes = [sc.solid("ellipsoid", 1500, seed=21), sc.solid("box", 3000, seed=22)]
+    tips = np.stack([np.repeat((sc.top_point(m) + np.array([0.0, 0.02, 0.0]))[None], 4, axis=0) for m in meshes])
+    return meshes, tips.astype(np.float32)
+
+
+def _server(pkg, gpu, capacity=(4, 4), budget=1e12, max_batch=4, **kw):
+    S = pkg.serve
+    meshes, tips = _classes(pkg)
+    return S.SceneServer(meshes, capacity, budget, PHYS, tips, device=gpu, resolution=32, max_batch=max_batch,
+                         max_inflight=3, streams=2, grasp_radius=0.08, **kw)
+
+
+def _script(pkg, actors, steps=6, seed=3):
+    """Per actor: reset, `steps` steps (clamp closed on the first), and for odd
+    actors a reset and two more steps."""
+    S = pkg.serve
+    rng = np.random.default_rng(seed)
+    out = {}
+    for a in actors:
+        reqs = [S.Request(a, S.REQ_RESET)]
+        for k in range(steps):
+            d = rng.normal(0.0, 0.01, 3)
+            reqs.append(S.Request(a, S.REQ_STEP, (d[0], d[1], d[2], S.CLAMP_CLOSE if k == 0 else S.CLAMP_KEEP)))
+        if a % 2:
+            reqs.append(S.Request(a, S.REQ_RESET))
+            for k in range(2):
+                reqs.append(S.Request(a, S.REQ_STEP, (0.0, 0.02, 0.0, S.CLAMP_CLOSE if k == 0 else S.CLAMP_KEEP)))
+        for i, r in enumerate(reqs):
+            r.seq = i
+        out[a] = reqs
+    return out
+
+
+def _run(server, scripts, classes):
+    got = {}
+
+    def record(c, srv):
+        if srv is None:
+            return
+        got.setdefault(c.actor, []).append((c.seq, srv.depth[c.slot].clone(), srv.obs[c.slot].clone()))
+
+    server.on_complete = record
+    for a, reqs in scripts.items():
+        server.register(a, classes[a])
+    # interleave the actors' requests
+    longest = max(len(r) for r in scripts.values())
+    for k in range(longest):
+        for a, reqs in scripts.items():
+            if k < len(reqs):
+                server.submit(reqs[k])
+    server.run_until_idle()
+    torch.cuda.synchronize()
+    return got
+
+
+def test_served_alone_equals_in_mix(pkg, gpu):
+    actors = list(range(6))
+    classes = {a: a % 2 for a in actors}
+    scripts = _script(pkg, actors)
+    mix = _server(pkg, gpu)
+    got_mix = _run(mix, scripts, classes)
+    assert mix.stats["ticks"] < sum(len(r) for r in scripts.values())       # requests were batched
+    assert mix.stats["tick_requests"] == sum(len(r) for r in scripts.values())
+    for a in actors:
+        alone = _server(pkg, gpu, max_batch=1)
+        got = _run(alone, {a: scripts[a]}, classes)
+        assert [s for s, _, _ in got[a]] == list(range(len(scripts[a])))
+        assert [s for s, _, _ in got_mix[a]] == list(range(len(scripts[a])))
+        for (s, d1, o1), (_, d2, o2) in zip(got[a], got_mix[a]):
+            assert torch.equal(d1, d2), (a, s)
+            assert torch.equal(o1, o2), (a, s)
+        depth_last, obs_last = got[a][-1][1], got[a][-1][2]
+        assert bool(torch.isfinite(obs_last).all())
+        assert float((depth_last > 0).float().mean()) > 0.05                 # the scene is in view
+        assert obs_last[25] > 0                                              # clamp closed: vertices grasped
+        print(f"\n[serve] actor {a}: {len(got[a])} requests identical alone and in the mix; "
+              f"grasped {int(obs_last[25])}, max displacement {float(obs_last[30]):.4f}")
+
+
+def test_reset_restores_the_initial_observation(pkg, gpu):
+    S = pkg.serve
+    srv = _server(pkg, gpu)
+    srv.register(0, 1)
+    reqs = [S.Request(0, S.REQ_RESET)] + [S.Request(0, S.REQ_STEP, (0.0, 0.05, 0.0, S.CLAMP_CLOSE))] * 3 + \
+        [S.Request(0, S.REQ_RESET)]
+    got = _run(srv, {0: [S.Request(r.actor, r.kind, r.action, seq=i) for i, r in enumerate(reqs)]}, {0: 1})
+    first, last = got[0][0], got[0][-1]
+    assert torch.equal(first[1], last[1]) and torch.equal(first[2], last[2])
+    assert not torch.equal(first[1], got[0][3][1])                           # the steps changed the scene
+    assert float(first[2][30]) == 0.0 and float(first[2][25]) == 0.0          # at rest, nothing grasped
+
+
+def test_admission_waits_for_memory(pkg, gpu):
+    S = pkg.serve
+    srv = _server(pkg, gpu, capacity=(2, 4), max_batch=8)
+    budget = 2 * srv.class_bytes[1] + srv.class_bytes[1] // 2                 # room for two large scenes
+    srv.budget = budget
+    peak = []
+
+    def watch(c, s):
+        peak.append(srv.resident_bytes)
+
+    actors = [10, 11, 12, 13]
+    for a in actors:
+        srv.register(a, 1)
+    srv.on_complete = watch
+    for a in actors:
+        srv.submit(S.Request(a, S.REQ_RESET))
+        srv.submit(S.Request(a, S.REQ_STEP, (0.0, 0.01, 0.0, S.CLAMP_CLOSE)))
+        srv.submit(S.Request(a, S.CLOSE))
+    assert srv.stats["admission_waits"] == 2
+    srv.run_until_idle()
+    torch.cuda.synchronize()
+    assert srv.stats["admitted"] == 4 and srv.stats["closed"] == 4
+    assert max(peak) <= budget
+    done = [(c.actor, c.kind) for c in srv.completions]
+    for a in actors:
+        assert (a, S.REQ_RESET) in done and (a, S.REQ_STEP) in done and (a, S.CLOSE) in done
+    assert srv.resident_bytes == 0
+
+
+def test_first_step_of_a_new_scene_is_a_reset(pkg, gpu):
+    S = pkg.serve
+    srv = _server(pkg, gpu)
+    srv.register(5, 0)
+    srv.submit(S.Request(5, S.REQ_STEP, (0.0, 0.0, 0.0, S.CLAMP_KEEP)))
+    srv.run_until_idle()
+    assert srv.stats["step_as_reset"] == 1
+    assert srv.completions[-1].kind == S.REQ_RESET
+
+
+def test_poisson_driver(pkg, gpu):
+    S = pkg.serve
+    meshes, tips = _classes(pkg)
+    load = S.ServingLoad(rate=4000.0, actors=32, class_p=(0.6, 0.4), seed=1)
+    cls, reqs = load.generate(0.5)
+    srv = S.SceneServer(meshes, (32, 32), 1e12, PHYS, tips, device=gpu, resolution=32, max_batch=64)
+    for a in range(load.actors):
+        srv.register(a, int(cls[a]))
+    out = S.serve(srv, reqs, 0.5)
+    print(f"\n[serve] {out}")
+    assert out["completed_steps"] + out["completed_resets"] == len(reqs)
+    assert 0.0 < out["p50_step_ms"] <= out["p99_step_ms"] < 1000.0
+    assert all(c.latency > 0 for c in srv.completions)
