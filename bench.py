@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "ellipsoids", "cube", "large"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "ellipsoids", "cube", "large", "serve"])
     ap.add_argument("--envs", type=int, default=None)
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--plain-frames", type=int, default=2)
@@ -44,6 +44,13 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the serving run after the batched measurement (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20000.0)
+    ap.add_argument("--serve-actors", type=int, default=4096)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-streams", type=int, default=2)
+    ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy before timing (HBM counter calibration)")
     return ap.parse_args()
@@ -132,6 +139,37 @@ def cpu_baseline(pkg, orc, w, model, ren, seconds, seed):
                       f"{mean_t_sample:.0f}/{mean_t_pop:.0f}"}
 
 
+def serving_run(pkg, wl, args, dev):
+    """Poisson stream of step / reset requests from RL actors, continuously
+    batched (ffsrl_apply_requests + ffsrl_frame_envs + ffsrl_render_envs +
+    ffsrl_observe_envs per tick); latency is arrival -> GPU completion."""
+    import torch
+    S = pkg.serve
+    sw = wl.serving(actors=args.serve_actors, rate=args.serve_rate, seed=args.seed)
+    load = S.ServingLoad(rate=sw.rate, actors=sw.actors, class_p=sw.class_p, seed=args.seed)
+    cls, reqs = load.generate(args.serve_seconds)
+    capacity = np.bincount(cls, minlength=len(sw.classes))
+    srv = S.SceneServer(sw.classes, capacity, args.serve_budget_gb * 1e9, sw.model_kwargs, sw.tips, device=dev,
+                        resolution=sw.resolution, camera_pos=sw.camera_pos, look_at=sw.look_at,
+                        block=args.block, streams=args.serve_streams, max_inflight=args.serve_inflight)
+    for a in range(sw.actors):
+        srv.register(a, int(cls[a]))
+    log(f"serving {sw.name}: {len(reqs)} requests over {args.serve_seconds:.0f} s, "
+        f"{srv.n_slots} slots, class bytes {srv.class_bytes}")
+    out = S.serve(srv, reqs, args.serve_seconds)
+    st = srv.stats
+    out.update(workload=sw.name, duration_s=args.serve_seconds, requests=len(reqs), actors=sw.actors,
+               rate_req_per_s=sw.rate, ticks=int(st["ticks"]),
+               mean_tick_requests=st["tick_requests"] / max(st["ticks"], 1), admitted=int(st["admitted"]),
+               streams=args.serve_streams, max_inflight=args.serve_inflight,
+               admission_waits=int(st["admission_waits"]), budget_gb=args.serve_budget_gb,
+               resident_gb=srv.resident_bytes / 1e9,
+               obs="64x64 fp32 depth + 32 scalars per request" if sw.resolution == 64 else f"{sw.resolution}^2 depth")
+    del srv
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -139,6 +177,15 @@ def main():
                                    "described in DESIGN.md, not launched", "n_gpus": args.gpus}))
         sys.exit(2)
     wl = importlib.import_module("ff-srl_amd.workloads")
+    if args.workload == "serve":
+        import torch
+        pkg = importlib.import_module("ff-srl_amd")
+        srv = serving_run(pkg, wl, args, torch.device("cuda:0"))
+        print(json.dumps({"metric": "serving: env-steps/s + p50/p99 step latency", "value": srv["env_steps_per_s"],
+                          "unit": "env-steps/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+                          "data": "synthetic (seeded meshes, seeded Poisson requests)",
+                          "config": {"workload": srv["workload"]}, "serving": srv}), flush=True)
+        return
     t_setup = time.perf_counter()
     w = make_workload(wl, args)                       # meshes generated before the GPU runtime starts
     log(f"workload {w.name}: {w.totals()} in {time.perf_counter() - t_setup:.1f} s")
@@ -262,6 +309,10 @@ def main():
         out["plain_env_steps_per_s"] = n * args.plain_frames / plain_el
         out["plain_p50_step_ms"] = float(np.percentile(pms, 50))
         out["batched_over_plain"] = value / out["plain_env_steps_per_s"]
+
+    if args.serve_seconds > 0:
+        log("serving")
+        out["serving"] = serving_run(pkg, wl, args, dev)
 
     if not args.no_cpu_baseline:
         log("cpu baseline")

