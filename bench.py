@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--workload", default="mixed", choices=["mixed", "ellipsoids", "cube", "large", "serve"])
     ap.add_argument("--envs", type=int, default=None)
     ap.add_argument("--block", type=int, default=1024)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --block and largest-first dispatch instead of the autotuned launch shape")
     ap.add_argument("--plain-frames", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -212,6 +214,11 @@ def main():
         lib.ffsrl_render(model.batch, ren.camera, 0, n, depth_out.data_ptr(), None, None, None, None, None, None, sh)
 
     pkg.SimBVH(dev, model)
+    tuned = None
+    if not args.no_autotune:
+        t_tune = time.perf_counter()
+        tuned = integ.autotune(model)
+        log(f"autotuned {tuned} in {time.perf_counter() - t_tune:.1f} s")
     if args.calibrate_copy:
         a = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -254,7 +261,9 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded meshes, seeded tool random walk)",
         "config": {"workload": w.name, "envs": n, **{k: int(v) for k, v in w.totals().items()},
                    "substeps": w.model_kwargs["simSubsteps"], "iterations": w.model_kwargs["simConstraintsSteps"],
-                   "render": f"{w.resolution}x{w.resolution} depth (2x2 supersampled)", "block": args.block,
+                   "render": f"{w.resolution}x{w.resolution} depth (2x2 supersampled)", "block": integ.block,
+                   "dispatch": "lpt" if integ.dispatch_sorted else "table",
+                   "chunk": model.topo[0].chunk, "autotuned": tuned is not None,
                    "form": "batched (1 frame kernel + 1 render kernel per step)"},
         "p50_step_ms": float(np.percentile(step_ms, 50)), "p99_step_ms": float(np.percentile(step_ms, 99)),
         "frame_kernel_ms": float(frame_ms.mean()), "render_kernel_ms": float(render_ms.mean()),

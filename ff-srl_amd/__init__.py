@@ -9,10 +9,11 @@ from .bvh import SimBVH
 from .integrator import SimIntegratorDO
 from .model import SimModelDO
 from .render import RaycastRendererDO, RenderMode, WarpRaycastRendererDO
+from . import autotune
 from . import scenes
 from . import serve
 
 __all__ = [
     "FFSRLError", "lib", "SimBVH", "SimIntegratorDO", "SimModelDO", "RaycastRendererDO", "RenderMode",
-    "WarpRaycastRendererDO", "scenes", "serve",
+    "WarpRaycastRendererDO", "autotune", "scenes", "serve",
 ]

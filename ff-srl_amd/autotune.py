@@ -1,0 +1,123 @@
+"""Autotuner of the batched frame: picks the launch shape for a batch of
+environments by timing the candidates on the device, and caches the choice
+keyed on the env count and the tets per env.
+
+Knobs (all leave the results unchanged; the frame's Jacobi sums are integers):
+  block     threads per environment workgroup (256, 512, 1024);
+  dispatch  environment dispatch order: "lpt" (largest first) or "table";
+  chunk     run length of the colour sort inside the Morton constraint order
+            (the lanes of one LDS atomic rarely share a vertex within a run,
+            and a wavefront's position gathers stay local).
+
+A trial runs whole frames on a snapshot of the state, which is restored
+afterwards, so tuning does not advance the simulation.  The cache is a JSON
+file; its key is (env-count bucket, mean-tets bucket, max-tets bucket,
+constraint set), buckets being powers of two.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import math
+import os
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+
+from ._lib import lib
+from .model import LOCAL_CHUNK, SimModelDO
+
+BLOCKS = (256, 512, 1024)
+DISPATCH = ("lpt", "table")
+CHUNKS = (64, 128, 256)
+DEFAULT = {"block": 1024, "dispatch": "lpt", "chunk": LOCAL_CHUNK}
+
+
+def _bucket(x: float) -> int:
+    return 1 << max(0, int(math.ceil(math.log2(max(x, 1.0)))))
+
+
+def key_of(model: SimModelDO) -> str:
+    tets = np.array([model.meshes[k].num_tets for k in model.env_mesh], dtype=np.float64)
+    flags = f"d{model.dist_mode}v{model.vol_mode}n{int(model.nh_on)}s{model.simSubsteps}i{model.simConstraints}"
+    return f"envs{_bucket(model.numEnvs)}-tets{_bucket(tets.mean())}-max{_bucket(tets.max())}-{flags}"
+
+
+class Autotuner:
+    def __init__(self, cache_path: Optional[str] = None):
+        self.cache_path = cache_path
+        self.cache: Dict[str, dict] = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.cache = json.load(f)
+
+    def save(self) -> None:
+        if self.cache_path:
+            os.makedirs(os.path.dirname(os.path.abspath(self.cache_path)), exist_ok=True)
+            tmp = self.cache_path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(self.cache, f, indent=1, sort_keys=True)
+            os.replace(tmp, self.cache_path)
+
+    @staticmethod
+    def _state(model: SimModelDO):
+        return [model.x, model.p, model.v, model.bvh_lo, model.bvh_hi]
+
+    @staticmethod
+    def _lambdas(model: SimModelDO):
+        return [] if model.lam_t is None else [model.lam_t, model.lam_e]
+
+    @staticmethod
+    def _time(model: SimModelDO, cfg: dict, frames: int, stream) -> float:
+        order = model.dispatch_order().data_ptr() if cfg["dispatch"] == "lpt" else None
+        s = stream.cuda_stream
+        lib.ffsrl_frame(model.batch, model.params, order, cfg["block"], 1, s)          # warm
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(frames):
+            lib.ffsrl_frame(model.batch, model.params, order, cfg["block"], 1, s)
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / frames
+
+    def tune(self, model: SimModelDO, frames: int = 2, force: bool = False) -> dict:
+        """Best {block, dispatch, chunk} for this batch (cached), applied to
+        the model's constraint order; the state is left as it was."""
+        key = key_of(model)
+        if not force and key in self.cache:
+            cfg = dict(self.cache[key]["best"])
+            self.apply(model, cfg)
+            return cfg
+        stream = torch.cuda.current_stream(model.device)
+        orig = (model.topo[0].constraint_order, model.topo[0].chunk) if model.topo else ("local", LOCAL_CHUNK)
+        state = [(t, t.clone()) for t in self._state(model)]
+        lam0 = [(t, t.clone()) for t in self._lambdas(model)]
+        results = []
+        for chunk in CHUNKS:
+            model.set_constraint_order("local", chunk)          # carried lambdas follow their constraints
+            lam = [(t, t.clone()) for t in self._lambdas(model)]
+            torch.cuda.synchronize(model.device)
+            for block, dispatch in itertools.product(BLOCKS, DISPATCH):
+                cfg = {"block": block, "dispatch": dispatch, "chunk": chunk}
+                ms = self._time(model, cfg, frames, stream)
+                for t, saved in state + lam:
+                    t.copy_(saved)
+                results.append((ms, cfg))
+        model.set_constraint_order(*orig)
+        for t, saved in lam0:
+            t.copy_(saved)
+        results.sort(key=lambda r: r[0])
+        best = dict(results[0][1])
+        self.apply(model, best)
+        torch.cuda.synchronize(model.device)
+        self.cache[key] = {"best": best, "ms": results[0][0],
+                           "trials": [{"ms": ms, **cfg} for ms, cfg in results]}
+        self.save()
+        return best
+
+    @staticmethod
+    def apply(model: SimModelDO, cfg: dict) -> None:
+        t = model.topo[0] if model.topo else None
+        if t is not None and (t.constraint_order != "local" or t.chunk != cfg["chunk"]):
+            model.set_constraint_order("local", cfg["chunk"])

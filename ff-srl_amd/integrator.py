@@ -25,6 +25,18 @@ class SimIntegratorDO:
         self.block = block
         self.dispatch_sorted = dispatch_sorted
 
+    def configure(self, cfg: dict) -> None:
+        """Launch shape from the autotuner: {block, dispatch ("lpt"/"table"), ...}."""
+        self.block = int(cfg["block"])
+        self.dispatch_sorted = cfg["dispatch"] == "lpt"
+
+    def autotune(self, simModel: SimModelDO, cache_path=None, frames: int = 2) -> dict:
+        """Tune (or look up) the launch shape for this batch and use it."""
+        from .autotune import Autotuner
+        cfg = Autotuner(cache_path).tune(simModel, frames=frames)
+        self.configure(cfg)
+        return cfg
+
     def stepModel(self, simModel: SimModelDO, refit: bool = False) -> None:
         """One frame of all environments in one launch (``ffsrl_frame``)."""
         m = simModel

@@ -16,8 +16,10 @@ observations are torch tensors.  Differences by design:
 """
 from __future__ import annotations
 
+import concurrent.futures
 import ctypes as C
 import math
+import os
 from typing import Optional, Sequence
 
 import numpy as np
@@ -27,6 +29,17 @@ from ._lib import Batch, EnvDesc, ENV_DESC_FIELDS, OFF, PBD, XPBD, Params, lib
 from .scenes import TetMesh
 
 MAX_STACK = 32  # traversal stack entries per ray (render.py:424, 924)
+
+
+def _parallel(fn, items):
+    """Map over host threads: the library's host routines (BVH build,
+    constraint order) release the interpreter lock while they run."""
+    items = list(items)
+    if len(items) < 8:
+        return [fn(x) for x in items]
+    lib.load()
+    with concurrent.futures.ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(fn, items))
 LOCAL_CHUNK = 128  # constraints per colour-sorted run of the "local" order
 
 _MODES = {None: OFF, "off": OFF, "pbd": PBD, "xpbd": XPBD}
@@ -39,7 +52,7 @@ def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
 class MeshTopology:
     """Host-side topology of one mesh plus its BVH (built once)."""
 
-    def __init__(self, mesh: TetMesh, bins: int = 8, constraint_order: str = "local"):
+    def __init__(self, mesh: TetMesh, bins: int = 8, constraint_order: str = "local", chunk: int = LOCAL_CHUNK):
         self.mesh = mesh
         surf = mesh.surface_vertices()
         local = np.full(mesh.num_vertices, -1, dtype=np.int32)
@@ -78,10 +91,15 @@ class MeshTopology:
         self.level_nodes = np.concatenate(order).astype(np.int32)
         self.level_off = np.asarray(self.level_off, dtype=np.int32)
         self.tri_colors_leaf = mesh.tri_colors[tri_map] if mesh.tri_colors is not None else None
-        # Constraint order on the device (see ``constraint_order``); the
-        # frame's integer Jacobi sums make results independent of it.
-        self.tet_perm = self._order(mesh.tets, mesh, constraint_order)
-        self.edge_perm = self._order(mesh.edges, mesh, constraint_order)
+        self.set_order(constraint_order, chunk)
+
+    def set_order(self, constraint_order: str = "local", chunk: int = LOCAL_CHUNK) -> None:
+        """Constraint order on the device (see ``_order``); the frame's integer
+        Jacobi sums make results independent of it."""
+        mesh = self.mesh
+        self.constraint_order, self.chunk = constraint_order, int(chunk)
+        self.tet_perm = self._order(mesh.tets, mesh, constraint_order, chunk)
+        self.edge_perm = self._order(mesh.edges, mesh, constraint_order, chunk)
         self.tets = np.ascontiguousarray(mesh.tets[self.tet_perm]).astype(np.int32)
         self.tet_rest_volume = np.ascontiguousarray(mesh.tet_rest_volume[self.tet_perm])
         self.tet_inv_rest = np.ascontiguousarray(mesh.tet_inv_rest[self.tet_perm].reshape(-1, 9))
@@ -89,11 +107,11 @@ class MeshTopology:
         self.edge_rest = np.ascontiguousarray(mesh.edge_rest[self.edge_perm])
 
     @staticmethod
-    def _order(elems: np.ndarray, mesh: TetMesh, order: str) -> np.ndarray:
+    def _order(elems: np.ndarray, mesh: TetMesh, order: str, chunk: int = LOCAL_CHUNK) -> np.ndarray:
         """Visit order of the constraints in the frame kernel:
         "local" - Morton order of the centroids, greedy-coloured within runs of
-                  128 (position gathers of a wavefront hit few cache lines and
-                  its LDS atomics rarely collide);
+                  `chunk` (position gathers of a wavefront hit few cache lines
+                  and its LDS atomics rarely collide);
         "color" - stable sort by a greedy colour of the whole mesh;
         "mesh"  - the mesh's own order."""
         elems = np.ascontiguousarray(elems, dtype=np.int32)
@@ -111,7 +129,7 @@ class MeshTopology:
         verts = np.ascontiguousarray(mesh.verts, dtype=np.float32)
         perm = np.empty(n, np.int64)
         lib.ffsrl_order_constraints(elems.ctypes.data, n, elems.shape[1], verts.ctypes.data, mesh.num_vertices,
-                                    LOCAL_CHUNK, perm.ctypes.data)
+                                    int(chunk), perm.ctypes.data)
         return perm
 
 
@@ -161,7 +179,7 @@ class SimModelDO:
         self.params = self._make_params()
 
         self.constraint_order = constraint_order
-        self.topo = [MeshTopology(m, bins, constraint_order) for m in self.meshes]
+        self.topo = _parallel(lambda m: MeshTopology(m, bins, constraint_order), self.meshes)
         self._build_topology()
         self._build_envs()
         self._build_state(initialVelocity, tips)
@@ -340,6 +358,36 @@ class SimModelDO:
     @property
     def batch(self) -> Batch:
         return self._batch
+
+    def set_constraint_order(self, constraint_order: str = "local", chunk: int = LOCAL_CHUNK) -> None:
+        """Re-order every mesh's constraints on the device in place (the
+        autotuner's chunk knob).  Carried XPBD lambdas follow their
+        constraints; positions are unaffected."""
+        dev = self.device
+        old = [(t.tet_perm, t.edge_perm) for t in self.topo]
+        _parallel(lambda t: t.set_order(constraint_order, chunk), self.topo)
+        cat = np.concatenate
+        self.t_tets.copy_(torch.from_numpy(cat([t.tets for t in self.topo])))
+        self.t_tet_rest_volume.copy_(torch.from_numpy(cat([t.tet_rest_volume for t in self.topo])))
+        inv_rest = cat([t.tet_inv_rest for t in self.topo]).astype(np.float32)
+        self.t_tet_inv_rest.copy_(torch.from_numpy(np.ascontiguousarray(inv_rest.T)))
+        self.t_edges.copy_(torch.from_numpy(cat([t.edges for t in self.topo])))
+        self.t_edge_rest.copy_(torch.from_numpy(cat([t.edge_rest for t in self.topo])))
+        if self.lam_t is not None:
+            for k, t in enumerate(self.topo):
+                envs = np.nonzero(self.env_mesh == k)[0]
+                for lam, base_f, o, new in ((self.lam_t, "lam_tbase", old[k][0], t.tet_perm),
+                                            (self.lam_e, "lam_ebase", old[k][1], t.edge_perm)):
+                    if o.size == 0 or envs.size == 0:
+                        continue
+                    inv_new = np.empty_like(new)
+                    inv_new[new] = np.arange(new.size)
+                    dst = torch.from_numpy(inv_new[o]).to(dev)           # old position -> new position
+                    base = torch.from_numpy(self.env_desc_host[envs, ENV_DESC_FIELDS.index(base_f)].astype(np.int64))
+                    base = base.to(dev)[:, None]
+                    src = (base + torch.arange(o.size, device=dev)[None]).reshape(-1)
+                    lam[(base + dst[None]).reshape(-1)] = lam[src].clone()
+        self.constraint_order = constraint_order
 
     def dispatch_order(self) -> torch.Tensor:
         """Environments largest first: the hardware dispatcher then assigns

@@ -355,3 +355,36 @@ def test_reset_restores_initial_state(pkg, gpu):
     assert torch.equal(model.env_vertices(0), model.initialVertex[model.env_slice(0), :3])
     assert torch.equal(model.env_vertices(1), before)
     assert not model.v[model.env_slice(0)].any()
+
+
+@pytest.mark.parametrize("iters", [1, 2])
+def test_autotune_keeps_state_and_results(pkg, gpu, tmp_path, iters):
+    """Tuning restores the state it timed on, the tuned launch shape gives
+    bit-identical frames, and a second tune is served from the cache."""
+    sc = pkg.scenes
+    meshes = [sc.solid("ellipsoid", 3000, seed=31), sc.solid("box", 1200, seed=32), sc.solid("ellipsoid", 800, seed=33)]
+    kw = dict(NH, distance="xpbd", volume="xpbd", distanceCompliance=1e-6, volumeComplianceXPBD=0.0,
+              simConstraintsSteps=iters, lambdaLegacy=True)
+    env_mesh = [0, 1, 2, 1, 0, 2, 2, 1]
+    ref = pkg.SimModelDO(meshes, env_mesh, device=gpu, **kw)
+    tuned = pkg.SimModelDO(meshes, env_mesh, device=gpu, **kw)
+    integ_ref = pkg.SimIntegratorDO(gpu)
+    integ = pkg.SimIntegratorDO(gpu)
+    for _ in range(3):                                  # some state (and carried lambdas) before tuning
+        integ_ref.stepModel(ref, refit=True)
+        integ.stepModel(tuned, refit=True)
+    before = [t.clone() for t in (tuned.x, tuned.v, tuned.bvh_lo)]
+    cache = str(tmp_path / "tune.json")
+    cfg = integ.autotune(tuned, cache_path=cache)
+    print(f"\n[autotune] iterations {iters}: {cfg}")
+    assert cfg["block"] in (256, 512, 1024) and cfg["dispatch"] in ("lpt", "table") and cfg["chunk"] in (64, 128, 256)
+    for a, b in zip(before, (tuned.x, tuned.v, tuned.bvh_lo)):
+        assert torch.equal(a, b)
+    for _ in range(4):
+        integ_ref.stepModel(ref, refit=True)
+        integ.stepModel(tuned, refit=True)
+    assert torch.equal(ref.x, tuned.x) and torch.equal(ref.v, tuned.v) and torch.equal(ref.bvh_hi, tuned.bvh_hi)
+    assert bool(torch.isfinite(tuned.x).all())
+    again = pkg.autotune.Autotuner(cache)
+    assert pkg.autotune.key_of(tuned) in again.cache
+    assert again.tune(tuned) == cfg
