@@ -195,34 +195,23 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
   }
 }
 
-// Area-weighted vertex normals of one env's render mesh, summed in LDS.
+// Area-weighted vertex normals: one thread per vertex sums the area normals
+// cross(b - a, c - a) of its faces in increasing face id, then normalizes.
 __global__ __launch_bounds__(256) void k_normals(ffsrl_batch b, int env_begin, float* normals) {
-  extern __shared__ float lds[];
-  const ffsrl_env_desc e = b.envs[env_begin + blockIdx.x];
-  const int nv = e.nv;
-  float* sx = lds;
-  float* sy = lds + nv;
-  float* sz = lds + 2 * nv;
-  for (int i = threadIdx.x; i < nv; i += 256) sx[i] = sy[i] = sz[i] = 0.0f;
-  __syncthreads();
+  const ffsrl_env_desc e = b.envs[env_begin + blockIdx.y];
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v >= e.nv) return;
   const int64_t vb = e.vbase;
-  for (int f = threadIdx.x; f < e.nf; f += 256) {
-    const int4 t = ldi4(b.tris, (int64_t)e.fbase + f);
+  const int32_t* off = b.vf_off + e.vtopo_base;
+  const int32_t* idx = b.vf_idx + 3 * (int64_t)e.fbase;
+  f3 s = mk(0.0f, 0.0f, 0.0f);
+  for (int k = off[v]; k < off[v + 1]; ++k) {
+    const int4 t = ldi4(b.tris, (int64_t)e.fbase + idx[k]);
     const f3 A = xyz(ld4(b.x, vb + t.x)), B = xyz(ld4(b.x, vb + t.y)), C = xyz(ld4(b.x, vb + t.z));
-    const f3 n = cross(B - A, C - A);
-    const int ids[3] = {t.x, t.y, t.z};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      atomicAdd(&sx[ids[k]], n.x);
-      atomicAdd(&sy[ids[k]], n.y);
-      atomicAdd(&sz[ids[k]], n.z);
-    }
+    s = s + cross(B - A, C - A);
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < nv; i += 256) {
-    const f3 n = normalize(mk(sx[i], sy[i], sz[i]));
-    st4(normals, vb + i, make_float4(n.x, n.y, n.z, 0.0f));
-  }
+  const f3 n = normalize(s);
+  st4(normals, vb + v, make_float4(n.x, n.y, n.z, 0.0f));
 }
 
 }  // namespace ffsrl
@@ -285,14 +274,13 @@ int ffsrl_render_envs(const ffsrl_batch* b, const ffsrl_camera* cam, const int32
 }
 
 int ffsrl_vertex_normals(const ffsrl_batch* b, int32_t eb, int32_t ee, float* normals, void* stream) {
-  if (!b || !b->envs || !b->x || !b->tris || !normals) return fail(FFSRL_E_ARG, "normals: NULL argument");
+  if (!b || !b->envs || !b->x || !b->tris || !normals || !b->vf_off || !b->vf_idx)
+    return fail(FFSRL_E_ARG, "normals: NULL argument (positions, triangles, vertex-face adjacency, output)");
   if (eb < 0 || ee > b->n_env || eb > ee) return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", eb, ee, b->n_env);
-  if (b->max_nv > 160 * 1024 / 12) return fail(FFSRL_E_LIMIT, "max_nv %d exceeds the LDS capacity of normals", b->max_nv);
   if (ee == eb) return 0;
-  const int lds = 12 * (b->max_nv < 1 ? 1 : b->max_nv);
-  FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_normals), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                lds));
-  hipLaunchKernelGGL(k_normals, dim3(ee - eb), dim3(256), lds, as_stream(stream), *b, eb, normals);
+  if (ee - eb > 65535) return fail(FFSRL_E_LIMIT, "at most 65535 environments per normals call");
+  const int nv = b->max_nv < 1 ? 1 : b->max_nv;
+  hipLaunchKernelGGL(k_normals, dim3((nv + 255) / 256, ee - eb), dim3(256), 0, as_stream(stream), *b, eb, normals);
   FFSRL_HIP(hipGetLastError());
   return 0;
 }

@@ -91,6 +91,15 @@ class MeshTopology:
         self.level_nodes = np.concatenate(order).astype(np.int32)
         self.level_off = np.asarray(self.level_off, dtype=np.int32)
         self.tri_colors_leaf = mesh.tri_colors[tri_map] if mesh.tri_colors is not None else None
+        # vertex -> faces adjacency for the normals: faces of each vertex in
+        # increasing original face id, stored as leaf positions
+        leaf_of = np.empty(nf, np.int64)
+        leaf_of[tri_map] = np.arange(nf)
+        corner_v = mesh.tris.reshape(-1).astype(np.int64)
+        order_vf = np.argsort(corner_v, kind="stable")            # face ids stay increasing per vertex
+        self.vf_idx = leaf_of[order_vf // 3].astype(np.int32)
+        self.vf_off = np.zeros(mesh.num_vertices + 1, np.int32)
+        np.cumsum(np.bincount(corner_v, minlength=mesh.num_vertices), out=self.vf_off[1:])
         self.set_order(constraint_order, chunk)
 
     def set_order(self, constraint_order: str = "local", chunk: int = LOCAL_CHUNK) -> None:
@@ -213,7 +222,7 @@ class SimModelDO:
 
     def _build_topology(self):
         dev = self.device
-        off = {"e": [0], "t": [0], "f": [0], "n": [0], "l": [0]}
+        off = {"e": [0], "t": [0], "f": [0], "n": [0], "l": [0], "vt": [0]}
         for t in self.topo:
             m = t.mesh
             off["e"].append(off["e"][-1] + m.num_edges)
@@ -221,6 +230,7 @@ class SimModelDO:
             off["f"].append(off["f"][-1] + m.num_tris)
             off["n"].append(off["n"][-1] + t.num_nodes)
             off["l"].append(off["l"][-1] + t.level_off.size)
+            off["vt"].append(off["vt"][-1] + t.vf_off.size)
         self.mesh_off = {k: np.asarray(v, dtype=np.int64) for k, v in off.items()}
         cat = np.concatenate
         self.t_edges = torch.from_numpy(cat([t.edges for t in self.topo])).to(dev)
@@ -234,6 +244,8 @@ class SimModelDO:
         self.t_node_first = torch.from_numpy(cat([t.first for t in self.topo])).to(dev)
         self.t_node_count = torch.from_numpy(cat([t.count for t in self.topo])).to(dev)
         self.t_level_nodes = torch.from_numpy(cat([t.level_nodes for t in self.topo])).to(dev)
+        self.t_vf_off = torch.from_numpy(cat([t.vf_off for t in self.topo])).to(dev)
+        self.t_vf_idx = torch.from_numpy(cat([t.vf_idx for t in self.topo])).to(dev)
         self.level_off_host = np.ascontiguousarray(cat([t.level_off for t in self.topo]).astype(np.int32))
         self.t_level_off = torch.from_numpy(self.level_off_host).to(dev)
         tc = [t.tri_colors_leaf if t.tri_colors_leaf is not None else np.ones((t.mesh.num_tris, 3), np.float32)
@@ -266,6 +278,7 @@ class SimModelDO:
             # mesh diagonal rounded down to a power of two (range 8..16 diagonals)
             diag = max(m.bbox_diagonal(), 1e-6)
             row[F["dp_exp"]] = int(math.floor(math.log2(diag))) - 27
+            row[F["vtopo_base"]] = self.mesh_off["vt"][k]
             vb += m.num_vertices
             nb += t.num_nodes
             if carry:
@@ -332,6 +345,8 @@ class SimModelDO:
         b.node_count = _ptr(self.t_node_count)
         b.level_nodes = _ptr(self.t_level_nodes)
         b.level_off = _ptr(self.t_level_off)
+        b.vf_off = _ptr(self.t_vf_off)
+        b.vf_idx = _ptr(self.t_vf_idx)
         b.x = _ptr(self.x)
         b.p = _ptr(self.p)
         b.v = _ptr(self.v)

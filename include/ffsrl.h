@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 2
+#define FFSRL_ABI_VERSION 3
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -37,7 +37,7 @@ extern "C" {
 #define FFSRL_PBD 1   /* stiffness form: distanceConstraints / volumeConstraints */
 #define FFSRL_XPBD 2  /* compliance form: *ConstraintsXPBD */
 
-/* One environment of the CSR environment table (64 bytes, int32 fields).
+/* One environment of the CSR environment table (80 bytes, int32 fields).
  * Ranges index the shared topology arrays of ffsrl_batch; state ranges index
  * the per-environment state arrays. */
 typedef struct ffsrl_env_desc {
@@ -52,6 +52,8 @@ typedef struct ffsrl_env_desc {
   int32_t lam_tbase;       /* first tet lambda of this env */
   int32_t dp_exp;          /* quantum of the fixed-point Jacobi sums of the batched
                               frame: 2^dp_exp (about 2^-27 x bbox diagonal) */
+  int32_t vtopo_base;      /* first vf_off entry of the env's mesh */
+  int32_t reserved0, reserved1, reserved2;
 } ffsrl_env_desc;
 
 /* Device arrays of a batch of environments. */
@@ -72,6 +74,9 @@ typedef struct ffsrl_batch {
   const int32_t* node_count;      /* [N] triangle count of a leaf, 0 for inner nodes */
   const int32_t* level_nodes;     /* [N] node ids grouped by level */
   const int32_t* level_off;       /* per env lvlbase: offsets into level_nodes (relative to nodebase) */
+  const int32_t* vf_off;          /* per mesh [nv+1] at vtopo_base: CSR offsets of each vertex's faces */
+  const int32_t* vf_idx;          /* per mesh [3F] at 3*fbase: leaf positions of a vertex's faces,
+                                     in increasing original face id */
   /* state (per environment) */
   float* x;                       /* float4 [V] positions + inverse mass */
   float* p;                       /* float4 [V] predicted positions + inverse mass */
@@ -180,7 +185,10 @@ int ffsrl_render_envs(const ffsrl_batch* b, const ffsrl_camera* cam, const int32
                       const float* tcolor, int32_t* hit, float* thit, void* stream);
 
 /* Area-weighted vertex normals (float4 [V]) of the render mesh.  Replaces
- * zeroVec3Array + vertex_normal_sum_kernel + normalize_kernel (render.py:81-121). */
+ * zeroVec3Array + vertex_normal_sum_kernel + normalize_kernel (render.py:81-121).
+ * Each vertex sums the area normals of its faces in increasing face id (the
+ * vf_off / vf_idx adjacency), the order in which the reference's atomics
+ * accumulate when its threads run in tid order, so the sums are deterministic. */
 int ffsrl_vertex_normals(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, float* normals,
                          void* stream);
 

@@ -208,6 +208,36 @@ def render_depth(H, W, origin, dirs, bvh, aabb_min, aabb_max, faces, vertex, dty
     return depth, hit, t, ms.value
 
 
+def vertex_normals(faces, vertex, dtype=np.float32):
+    """Restated zeroVec3Array + vertex_normal_sum_kernel + normalize_kernel
+    (render.py:81-121): area-weighted normals summed in face order."""
+    faces = np.ascontiguousarray(np.asarray(faces).reshape(-1, 3), dtype=np.int32)
+    vert = np.ascontiguousarray(vertex, dtype=dtype)
+    out = np.zeros_like(vert)
+    sfx = "_f32" if dtype == np.float32 else "_f64"
+    getattr(lib(), "oracle_vertex_normals" + sfx)(C.c_long(faces.shape[0]), _p(faces), C.c_long(vert.shape[0]),
+                                                   _p(vert), _p(out))
+    return out
+
+
+def render_rgb(H, W, mode, origin, dirs, bvh, aabb_min, aabb_max, faces, vertex, normals, vcolor=None, tcolor=None,
+               light=(3.0, 0.0, 0.0), intensity=0.3, dtype=np.float32):
+    """Restated drawSimBVHKernelManyEnvs colour modes (0 grayscale, 1 per-face
+    colour, 2 normal map, 3 vertex colour) + per-channel downsample.
+    Returns rgb (H, W, 3)."""
+    rgb = np.zeros((H, W, 3), dtype)
+    sfx = "_f32" if dtype == np.float32 else "_f64"
+    fn = getattr(lib(), "oracle_render_rgb" + sfx)
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=dtype)
+    arrs = [f(origin), f(np.asarray(dirs)[:, :3]), f(aabb_min), f(aabb_max), i32(bvh["left"]), i32(bvh["first"]),
+            i32(bvh["count"]), i32(bvh["tri_map"]), i32(np.asarray(faces).reshape(-1)), f(vertex), f(normals),
+            f(vcolor), f(tcolor), f(np.asarray(light))]
+    fn(C.c_int(H), C.c_int(W), C.c_int(mode), *[_p(a) for a in arrs], (C.c_float if dtype == np.float32 else
+                                                                      C.c_double)(intensity), _p(rgb))
+    return rgb
+
+
 # ------------------------------------------------------------ BVH build
 def bvh_build(vertex, faces, bins=8):
     """NumPy restatement of SimBVH.buildBVH (bvh.py:456-518) with

@@ -507,6 +507,83 @@ void FN(oracle_render_depth)(int H, int W, const REAL* origin, const REAL* dirs,
   free(ray);
 }
 
+
+/* zeroVec3Array + vertex_normal_sum_kernel + normalize_kernel (render.py:81-121):
+ * area-weighted vertex normals, summed in serial face (tid) order.
+ * faces: F x 3, vertex / normals: nv x 3. */
+void FN(oracle_vertex_normals)(long nf, const int* faces, long nv, const REAL* vertex, REAL* normals) {
+  for (long v = 0; v < nv; ++v) FN(st)(normals, v, FN(v3)(R(0), R(0), R(0)));
+  for (long tid = 0; tid < nf; ++tid) {
+    int i = faces[3 * tid], j = faces[3 * tid + 1], k = faces[3 * tid + 2];
+    V3 a = FN(ld)(vertex, i), b = FN(ld)(vertex, j), c = FN(ld)(vertex, k);
+    V3 n = FN(cross)(FN(sub)(b, a), FN(sub)(c, a));
+    FN(st)(normals, i, FN(add)(FN(ld)(normals, i), n));
+    FN(st)(normals, j, FN(add)(FN(ld)(normals, j), n));
+    FN(st)(normals, k, FN(add)(FN(ld)(normals, k), n));
+  }
+  for (long v = 0; v < nv; ++v) FN(st)(normals, v, FN(normalize)(FN(ld)(normals, v)));
+}
+
+/* drawSimBVHKernelManyEnvs (render.py:276-409) colour modes for one
+ * environment, plus downsample_kernel (render.py:699-724) per channel.
+ *   mode 0 grayscale, 3 vertex colour, 1 per-triangle colour (stands in for
+ *   the texture lookup, indexed by face id), each times the Lambert term
+ *   max(0, intensity * dot(normal, light)) of the single light whose
+ *   direction is the raw light position (render.py:387-393); 2 normal map.
+ * Barycentrics from cross-product lengths, normal = u na + v nb + w nc
+ * (render.py:329-350).  Each channel is clamped to 1.  rgb: H x W x 3. */
+void FN(oracle_render_rgb)(int H, int W, int mode, const REAL* origin, const REAL* dirs,
+                           const REAL* aabbMin, const REAL* aabbMax, const int* leftNode,
+                           const int* firstTri, const int* triCount, const int* triMap,
+                           const int* faces, const REAL* vertex, const REAL* normals, const REAL* vcolor,
+                           const REAL* tcolor, const REAL* light, REAL intensity, REAL* rgb) {
+  int RW = 2 * W, RH = 2 * H, ms = 0;
+  V3 o = FN(v3)(origin[0], origin[1], origin[2]);
+  V3 L = FN(v3)(light[0], light[1], light[2]);
+  REAL* ray = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)RW * RH);
+  for (int r = 0; r < RW * RH; ++r) {
+    REAL t;
+    V3 d = FN(ld)(dirs, r);
+    int h = FN(oracle_ray)(o, d, aabbMin, aabbMax, leftNode, firstTri, triCount, triMap, faces, vertex, &t, &ms);
+    V3 color = FN(v3)(R(0), R(0), R(0));
+    if (h > -1) {
+      int i = faces[3 * h], j = faces[3 * h + 1], k = faces[3 * h + 2];
+      V3 a = FN(ld)(vertex, i), b = FN(ld)(vertex, j), c = FN(ld)(vertex, k);
+      V3 p = FN(add)(o, FN(mul)(d, t));
+      REAL area = FN(length)(FN(cross)(FN(sub)(b, a), FN(sub)(c, a)));
+      REAL w = FN(length)(FN(cross)(FN(sub)(b, a), FN(sub)(p, a))) / area;
+      REAL v = FN(length)(FN(cross)(FN(sub)(p, a), FN(sub)(c, a))) / area;
+      REAL u = R(1.0) - w - v;
+      V3 normal = FN(add)(FN(add)(FN(mul)(FN(ld)(normals, i), u), FN(mul)(FN(ld)(normals, j), v)),
+                          FN(mul)(FN(ld)(normals, k), w));
+      if (mode == 2) {
+        color = FN(add)(FN(mul)(normal, R(0.5)), FN(v3)(R(0.5), R(0.5), R(0.5)));
+      } else {
+        if (mode == 0) color = FN(v3)(R(1), R(1), R(1));
+        else if (mode == 1) color = FN(ld)(tcolor, h);
+        else color = FN(divs)(FN(add)(FN(add)(FN(ld)(vcolor, i), FN(ld)(vcolor, j)), FN(ld)(vcolor, k)), R(3.0));
+        REAL val = intensity * FN(dot)(normal, L);
+        if (val < R(0)) val = R(0);
+        REAL lambert = R(0.0) + val;
+        color = FN(mul)(color, lambert);
+      }
+      if (color.x > R(1)) color.x = R(1);
+      if (color.y > R(1)) color.y = R(1);
+      if (color.z > R(1)) color.z = R(1);
+    }
+    FN(st)(ray, r, color);
+  }
+  for (int py = 0; py < H; ++py)
+    for (int px = 0; px < W; ++px) {
+      int start = py * 2 * RW + px * 2;
+      V3 c = FN(v3)(R(0), R(0), R(0));
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) c = FN(add)(c, FN(ld)(ray, start + i * RW + j));
+      FN(st)(rgb, (long)py * W + px, FN(divs)(c, R(4.0)));
+    }
+  free(ray);
+}
+
 #undef V3
 #undef FEPS
 #undef R

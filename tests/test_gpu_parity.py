@@ -388,3 +388,39 @@ def test_autotune_keeps_state_and_results(pkg, gpu, tmp_path, iters):
     again = pkg.autotune.Autotuner(cache)
     assert pkg.autotune.key_of(tuned) in again.cache
     assert again.tune(tuned) == cfg
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_rgb_modes_and_normals_bit_exact(pkg, orc, gpu, mode):
+    """Colour renders (grayscale, per-triangle colour, normal map, vertex
+    colour, each Lambert-lit by the raw light position) and the vertex normals
+    equal the oracle's on the same deformed positions."""
+    sc = pkg.scenes
+    meshes = [sc.solid("ellipsoid", 3000, seed=41), sc.solid("box", 2000, seed=42)]
+    tips = np.stack([np.repeat((sc.top_point(m) + np.array([0.0, 0.05, 0.0]))[None], 4, axis=0) for m in meshes])
+    model = pkg.SimModelDO(meshes, [0, 1], device=gpu, tips=tips.astype(np.float32), **NH_DV)
+    for i, m in enumerate(meshes):
+        model.set_grasp(i, sc.nearest_surface_vertices(m, tips[i, 2], 4))
+    integ = pkg.SimIntegratorDO(gpu)
+    for _ in range(5):
+        integ.stepModel(model, refit=True)
+    ren = pkg.RaycastRendererDO(gpu, model, cameraPos=(0.0, 1.6, 5.5), resolution=64, lightPos=(3.0, 2.0, 4.0),
+                                lightIntensity=0.3, render_mode=mode)
+    ren.lookAt((0.0, 0.45, 0.0))
+    rgb = ren.render_rgb()
+    torch.cuda.synchronize()
+    for e, m in enumerate(meshes):
+        x = model.env_vertices(e).cpu().numpy()
+        nrm = orc.vertex_normals(m.tris, x)
+        assert np.array_equal(model.normals[model.env_slice(e), :3].cpu().numpy(), nrm)
+        topo = model.topo[e]
+        bvh = _bvh_dict(topo)
+        mn, mx = orc.refit(bvh, m.tris, x)
+        ref = orc.render_rgb(64, 64, mode, np.asarray(ren.camera.origin[:], np.float32), ren.dirs.cpu().numpy(),
+                             bvh, mn, mx, m.tris, x, nrm, vcolor=m.colors, tcolor=m.tri_colors,
+                             light=np.asarray(ren.camera.light_dir[:], np.float32), intensity=ren.camera.light_intensity)
+        got = rgb[e].cpu().numpy()
+        lit = float((ref.sum(-1) > 0).mean())
+        print(f"\n[rgb mode {mode}] env {e}: lit pixels {lit:.2f}, max |gpu-oracle| {np.abs(got - ref).max():.2e}")
+        assert lit > 0.05
+        assert np.array_equal(got, ref)
