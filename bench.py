@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host threads of the multi-core CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--serve-seconds", type=float, default=10.0,
@@ -94,11 +96,16 @@ def algorithmic_bytes(w, model):
     return total
 
 
-def cpu_baseline(pkg, orc, w, model, ren, seconds, seed):
-    """The oracle (1 core, fp32) on a random sample of the same envs: one frame
-    (physics + refit + depth render) per sampled env until `seconds` elapse."""
+def cpu_baseline(pkg, orc, w, model, ren, seconds, seed, threads):
+    """The oracle (fp32 serial C restatement) on a random sample of the same
+    envs: one frame (physics + refit + depth, + normals and RGB when the
+    workload renders colour) per sampled env.  Envs are independent, so the
+    multi-core figure runs one env per host thread (the library releases the
+    interpreter lock); the 1-core figure runs them one after another."""
+    import concurrent.futures
+    import threading
     rng = np.random.default_rng(seed + 17)
-    order = rng.permutation(w.num_envs)
+    order = list(rng.permutation(w.num_envs))
     kw = w.model_kwargs
     cfg = orc.StepConfig(substeps=kw["simSubsteps"], iterations=kw["simConstraintsSteps"],
                          fps=kw["simFrameRate"], gravity=kw["environmentGravity"],
@@ -112,9 +119,13 @@ def cpu_baseline(pkg, orc, w, model, ren, seconds, seed):
                          vol_compliance=kw.get("volumeComplianceXPBD", 0.0), drag_ks=kw.get("globalKsDrag", 1.0))
     dirs = ren.dirs.cpu().numpy()
     origin = np.asarray(ren.camera.origin[:], np.float32)
-    done, tets = 0, 0
-    t0 = time.perf_counter()
-    for e in order:
+    light = np.asarray(ren.camera.light_dir[:], np.float32)
+    bvhs = {}
+    for k in set(w.env_mesh[e] for e in order[:4096]):
+        t = model.topo[k]
+        bvhs[k] = {"left": t.left, "first": t.first, "count": t.count, "level": t.level, "tri_map": t.tri_map}
+
+    def one(e):
         k = w.env_mesh[e]
         m = w.meshes[k]
         sim = orc.OracleSim([m], [0], np.float32, tips=w.tips[e:e + 1] if w.tips is not None else None,
@@ -122,23 +133,51 @@ def cpu_baseline(pkg, orc, w, model, ren, seconds, seed):
         if w.grasp is not None and w.grasp[e] is not None:
             sim.drag[w.grasp[e]] = 1.0
         sim.step(cfg, 1)
-        t = model.topo[k]
-        bvh = {"left": t.left, "first": t.first, "count": t.count, "level": t.level, "tri_map": t.tri_map}
+        bvh = bvhs[k]
         mn, mx = orc.refit(bvh, m.tris, sim.x)
         orc.render_depth(ren.height, ren.width, origin, dirs, bvh, mn, mx, m.tris, sim.x)
-        done += 1
-        tets += m.num_tets
-        if time.perf_counter() - t0 >= seconds and done >= 4:
-            break
-    el = time.perf_counter() - t0
-    mean_t_sample = tets / done
-    mean_t_pop = sum(w.meshes[k].num_tets for k in w.env_mesh) / w.num_envs
-    # scale the sampled per-env cost to the population's mean env size
-    rate = done / el * (mean_t_sample / mean_t_pop)
-    return {"value": rate, "unit": "env-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle fp32 serial C restatement, {done} randomly sampled envs x 1 frame "
-                      f"(physics + refit + {ren.height}x{ren.width} depth) in {el:.1f} s, scaled by mean tets "
-                      f"{mean_t_sample:.0f}/{mean_t_pop:.0f}"}
+        if w.rgb:
+            nrm = orc.vertex_normals(m.tris, sim.x)
+            orc.render_rgb(ren.height, ren.width, 1, origin, dirs, bvh, mn, mx, m.tris, sim.x, nrm,
+                           tcolor=m.tri_colors, light=light, intensity=ren.camera.light_intensity)
+        return m.num_tets
+
+    def run(workers, budget, envs):
+        it = iter(envs)
+        lock = threading.Lock()
+        done, tets = [0], [0]
+        t0 = time.perf_counter()
+
+        def worker():
+            while time.perf_counter() - t0 < budget:
+                with lock:
+                    e = next(it, None)
+                if e is None:
+                    return
+                nt = one(e)
+                with lock:
+                    done[0] += 1
+                    tets[0] += nt
+
+        with concurrent.futures.ThreadPoolExecutor(workers) as ex:
+            for f in [ex.submit(worker) for _ in range(workers)]:
+                f.result()
+        el = time.perf_counter() - t0
+        mean_t_sample = tets[0] / max(done[0], 1)
+        mean_t_pop = sum(w.meshes[k].num_tets for k in w.env_mesh) / w.num_envs
+        return done[0] / el * (mean_t_sample / mean_t_pop), done[0], el, mean_t_sample, mean_t_pop
+
+    what = f"physics + refit + {ren.height}x{ren.width} depth{' + normals + RGB' if w.rgb else ''}"
+    r1, d1, el1, ms1, mp = run(1, seconds / 3.0, order)
+    rN, dN, elN, msN, _ = run(threads, 2.0 * seconds / 3.0, order[d1:] + order[:d1])
+    one_core = {"value": r1, "unit": "env-steps/s", "cores": 1, "kind": "port",
+                "sample": f"oracle fp32 serial C restatement, {d1} randomly sampled envs x 1 frame ({what}) "
+                          f"in {el1:.1f} s, scaled by mean tets {ms1:.0f}/{mp:.0f}"}
+    all_cores = {"value": rN, "unit": "env-steps/s", "cores": threads, "kind": "port",
+                 "sample": f"oracle fp32 C restatement, one env per host thread, {dN} randomly sampled envs x "
+                           f"1 frame ({what}) in {elN:.1f} s on {threads} threads, scaled by mean tets "
+                           f"{msN:.0f}/{mp:.0f}"}
+    return all_cores, one_core
 
 
 def serving_run(pkg, wl, args, dev):
@@ -208,10 +247,22 @@ def main():
     stream = torch.cuda.current_stream()
     s = stream.cuda_stream
 
+    cam_rgb = pkg._lib.Camera()
+    C.pointer(cam_rgb)[0] = ren.camera
+    cam_rgb.mode = pkg.RenderMode.texture              # per-triangle colours, Lambert-lit
+
+    def render(sh, eb=0, ee=n):
+        lib.ffsrl_render(model.batch, ren.camera, eb, ee, depth_out[eb:].data_ptr(), None, None, None, None, None,
+                         None, sh)
+        if w.rgb:
+            lib.ffsrl_vertex_normals(model.batch, eb, ee, model.normals.data_ptr(), sh)
+            lib.ffsrl_render(model.batch, cam_rgb, eb, ee, None, ren.pixels[eb:].data_ptr(), model.normals.data_ptr(),
+                             model.vertexColors.data_ptr(), model.t_tri_colors.data_ptr(), None, None, sh)
+
     def step(f, sh):
         lib.ffsrl_move_tools(model.batch, walk[f].data_ptr(), None, None, sh)
         integ.stepModel(model, refit=True)       # on torch's current stream, which is sh
-        lib.ffsrl_render(model.batch, ren.camera, 0, n, depth_out.data_ptr(), None, None, None, None, None, None, sh)
+        render(sh)
 
     pkg.SimBVH(dev, model)
     tuned = None
@@ -231,6 +282,16 @@ def main():
     for f in range(args.warmup):
         step(f, s)
     torch.cuda.synchronize()
+    # every timed form (eager, graph, plain) starts from this state
+    state = [t for t in (model.x, model.p, model.v, model.drag, model.laparoscopeTip, model.bvh_lo, model.bvh_hi,
+                         model.lam_t, model.lam_e) if t is not None]
+    snapshot = [(t, t.clone()) for t in state]
+
+    def restore():
+        torch.cuda.synchronize()
+        for t, saved in snapshot:
+            t.copy_(saved)
+        torch.cuda.synchronize()
 
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
     torch.cuda.synchronize()
@@ -243,7 +304,7 @@ def main():
         e1.record(stream)
         integ.stepModel(model, refit=True)
         e2.record(stream)
-        lib.ffsrl_render(model.batch, ren.camera, 0, n, depth_out.data_ptr(), None, None, None, None, None, None, s)
+        render(s)
         e3.record(stream)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -261,13 +322,14 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded meshes, seeded tool random walk)",
         "config": {"workload": w.name, "envs": n, **{k: int(v) for k, v in w.totals().items()},
                    "substeps": w.model_kwargs["simSubsteps"], "iterations": w.model_kwargs["simConstraintsSteps"],
-                   "render": f"{w.resolution}x{w.resolution} depth (2x2 supersampled)", "block": integ.block,
+                   "render": f"{w.resolution}x{w.resolution} depth{' + RGB (per-triangle colours)' if w.rgb else ''} "
+                             f"(2x2 supersampled)", "block": integ.block,
                    "dispatch": "lpt" if integ.dispatch_sorted else "table",
                    "chunk": model.topo[0].chunk, "autotuned": tuned is not None,
                    "form": "batched (1 frame kernel + 1 render kernel per step)"},
         "p50_step_ms": float(np.percentile(step_ms, 50)), "p99_step_ms": float(np.percentile(step_ms, 99)),
         "frame_kernel_ms": float(frame_ms.mean()), "render_kernel_ms": float(render_ms.mean()),
-        "rays_per_s": float(4 * w.resolution ** 2 * n / (render_ms.mean() * 1e-3)),
+        "rays_per_s": float((2 if w.rgb else 1) * 4 * w.resolution ** 2 * n / (render_ms.mean() * 1e-3)),
         "finite": finite, "setup_s": setup_s,
     }
     alg = algorithmic_bytes(w, model)
@@ -278,6 +340,7 @@ def main():
 
     if not args.no_graph:
         log("graph replay")
+        restore()
         g = C.c_void_p()
         cs = torch.cuda.Stream()
         cs.wait_stream(stream)
@@ -295,6 +358,7 @@ def main():
 
     if not args.no_plain and args.plain_frames > 0:
         log("plain form")
+        restore()
         model.enable_plain()
 
         def plain_step(f):
@@ -302,8 +366,7 @@ def main():
             integ.stepModelPlain(model)
             lib.ffsrl_bvh_refit_plain(model.batch, 0, n, s)
             for e in range(n):
-                lib.ffsrl_render(model.batch, ren.camera, e, e + 1, depth_out[e].data_ptr(), None, None, None, None,
-                                 None, None, s)
+                render(s, e, e + 1)
 
         plain_step(0)
         torch.cuda.synchronize()
@@ -327,7 +390,8 @@ def main():
         log("cpu baseline")
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
-        out["cpu_baseline"] = cpu_baseline(pkg, orc, w, model, ren, args.cpu_seconds, args.seed)
+        out["cpu_baseline"], out["cpu_baseline_1core"] = cpu_baseline(pkg, orc, w, model, ren, args.cpu_seconds,
+                                                                      args.seed, args.cpu_threads)
     print(json.dumps(out), flush=True)
 
 
