@@ -24,6 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_frame launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (x2.0 and x1.0 calibration factors from a 256 MiB copy), by
+# workload; profiles/k_frame_k_render_mixed1024.md holds the runs.
+MEASURED_TRAFFIC = {"mixed-1024env-logU[1000,40000]tets": 12.22e9}
 
 
 def log(msg):
@@ -335,8 +339,8 @@ def main():
     alg = algorithmic_bytes(w, model)
     achieved = alg / (frame_ms.mean() * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_frame (physics + refit)",
-                       "algorithmic_bytes_per_launch": alg}
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(w.name),
+                       "kernel": "k_frame (physics + refit)", "algorithmic_bytes_per_launch": alg}
 
     if not args.no_graph:
         log("graph replay")
