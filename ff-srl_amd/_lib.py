@@ -34,7 +34,7 @@ class Batch(C.Structure):
     _fields_ = [
         ("n_env", C.c_int32), ("max_nv", C.c_int32), ("n_tet_total", C.c_int64),
         ("envs", _vp),
-        ("edges", _vp), ("edge_rest", _vp), ("tets", _vp), ("tet_rest_volume", _vp),
+        ("edges", _vp), ("edge_rest", _vp), ("tets", _vp), ("tets16", _vp), ("edges16", _vp), ("tet_rest_volume", _vp),
         ("tet_inv_rest", _vp), ("tris", _vp), ("node_left", _vp), ("node_first", _vp),
         ("node_count", _vp), ("level_nodes", _vp), ("level_off", _vp), ("vf_off", _vp), ("vf_idx", _vp),
         ("x", _vp), ("p", _vp), ("v", _vp), ("drag", _vp), ("lam_e", _vp), ("lam_t", _vp),

@@ -107,15 +107,19 @@ struct EdgeIn {
   float rest;
 };
 
+// The frame reads the 16-bit packed ids (half the index bytes of int4 / int2;
+// an env of the batched frame has at most 10,240 vertices).
 __device__ __forceinline__ TetIn load_tet(const ffsrl_batch& b, int64_t ti) {
   TetIn t;
-  t.q = ldi4(b.tets, ti);
+  const uint64_t w = b.tets16[ti];
+  t.q = make_int4((int)(w & 0xffffu), (int)((w >> 16) & 0xffffu), (int)((w >> 32) & 0xffffu), (int)(w >> 48));
   t.rest = b.tet_rest_volume[ti];
   return t;
 }
 __device__ __forceinline__ EdgeIn load_edge(const ffsrl_batch& b, int64_t ei) {
   EdgeIn t;
-  t.q = ldi2(b.edges, ei);
+  const uint32_t w = b.edges16[ei];
+  t.q = make_int2((int)(w & 0xffffu), (int)(w >> 16));
   t.rest = b.edge_rest[ei];
   return t;
 }
@@ -585,6 +589,8 @@ static int frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* e
     return fail(FFSRL_E_ARG, "XPBD with iterations > 1 needs lambda arrays");
   if (refit && (!b->tris || !b->node_left || !b->level_nodes || !b->level_off || !b->bvh_lo || !b->bvh_hi))
     return fail(FFSRL_E_ARG, "refit requested without BVH arrays");
+  if ((prm->vol_mode || prm->nh_on) && !b->tets16) return fail(FFSRL_E_ARG, "batched frame needs packed tet ids");
+  if (prm->dist_mode && !b->edges16) return fail(FFSRL_E_ARG, "batched frame needs packed edge ids");
   if (n < 0 || n > b->n_env) return fail(FFSRL_E_ARG, "%d environments listed, batch has %d", n, b->n_env);
   if (n == 0) return 0;
   const int lds = lds_bytes_for(b->max_nv < 1 ? 1 : b->max_nv);

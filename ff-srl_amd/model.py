@@ -31,6 +31,20 @@ from .scenes import TetMesh
 MAX_STACK = 32  # traversal stack entries per ray (render.py:424, 924)
 
 
+def _pack16(ids: np.ndarray) -> np.ndarray:
+    """Rows of 4 (tets) or 2 (edges) mesh-local ids packed as 16-bit fields,
+    first id in the low bits: int64 [n] for 4 ids, int32 [n] for 2.  Ids of
+    meshes with more than 65,536 vertices are stored modulo 2^16 (such meshes
+    exceed the batched frame's 10,240-vertex limit and use the plain form)."""
+    ids = np.asarray(ids).astype(np.uint64) & np.uint64(0xffff)
+    if ids.ndim != 2 or ids.shape[1] not in (2, 4):
+        raise ValueError("expected (n, 2) or (n, 4) ids")
+    out = np.zeros(ids.shape[0], np.uint64)
+    for k in range(ids.shape[1]):
+        out |= ids[:, k] << np.uint64(16 * k)
+    return out.view(np.int64) if ids.shape[1] == 4 else out.astype(np.uint32).view(np.int32)
+
+
 def _parallel(fn, items):
     """Map over host threads: the library's host routines (BVH build,
     constraint order) release the interpreter lock while they run."""
@@ -236,7 +250,10 @@ class SimModelDO:
         self.t_edges = torch.from_numpy(cat([t.edges for t in self.topo])).to(dev)
         self.t_edge_rest = torch.from_numpy(cat([t.edge_rest for t in self.topo])).to(dev)
         self.t_tets = torch.from_numpy(cat([t.tets for t in self.topo])).to(dev)
+        self.t_tets16 = torch.from_numpy(_pack16(cat([t.tets for t in self.topo]))).to(dev)
+        self.t_edges16 = None
         self.t_tet_rest_volume = torch.from_numpy(cat([t.tet_rest_volume for t in self.topo])).to(dev)
+        self.t_edges16 = torch.from_numpy(_pack16(cat([t.edges for t in self.topo]))).to(dev)
         inv_rest = cat([t.tet_inv_rest for t in self.topo]).astype(np.float32)
         self.t_tet_inv_rest = torch.from_numpy(np.ascontiguousarray(inv_rest.T)).to(dev)
         self.t_tris = torch.from_numpy(cat([t.tris_leaf for t in self.topo])).to(dev)
@@ -337,6 +354,8 @@ class SimModelDO:
         b.edges = _ptr(self.t_edges)
         b.edge_rest = _ptr(self.t_edge_rest)
         b.tets = _ptr(self.t_tets)
+        b.tets16 = _ptr(self.t_tets16)
+        b.edges16 = _ptr(self.t_edges16)
         b.tet_rest_volume = _ptr(self.t_tet_rest_volume)
         b.tet_inv_rest = _ptr(self.t_tet_inv_rest)
         b.tris = _ptr(self.t_tris)
@@ -383,6 +402,8 @@ class SimModelDO:
         _parallel(lambda t: t.set_order(constraint_order, chunk), self.topo)
         cat = np.concatenate
         self.t_tets.copy_(torch.from_numpy(cat([t.tets for t in self.topo])))
+        self.t_tets16.copy_(torch.from_numpy(_pack16(cat([t.tets for t in self.topo]))))
+        self.t_edges16.copy_(torch.from_numpy(_pack16(cat([t.edges for t in self.topo]))))
         self.t_tet_rest_volume.copy_(torch.from_numpy(cat([t.tet_rest_volume for t in self.topo])))
         inv_rest = cat([t.tet_inv_rest for t in self.topo]).astype(np.float32)
         self.t_tet_inv_rest.copy_(torch.from_numpy(np.ascontiguousarray(inv_rest.T)))

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 3
+#define FFSRL_ABI_VERSION 4
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -66,6 +66,9 @@ typedef struct ffsrl_batch {
   const int32_t* edges;           /* int2 [E]: mesh-local (a, b) */
   const float* edge_rest;         /* [E] */
   const int32_t* tets;            /* int4 [T]: mesh-local (a, b, c, d) */
+  const uint64_t* tets16;         /* [T]: the same ids packed as 4 x 16 bits (a in the low bits);
+                                     read by the batched frame */
+  const uint32_t* edges16;        /* [E]: edge ids packed as 2 x 16 bits; read by the batched frame */
   const float* tet_rest_volume;   /* [T] */
   const float* tet_inv_rest;      /* [9][T] planes of the row-major inverse rest shape matrix */
   const int32_t* tris;            /* int4 [F]: mesh-local (a, b, c, original face id), leaf order */

@@ -33,7 +33,7 @@ def test_every_declared_symbol_is_exported(pkg):
 
 
 def test_abi_version(pkg):
-    assert pkg.lib.ffsrl_abi_version() == 3
+    assert pkg.lib.ffsrl_abi_version() == 4
     assert pkg.lib.ffsrl_frame_max_vertices() >= 10000
 
 
@@ -132,3 +132,15 @@ def test_header_has_no_hip_types():
     text = open(HEADER).read()
     assert "hipStream_t" not in re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     assert "#include <hip" not in text
+
+
+def test_pack16_layout(pkg):
+    """Packed constraint ids of the batched frame: 16-bit fields, first id in
+    the low bits (the layout ffsrl_batch.tets16 / edges16 declares)."""
+    t = np.array([[1, 2, 3, 4], [65535, 0, 10239, 7]], np.int32)
+    p = pkg.model._pack16(t).view(np.uint64)
+    for row, w in zip(t, p):
+        assert [(int(w) >> (16 * k)) & 0xffff for k in range(4)] == list(row)
+    e = np.array([[5, 9], [10239, 0]], np.int32)
+    q = pkg.model._pack16(e).view(np.uint32)
+    assert [(int(q[0]) & 0xffff, int(q[0]) >> 16), (int(q[1]) & 0xffff, int(q[1]) >> 16)] == [(5, 9), (10239, 0)]
