@@ -299,10 +299,14 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
   const int64_t T = b.n_tet_total;
 
   for (int i = tid; i < 2 * nv; i += BLOCK) c.acc[i] = 0ull;
-  // predict of substep 0
-  for (int i = tid; i < nv; i += BLOCK) {
+  // predict of substep 0; each thread notes which of its vertices (i = tid +
+  // k * BLOCK, k < 64 since nv <= 10,240) are grasped, so the apply phases
+  // read the grasp weight of those only
+  uint64_t grasped = 0;
+  for (int i = tid, k = 0; i < nv; i += BLOCK, ++k) {
     float4 x = ld4(X, vb + i), v = ld4(Vv, vb + i);
     st4(P, vb + i, with_w(predict(xyz(x), xyz(v), x.w, vs.g, vs.dt, vs.damp1), x.w));
+    if (b.drag[vb + i] != 0.0f) grasped |= 1ull << k;
   }
   __syncthreads();
 
@@ -324,7 +328,7 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
       // ---- apply (+ grasp; + ground, integrate and next predict) ----
       const bool last_it = (it == prm.iterations - 1);
       const bool next_sub = (s + 1 < prm.substeps);
-      for (int i = tid; i < nv; i += BLOCK) {
+      for (int i = tid, k = 0; i < nv; i += BLOCK, ++k) {
         const unsigned long long w0 = c.acc[2 * i], w1 = c.acc[2 * i + 1];
         c.acc[2 * i] = 0ull;
         c.acc[2 * i + 1] = 0ull;
@@ -337,8 +341,7 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
           const float s = c.q * __builtin_amdgcn_rcpf((float)n);
           p = p + mk((float)fx, (float)fy, (float)fz) * s;
         }
-        const float dr = b.drag[vb + i];
-        if (dr != 0.0f) p = grasp_pull<true>(p, tip, dr, vs.ks);
+        if ((grasped >> k) & 1ull) p = grasp_pull<true>(p, tip, b.drag[vb + i], vs.ks);
         if (!last_it) {
           st4(P, vb + i, with_w(p, p4.w));
           continue;
@@ -347,8 +350,11 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
         const float4 x4 = ld4(X, vb + i);
         const f3 v = (p - xyz(x4)) * vs.inv_dt;
         st4(X, vb + i, with_w(p, x4.w));
-        st4(Vv, vb + i, with_w(v, 0.0f));
-        if (next_sub) p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
+        // v feeds the next substep's predict here; only the frame's last is state
+        if (next_sub)
+          p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
+        else
+          st4(Vv, vb + i, with_w(v, 0.0f));
         st4(P, vb + i, with_w(p, p4.w));
       }
       __syncthreads();
