@@ -20,7 +20,7 @@ E_ARG, E_LIMIT, E_STATE = -1, -2, -3
 class EnvDesc(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "vbase", "nv", "ebase", "ne", "tbase", "nt", "fbase", "nf", "nodebase", "nn",
-        "boundbase", "lvlbase", "nlvl", "lam_ebase", "lam_tbase", "dp_exp", "vtopo_base", "reserved0",
+        "boundbase", "lvlbase", "nlvl", "lam_ebase", "lam_tbase", "dp_exp", "vtopo_base", "bvh_depth",
         "reserved1", "reserved2")]
 
 
@@ -36,7 +36,8 @@ class Batch(C.Structure):
         ("envs", _vp),
         ("edges", _vp), ("edge_rest", _vp), ("tets", _vp), ("tets16", _vp), ("edges16", _vp), ("tet_rest_volume", _vp),
         ("tet_inv_rest", _vp), ("tris", _vp), ("node_left", _vp), ("node_first", _vp),
-        ("node_count", _vp), ("level_nodes", _vp), ("level_off", _vp), ("vf_off", _vp), ("vf_idx", _vp),
+        ("node_count", _vp), ("level_nodes", _vp), ("level_info", _vp), ("level_off", _vp), ("vf_off", _vp),
+        ("vf_idx", _vp),
         ("x", _vp), ("p", _vp), ("v", _vp), ("drag", _vp), ("lam_e", _vp), ("lam_t", _vp),
         ("bvh_lo", _vp), ("bvh_hi", _vp), ("tips", _vp),
         ("dp", _vp), ("cnt", _vp), ("host_envs", _vp), ("host_level_off", _vp),

@@ -14,7 +14,7 @@ __global__ __launch_bounds__(BLOCK) void k_refit(ffsrl_batch b, int env_begin) {
 
 __global__ void k_refit_level(ffsrl_batch b, ffsrl_env_desc e, int beg, int end) {
   const int k = beg + blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < end) refit_node(b, e, b.level_nodes[(int64_t)e.nodebase + k]);
+  if (k < end) refit_node(b, e, k);
 }
 
 // ------------------------------------------------------------- host build
@@ -168,7 +168,7 @@ extern "C" {
 
 int ffsrl_bvh_refit(const ffsrl_batch* b, int32_t eb, int32_t ee, void* stream) {
   if (!b || !b->envs || !b->x || !b->tris || !b->node_left || !b->node_first || !b->node_count ||
-      !b->level_nodes || !b->level_off || !b->bvh_lo || !b->bvh_hi)
+      !b->level_nodes || !b->level_info || !b->level_off || !b->bvh_lo || !b->bvh_hi)
     return fail(FFSRL_E_ARG, "refit needs positions, triangles, BVH topology and bound arrays");
   if (eb < 0 || ee > b->n_env || eb > ee) return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", eb, ee, b->n_env);
   if (ee == eb) return 0;
@@ -178,7 +178,8 @@ int ffsrl_bvh_refit(const ffsrl_batch* b, int32_t eb, int32_t ee, void* stream) 
 }
 
 int ffsrl_bvh_refit_plain(const ffsrl_batch* b, int32_t eb, int32_t ee, void* stream) {
-  if (!b || !b->envs || !b->level_off) return fail(FFSRL_E_ARG, "refit needs the env table and BVH arrays");
+  if (!b || !b->envs || !b->level_off || !b->level_nodes || !b->level_info || !b->bvh_lo || !b->bvh_hi || !b->tris)
+    return fail(FFSRL_E_ARG, "refit needs the env table and BVH arrays");
   if (eb < 0 || ee > b->n_env || eb > ee) return fail(FFSRL_E_ARG, "env range [%d,%d) outside [0,%d)", eb, ee, b->n_env);
   hipStream_t st = as_stream(stream);
   // The per-level launches are issued from the host: use the host mirrors.

@@ -593,7 +593,7 @@ static int frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* e
   const bool carry = carries_lambda(*prm);
   if (carry && ((prm->dist_mode == FFSRL_XPBD && !b->lam_e) || (prm->vol_mode == FFSRL_XPBD && !b->lam_t)))
     return fail(FFSRL_E_ARG, "XPBD with iterations > 1 needs lambda arrays");
-  if (refit && (!b->tris || !b->node_left || !b->level_nodes || !b->level_off || !b->bvh_lo || !b->bvh_hi))
+  if (refit && (!b->tris || !b->level_nodes || !b->level_info || !b->level_off || !b->bvh_lo || !b->bvh_hi))
     return fail(FFSRL_E_ARG, "refit requested without BVH arrays");
   if ((prm->vol_mode || prm->nh_on) && !b->tets16) return fail(FFSRL_E_ARG, "batched frame needs packed tet ids");
   if (prm->dist_mode && !b->edges16) return fail(FFSRL_E_ARG, "batched frame needs packed edge ids");

@@ -242,7 +242,7 @@ static int render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t eb, con
   if (b->host_envs) {
     stack_depth = 2;
     const int lo = list ? 0 : eb, hi = list ? b->n_env : eb + n;
-    for (int i = lo; i < hi; ++i) stack_depth = std::max(stack_depth, (int)b->host_envs[i].nlvl + 1);
+    for (int i = lo; i < hi; ++i) stack_depth = std::max(stack_depth, (int)b->host_envs[i].bvh_depth + 1);
     if (stack_depth > kStack) return fail(FFSRL_E_LIMIT, "BVH with %d levels exceeds the %d-entry ray stack",
                                           stack_depth - 1, kStack);
   }

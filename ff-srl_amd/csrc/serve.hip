@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kServeBlock) void k_apply_requests(ffsrl_batch b, f
     }
     if (tid < 12) b.tips[env * 12 + tid] = r.tips0[env * 12 + tid];
     __syncthreads();
-    if (b.bvh_lo && b.level_nodes) refit_env<kServeBlock>(b, e);
+    if (b.bvh_lo && b.level_nodes && b.level_info) refit_env<kServeBlock>(b, e);
     return;
   }
   // step: move the four instrument points, then (optionally) reselect the grasp

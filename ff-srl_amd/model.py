@@ -96,14 +96,23 @@ class MeshTopology:
         # Unsplit root: the reference marks it with leftNode 0 (bvh.py:480); -1 is used here.
         # Leaf-order triangles: (a, b, c, original face id).
         self.tris_leaf = np.concatenate([mesh.tris[tri_map], tri_map[:, None]], axis=1).astype(np.int32)
-        order = []
-        self.level_off = [0]
+        # Refit groups: every leaf first (leaves depend only on positions), then
+        # the inner nodes level by level, deepest first.  level_info holds the
+        # node's (left child or first triangle, triangle count) per entry.
+        leaf = self.count > 0
+        order = [np.nonzero(leaf)[0]]
+        self.level_off = [0, order[0].size]
         for L in range(self.max_level, -1, -1):
-            ids = np.nonzero(self.level == L)[0]
-            order.append(ids)
-            self.level_off.append(self.level_off[-1] + ids.size)
+            ids = np.nonzero((self.level == L) & ~leaf)[0]
+            if ids.size:
+                order.append(ids)
+                self.level_off.append(self.level_off[-1] + ids.size)
         self.level_nodes = np.concatenate(order).astype(np.int32)
         self.level_off = np.asarray(self.level_off, dtype=np.int32)
+        self.num_groups = self.level_off.size - 1
+        nodes = self.level_nodes
+        self.level_info = np.stack([np.where(leaf[nodes], self.first[nodes], self.left[nodes]),
+                                    self.count[nodes]], axis=1).astype(np.int32)
         self.tri_colors_leaf = mesh.tri_colors[tri_map] if mesh.tri_colors is not None else None
         # vertex -> faces adjacency for the normals: faces of each vertex in
         # increasing original face id, stored as leaf positions
@@ -261,6 +270,7 @@ class SimModelDO:
         self.t_node_first = torch.from_numpy(cat([t.first for t in self.topo])).to(dev)
         self.t_node_count = torch.from_numpy(cat([t.count for t in self.topo])).to(dev)
         self.t_level_nodes = torch.from_numpy(cat([t.level_nodes for t in self.topo])).to(dev)
+        self.t_level_info = torch.from_numpy(cat([t.level_info for t in self.topo])).to(dev)
         self.t_vf_off = torch.from_numpy(cat([t.vf_off for t in self.topo])).to(dev)
         self.t_vf_idx = torch.from_numpy(cat([t.vf_idx for t in self.topo])).to(dev)
         self.level_off_host = np.ascontiguousarray(cat([t.level_off for t in self.topo]).astype(np.int32))
@@ -289,7 +299,8 @@ class SimModelDO:
             row[F["fbase"]], row[F["nf"]] = self.mesh_off["f"][k], m.num_tris
             row[F["nodebase"]], row[F["nn"]] = self.mesh_off["n"][k], t.num_nodes
             row[F["boundbase"]] = nb
-            row[F["lvlbase"]], row[F["nlvl"]] = self.mesh_off["l"][k], t.max_level + 1
+            row[F["lvlbase"]], row[F["nlvl"]] = self.mesh_off["l"][k], t.num_groups
+            row[F["bvh_depth"]] = t.max_level + 1
             row[F["lam_ebase"]], row[F["lam_tbase"]] = (le, lt) if carry else (0, 0)
             # fixed-point quantum of the batched frame's Jacobi sums: 2^-27 of the
             # mesh diagonal rounded down to a power of two (range 8..16 diagonals)
@@ -363,6 +374,7 @@ class SimModelDO:
         b.node_first = _ptr(self.t_node_first)
         b.node_count = _ptr(self.t_node_count)
         b.level_nodes = _ptr(self.t_level_nodes)
+        b.level_info = _ptr(self.t_level_info)
         b.level_off = _ptr(self.t_level_off)
         b.vf_off = _ptr(self.t_vf_off)
         b.vf_idx = _ptr(self.t_vf_idx)

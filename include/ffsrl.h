@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 4
+#define FFSRL_ABI_VERSION 5
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -47,13 +47,15 @@ typedef struct ffsrl_env_desc {
   int32_t fbase, nf;       /* render triangles of the mesh, BVH leaf order */
   int32_t nodebase, nn;    /* BVH topology nodes of the mesh */
   int32_t boundbase;       /* first BVH bound (state) of this env */
-  int32_t lvlbase, nlvl;   /* level_off[lvlbase .. lvlbase+nlvl] (levels deepest first) */
+  int32_t lvlbase, nlvl;   /* refit groups level_off[lvlbase .. lvlbase+nlvl]: all leaves, then
+                              inner nodes level by level, deepest first */
   int32_t lam_ebase;       /* first edge lambda of this env (XPBD, iterations > 1) */
   int32_t lam_tbase;       /* first tet lambda of this env */
   int32_t dp_exp;          /* quantum of the fixed-point Jacobi sums of the batched
                               frame: 2^dp_exp (about 2^-27 x bbox diagonal) */
   int32_t vtopo_base;      /* first vf_off entry of the env's mesh */
-  int32_t reserved0, reserved1, reserved2;
+  int32_t bvh_depth;       /* levels of the env's BVH (max node level + 1) */
+  int32_t reserved1, reserved2;
 } ffsrl_env_desc;
 
 /* Device arrays of a batch of environments. */
@@ -75,7 +77,9 @@ typedef struct ffsrl_batch {
   const int32_t* node_left;       /* [N] left child (mesh-local node id) or -1 for a leaf */
   const int32_t* node_first;      /* [N] first leaf-order triangle of a leaf */
   const int32_t* node_count;      /* [N] triangle count of a leaf, 0 for inner nodes */
-  const int32_t* level_nodes;     /* [N] node ids grouped by level */
+  const int32_t* level_nodes;     /* [N] node ids in refit-group order */
+  const int32_t* level_info;      /* int2 [N] per level_nodes entry: (left child or first triangle,
+                                     triangle count) */
   const int32_t* level_off;       /* per env lvlbase: offsets into level_nodes (relative to nodebase) */
   const int32_t* vf_off;          /* per mesh [nv+1] at vtopo_base: CSR offsets of each vertex's faces */
   const int32_t* vf_idx;          /* per mesh [3F] at 3*fbase: leaf positions of a vertex's faces,

@@ -186,3 +186,26 @@ def test_raycast_plane_analytic(orc):
     per_ray = np.where(hit >= 0, tr / 50.0, 0.0).reshape(RH, RW)
     box = per_ray.reshape(H, 2, W, 2).sum(axis=(1, 3)) / 4.0
     assert np.allclose(depth, box, rtol=1e-6)
+
+
+@pytest.mark.parametrize("which", ["cube", "ellipsoid"])
+def test_refit_groups(pkg, which):
+    """Refit groups: all leaves first, then inner nodes deepest level first;
+    every node once; an inner node's children are refit in earlier groups;
+    level_info carries (left or first, count)."""
+    sc = pkg.scenes
+    mesh = sc.cube_grid(7) if which == "cube" else sc.solid("ellipsoid", 3000, seed=8)
+    t = pkg.model.MeshTopology(mesh)
+    nodes, off = t.level_nodes, t.level_off
+    assert np.array_equal(np.sort(nodes), np.arange(t.num_nodes))
+    assert t.num_groups == off.size - 1 == t.max_level + 1
+    group = np.empty(t.num_nodes, np.int64)
+    for g in range(t.num_groups):
+        group[nodes[off[g]:off[g + 1]]] = g
+    leaf = t.count > 0
+    assert (group[leaf] == 0).all() and (group[~leaf] > 0).all()
+    inner = np.nonzero(~leaf)[0]
+    assert (group[t.left[inner]] < group[inner]).all() and (group[t.left[inner] + 1] < group[inner]).all()
+    info = t.level_info
+    assert np.array_equal(info[:, 1], t.count[nodes])
+    assert np.array_equal(info[:, 0], np.where(leaf[nodes], t.first[nodes], t.left[nodes]))
