@@ -39,7 +39,7 @@ class Batch(C.Structure):
         ("node_count", _vp), ("level_nodes", _vp), ("level_info", _vp), ("level_off", _vp), ("vf_off", _vp),
         ("vf_idx", _vp),
         ("x", _vp), ("p", _vp), ("v", _vp), ("drag", _vp), ("lam_e", _vp), ("lam_t", _vp),
-        ("bvh_lo", _vp), ("bvh_hi", _vp), ("tips", _vp),
+        ("bvh_lo", _vp), ("bvh_hi", _vp), ("tips", _vp), ("sched", _vp),
         ("dp", _vp), ("cnt", _vp), ("host_envs", _vp), ("host_level_off", _vp),
     ]
 

@@ -43,7 +43,7 @@ int check_hip(hipError_t e, const char* what) {
 // LDS budget of one workgroup on gfx950.
 constexpr int kLdsBytes = 160 * 1024;
 constexpr int lds_bytes_for(int nv) { return nv * 16; }
-constexpr int kMaxFrameVertices = kLdsBytes / 16;
+constexpr int kMaxFrameVertices = (kLdsBytes - 256) / 16;  // 256 B of static LDS for the scheduler
 
 // Jacobi sums in LDS are fixed point: each delta component is rounded to the
 // nearest multiple of the env's quantum q = 2^dp_exp (about 2^-27 of the mesh
@@ -273,11 +273,10 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
   }
 }
 
+// One frame of env `env` by the calling workgroup.
 template <int BLOCK, bool CARRY>
-__global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm, const int32_t* order,
-                                                 int refit) {
-  extern __shared__ unsigned long long lds[];
-  const int env = order ? order[blockIdx.x] : (int)blockIdx.x;
+__device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_params& prm, int env, int refit,
+                                          unsigned long long* lds) {
   const ffsrl_env_desc e = b.envs[env];
   const int nv = e.nv;
   FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
@@ -300,7 +299,7 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
 
   for (int i = tid; i < 2 * nv; i += BLOCK) c.acc[i] = 0ull;
   // predict of substep 0; each thread notes which of its vertices (i = tid +
-  // k * BLOCK, k < 64 since nv <= 10,240) are grasped, so the apply phases
+  // k * BLOCK, k < 64 since nv <= 10,224) are grasped, so the apply phases
   // read the grasp weight of those only
   uint64_t grasped = 0;
   for (int i = tid, k = 0; i < nv; i += BLOCK, ++k) {
@@ -363,13 +362,52 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
   if (refit) refit_env<BLOCK>(b, e);
 }
 
+// Envs are taken from a work counter (sched[0]) in `order`, so a workgroup
+// that finishes an env starts the next one at once: the launch has one
+// workgroup per CU and balances like greedy list scheduling over the
+// largest-first order, with no workgroup dispatch between envs.  The last
+// workgroup to finish (sched[1]) resets both counters for the next launch.
+// With sched == NULL each workgroup takes env order[blockIdx.x].
+template <int BLOCK, bool CARRY>
+__global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm, const int32_t* order, int n,
+                                                 int refit, uint32_t* sched) {
+  extern __shared__ unsigned long long lds[];
+  if (!sched) {
+    frame_env<BLOCK, CARRY>(b, prm, order ? order[blockIdx.x] : (int)blockIdx.x, refit, lds);
+    return;
+  }
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = (int)atomicAdd(&sched[0], 1u);
+    __syncthreads();
+    const int k = next;
+    __syncthreads();
+    if (k >= n) break;  // uniform: every thread read the same k
+    frame_env<BLOCK, CARRY>(b, prm, order ? order[k] : k, refit, lds);
+    __syncthreads();  // the env's LDS and phases are done before the next env
+  }
+  if (threadIdx.x == 0 && atomicAdd(&sched[1], 1u) == gridDim.x - 1) {
+    atomicExch(&sched[0], 0u);
+    atomicExch(&sched[1], 0u);
+  }
+}
+
 template <int BLOCK>
 static int launch_frame(const ffsrl_batch& b, const ffsrl_params& prm, const int32_t* order, int n, int refit,
-                        bool carry, int lds, hipStream_t st) {
+                        bool carry, int lds, uint32_t* sched, hipStream_t st) {
   auto kern = carry ? k_frame<BLOCK, true> : k_frame<BLOCK, false>;
   FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  hipLaunchKernelGGL(kern, dim3(n), dim3(BLOCK), lds, st, b, prm, order, refit);
+  int grid = n;
+  if (sched) {
+    // one resident workgroup per CU at most (the frame uses up to all LDS)
+    int dev = 0, cus = 0, per_cu = 0;
+    FFSRL_HIP(hipGetDevice(&dev));
+    FFSRL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    FFSRL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), BLOCK, lds));
+    grid = std::min(n, std::max(1, cus * std::max(1, per_cu)));
+  }
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), lds, st, b, prm, order, n, refit, sched);
   FFSRL_HIP(hipGetLastError());
   return 0;
 }
@@ -583,7 +621,7 @@ const char* ffsrl_last_error(void) { return g_err.c_str(); }
 int ffsrl_frame_max_vertices(void) { return kMaxFrameVertices; }
 
 static int frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
-                 int32_t refit, void* stream) {
+                 int32_t refit, uint32_t* sched, void* stream) {
   if (!prm) return fail(FFSRL_E_ARG, "params NULL");
   int rc = check_batch(b, prm);
   if (rc) return rc;
@@ -602,9 +640,9 @@ static int frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* e
   const int lds = lds_bytes_for(b->max_nv < 1 ? 1 : b->max_nv);
   hipStream_t st = as_stream(stream);
   switch (block) {
-    case 256: return launch_frame<256>(*b, *prm, envs, n, refit, carry, lds, st);
-    case 512: return launch_frame<512>(*b, *prm, envs, n, refit, carry, lds, st);
-    case 1024: return launch_frame<1024>(*b, *prm, envs, n, refit, carry, lds, st);
+    case 256: return launch_frame<256>(*b, *prm, envs, n, refit, carry, lds, sched, st);
+    case 512: return launch_frame<512>(*b, *prm, envs, n, refit, carry, lds, sched, st);
+    case 1024: return launch_frame<1024>(*b, *prm, envs, n, refit, carry, lds, sched, st);
     default: return fail(FFSRL_E_ARG, "block must be 256, 512 or 1024 (got %d)", block);
   }
 }
@@ -612,13 +650,13 @@ static int frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* e
 int ffsrl_frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* order, int32_t block,
                 int32_t refit, void* stream) {
   if (!b) return fail(FFSRL_E_ARG, "batch NULL");
-  return frame(b, prm, order, b->n_env, block, refit, stream);
+  return frame(b, prm, order, b->n_env, block, refit, b->sched, stream);
 }
 
 int ffsrl_frame_envs(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
                      int32_t refit, void* stream) {
   if (n > 0 && !envs) return fail(FFSRL_E_ARG, "env list NULL");
-  return frame(b, prm, envs, n, block, refit, stream);
+  return frame(b, prm, envs, n, block, refit, nullptr, stream);
 }
 
 int ffsrl_predict(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {

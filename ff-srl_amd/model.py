@@ -35,7 +35,7 @@ def _pack16(ids: np.ndarray) -> np.ndarray:
     """Rows of 4 (tets) or 2 (edges) mesh-local ids packed as 16-bit fields,
     first id in the low bits: int64 [n] for 4 ids, int32 [n] for 2.  Ids of
     meshes with more than 65,536 vertices are stored modulo 2^16 (such meshes
-    exceed the batched frame's 10,240-vertex limit and use the plain form)."""
+    exceed the batched frame's 10,224-vertex limit and use the plain form)."""
     ids = np.asarray(ids).astype(np.uint64) & np.uint64(0xffff)
     if ids.ndim != 2 or ids.shape[1] not in (2, 4):
         raise ValueError("expected (n, 2) or (n, 4) ids")
@@ -353,6 +353,7 @@ class SimModelDO:
         vc = np.concatenate([self.meshes[int(k)].colors for k in self.env_mesh]) if self.numEnvs else np.zeros((0, 3))
         self.vertexColors = torch.from_numpy(vc.astype(np.float32)).to(dev)
         self.normals = torch.zeros((V, 4), dtype=torch.float32, device=dev)
+        self.sched = torch.zeros(2, dtype=torch.int32, device=dev)      # frame work counters
         self.dp = None
         self.cnt = None
 
@@ -387,6 +388,7 @@ class SimModelDO:
         b.bvh_lo = _ptr(self.bvh_lo)
         b.bvh_hi = _ptr(self.bvh_hi)
         b.tips = _ptr(self.laparoscopeTip)
+        b.sched = _ptr(self.sched)
         b.dp = _ptr(self.dp)
         b.cnt = _ptr(self.cnt)
         b.host_envs = self.env_desc_host.ctypes.data

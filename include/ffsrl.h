@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 5
+#define FFSRL_ABI_VERSION 6
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -94,6 +94,8 @@ typedef struct ffsrl_batch {
   float* bvh_lo;                  /* float4 [sum N]: aabb min, .w = int bits (first tri or left child) */
   float* bvh_hi;                  /* float4 [sum N]: aabb max, .w = int bits (triangle count) */
   float* tips;                    /* [n_env*4*3] instrument points; grasp pulls to point 2 */
+  uint32_t* sched;                /* [2] work counters of ffsrl_frame (NULL: one workgroup per env);
+                                     zero before the first frame, left zero by every frame */
   /* plain-form scratch (per-kernel launches with global atomics) */
   float* dp;                      /* [3][V] planes (NULL unless the plain form is used) */
   int32_t* cnt;                   /* [V] */
