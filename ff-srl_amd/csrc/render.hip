@@ -39,14 +39,18 @@ __device__ __forceinline__ int isect_aabb(f3 o, f3 d, f3 lo, f3 hi, float* tmin_
 // one of the division form, so the visited nodes and hits are unchanged.
 __device__ __forceinline__ bool close_rel(float a, float b) { return fabsf(a - b) <= 1e-6f * (fabsf(a) + fabsf(b)); }
 
+//
+// On the fast path the six products are finite (their sum is: a NaN or an
+// infinity in any of them makes it non-finite), so hardware min/max give the
+// compare-select results of the reference's wp.min/wp.max up to the sign of
+// a zero, which no comparison below can see.
 __device__ __forceinline__ bool visit_node(f3 o, f3 d, f3 inv, f3 lo, f3 hi, float tMin) {
   const float t1 = (lo.x - o.x) * inv.x, t2 = (hi.x - o.x) * inv.x;
   const float t3 = (lo.y - o.y) * inv.y, t4 = (hi.y - o.y) * inv.y;
   const float t5 = (lo.z - o.z) * inv.z, t6 = (hi.z - o.z) * inv.z;
-  const float tmin = fmax_(fmax_(fmin_(t1, t2), fmin_(t3, t4)), fmin_(t5, t6));
-  const float tmax = fmin_(fmin_(fmax_(t1, t2), fmax_(t3, t4)), fmax_(t5, t6));
-  const bool finite = __builtin_isfinite(t1) && __builtin_isfinite(t2) && __builtin_isfinite(t3) &&
-                      __builtin_isfinite(t4) && __builtin_isfinite(t5) && __builtin_isfinite(t6);
+  const float tmin = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
+  const float tmax = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
+  const bool finite = __builtin_isfinite((((t1 + t2) + (t3 + t4)) + (t5 + t6)));
   const bool robust = finite && fabsf(tmax) > 1e-30f && !close_rel(tmax, tmin) && !close_rel(tmin, tMin);
   if (robust) return (tmax >= 0.0f) && (tmax >= tmin) && (tmin < tMin);
   float tb;
@@ -66,6 +70,11 @@ __device__ __forceinline__ float mt_isect(f3 a, f3 b, f3 c, f3 o, f3 d) {
   const float v = dot(d, qv) * inv;
   if (v < 0.0f || u + v > 1.0f) return -1.0f;
   return dot(e2, qv) * inv;
+}
+
+template <typename T>
+__device__ __forceinline__ T ldu4(const T* base, int i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint32_t)i * (uint32_t)sizeof(T));
 }
 
 struct RenderOut {
@@ -105,6 +114,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
   const f3 d = xyz(ld4(cam.dirs, r));
   const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
 
+  // env-uniform bases; node / triangle / vertex ids are non-negative int32,
+  // so loads take a 32-bit unsigned offset from a scalar base
+  const float4* blo = reinterpret_cast<const float4*>(b.bvh_lo) + sb;
+  const float4* bhi = reinterpret_cast<const float4*>(b.bvh_hi) + sb;
+  const int4* tris = reinterpret_cast<const int4*>(b.tris) + fb;
+  const float4* X = reinterpret_cast<const float4*>(b.x) + vb;
   int* stk = stack + wave * depth * 64 + lane;
   stk[0] = 0;
   int count = 1, best = -1;
@@ -112,14 +127,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
   while (count) {
     const int node = stk[(count - 1) * 64];
     count -= 1;
-    const float4 lo = ld4(b.bvh_lo, sb + node), hi = ld4(b.bvh_hi, sb + node);
+    const float4 lo = ldu4(blo, node), hi = ldu4(bhi, node);
     if (visit_node(o, d, inv, xyz(lo), xyz(hi), tMin)) {
       const int n = __float_as_int(hi.w);
       if (n > 0) {
         const int first = __float_as_int(lo.w);
         for (int k = 0; k < n; ++k) {
-          const int4 t = ldi4(b.tris, fb + first + k);
-          const float tt = mt_isect(xyz(ld4(b.x, vb + t.x)), xyz(ld4(b.x, vb + t.y)), xyz(ld4(b.x, vb + t.z)), o, d);
+          const int4 t = ldu4(tris, first + k);
+          const float tt = mt_isect(xyz(ldu4(X, t.x)), xyz(ldu4(X, t.y)), xyz(ldu4(X, t.z)), o, d);
           if (tt < tMin && tt > 0.0f) {
             tMin = tt;
             best = first + k;
