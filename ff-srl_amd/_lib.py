@@ -39,7 +39,8 @@ class Batch(C.Structure):
         ("node_count", _vp), ("level_nodes", _vp), ("level_info", _vp), ("level_off", _vp), ("vf_off", _vp),
         ("vf_idx", _vp),
         ("x", _vp), ("p", _vp), ("v", _vp), ("drag", _vp), ("lam_e", _vp), ("lam_t", _vp),
-        ("bvh_lo", _vp), ("bvh_hi", _vp), ("tips", _vp), ("sched", _vp),
+        ("bvh_lo", _vp), ("bvh_hi", _vp), ("tips", _vp), ("caps_base", _vp), ("caps_radius", _vp),
+        ("in_collision", _vp), ("sched", _vp),
         ("dp", _vp), ("cnt", _vp), ("host_envs", _vp), ("host_level_off", _vp),
     ]
 
@@ -51,7 +52,7 @@ class Params(C.Structure):
         ("dist_mode", C.c_int32), ("dist_k", C.c_float),
         ("vol_mode", C.c_int32), ("vol_k", C.c_float),
         ("nh_on", C.c_int32), ("nh_vol_c", C.c_float), ("nh_dev_c", C.c_float),
-        ("drag_ks", C.c_float), ("lambda_legacy", C.c_int32),
+        ("drag_ks", C.c_float), ("lambda_legacy", C.c_int32), ("collide", C.c_int32), ("contact_k", C.c_float),
     ]
 
 
@@ -65,7 +66,7 @@ class Camera(C.Structure):
 class Requests(C.Structure):
     _fields_ = [
         ("n", C.c_int32), ("envs", _vp), ("kind", _vp), ("action", _vp), ("x0", _vp), ("tips0", _vp),
-        ("lo", _vp), ("hi", _vp),
+        ("lo", _vp), ("hi", _vp), ("base0", _vp),
     ]
 
 
@@ -92,6 +93,7 @@ SIGNATURES = {
     "ffsrl_project_neohookean": [_P(Batch), _P(Params), _i32, _i32, _vp],
     "ffsrl_apply": [_P(Batch), _i32, _i32, _vp],
     "ffsrl_grasp": [_P(Batch), _P(Params), _i32, _i32, _vp],
+    "ffsrl_collide": [_P(Batch), _P(Params), _i32, _i32, _vp],
     "ffsrl_bounds_integrate": [_P(Batch), _P(Params), _i32, _i32, _vp],
     "ffsrl_zero_lambdas": [_P(Batch), _i32, _i32, _vp],
     "ffsrl_bvh_refit_plain": [_P(Batch), _i32, _i32, _vp],

@@ -245,6 +245,82 @@ __device__ __forceinline__ f3 predict(f3 x, f3 v, float w, f3 g, float dt, float
   return x + v * dt;
 }
 
+// closestPointOnEdge (integrator.py:1521-1534).
+__device__ __forceinline__ f3 closest_on_edge(f3 p, f3 ea, f3 eb) {
+  const f3 d = eb - ea;
+  float t;
+  if (length(d) < kFeps) {
+    t = 0.5f;
+  } else {
+    const float d2 = length(d) * length(d);
+    t = dot(d, p - ea) / d2;
+    if (t < 0.0f) t = 0.0f;
+    else if (t > 1.0f) t = 1.0f;
+  }
+  return ea + d * t;
+}
+
+// triangle_closest_point_barycentric (integrator.py:1536-1582).
+__device__ __forceinline__ f3 closest_bary(f3 a, f3 b, f3 c, f3 p) {
+  const f3 ab = b - a, ac = c - a, ap = p - a;
+  const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) return mk(1.0f, 0.0f, 0.0f);
+  const f3 bp = p - b;
+  const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) return mk(0.0f, 1.0f, 0.0f);
+  const float vc = d1 * d4 - d3 * d2;
+  float v = d1 / (d1 - d3);
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) return mk(1.0f - v, v, 0.0f);
+  const f3 cp = p - c;
+  const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) return mk(0.0f, 0.0f, 1.0f);
+  const float vb = d5 * d2 - d1 * d6;
+  float w = d2 / (d2 - d6);
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) return mk(1.0f - w, 0.0f, w);
+  const float va = d3 * d6 - d5 * d4;
+  w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) return mk(0.0f, w, 1.0f - w);
+  const float denom = 1.0f / (va + vb + vc);
+  v = vb * denom;
+  w = vc * denom;
+  return mk(1.0f - v - w, v, w);
+}
+
+// checkSdfCapsuleCollision (integrator.py:1303-1355): adds the responses of
+// the triangle's vertices to one capsule to r[0..2]; returns the sum of their
+// lengths (0: no contact).  As in the reference, the barycentric coordinates
+// of the line/plane intersection, not the point, are what closestPointOnEdge
+// receives.
+__device__ __forceinline__ float capsule_response(f3 tip, f3 base, float radius, f3 tri_n, f3 tri_c, f3 pA, f3 pB,
+                                                  f3 pC, float ks, f3 r[3]) {
+  const f3 cn = normalize(tip - base);
+  const f3 off = cn * radius;
+  const f3 posA = base + off, posB = tip - off;
+  const float denom = dot(tri_n, cn);
+  if (fabsf(denom) < kFeps) return 0.0f;
+  const float t = dot(tri_n, tri_c - base) / denom;
+  const f3 lpi = base + cn * t;
+  const f3 refp = closest_bary(pA, pB, pC, lpi);
+  const f3 sc = closest_on_edge(refp, posA, posB);
+  const f3 bary = closest_bary(pA, pB, pC, sc);
+  const f3 contact = (pA * bary.x + pB * bary.y) + pC * bary.z;
+  const f3 sv = sc - contact;
+  const float distance = length(sv);
+  const float pen = radius / length(sv) - 1.0f;
+  const f3 pv = mk(pen * sv.x, pen * sv.y, pen * sv.z);
+  const float plen = length(pv);
+  const f3 pn = normalize(pv);
+  const float s = bary.x * bary.x + bary.y * bary.y + bary.z * bary.z;
+  if (s < kFeps) return 0.0f;
+  const f3 resp = (pn * (plen / s)) * ks;
+  if (distance > radius) return 0.0f;
+  const f3 ra = resp * bary.x, rb = resp * bary.y, rc = resp * bary.z;
+  r[0] = r[0] + ra;
+  r[1] = r[1] + rb;
+  r[2] = r[2] + rc;
+  return (length(ra) + length(rb)) + length(rc);
+}
+
 // dragConstraintsDO (integrator.py:412-442): p -= |p-tip| * n * drag * ks / 2.
 template <bool FAST>
 __device__ __forceinline__ f3 grasp_pull(f3 p, f3 tip, float dr, float ks) {

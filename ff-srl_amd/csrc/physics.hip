@@ -273,8 +273,41 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
   }
 }
 
-// One frame of env `env` by the calling workgroup.
-template <int BLOCK, bool CARRY>
+// collideLaparoscopeDO (integrator.py:981-1050) for one env: every surface
+// triangle against capsules 0..2 of the env's instrument, responses read from
+// the positions at the start of the pass and summed (negated) into the LDS
+// accumulators for the vertices with inverse mass > 0; *contacts counts the
+// triangles with a response.
+__device__ __forceinline__ void collide_env(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
+                                            int env, const FrameCtx& c, const float* P, int64_t vb, int* contacts,
+                                            int block) {
+  for (int f = threadIdx.x; f < e.nf; f += block) {
+    const int4 t = ldi4(b.tris, (int64_t)e.fbase + f);
+    const float4 A = ld4(P, vb + t.x), B = ld4(P, vb + t.y), Cc = ld4(P, vb + t.z);
+    const f3 pA = xyz(A), pB = xyz(B), pC = xyz(Cc);
+    const f3 centre = ((pA + pB) + pC) * 0.3333f;
+    const f3 n = normalize(cross(pB - pA, pC - pA));
+    f3 r[3] = {mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f)};
+    float len = 0.0f;
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+      const int ci = env * 4 + i;
+      const f3 tip = mk(b.tips[3 * ci], b.tips[3 * ci + 1], b.tips[3 * ci + 2]);
+      const f3 base = mk(b.caps_base[3 * ci], b.caps_base[3 * ci + 1], b.caps_base[3 * ci + 2]);
+      len = len + capsule_response(tip, base, b.caps_radius[ci], n, centre, pA, pB, pC, prm.contact_k, r);
+    }
+    if (len > 0.0f) {
+      atomicAdd(contacts, 1);
+      if (A.w > 0.0f) lds_acc<-1>(c, t.x, r[0], 0u);
+      if (B.w > 0.0f) lds_acc<-1>(c, t.y, r[1], 0u);
+      if (Cc.w > 0.0f) lds_acc<-1>(c, t.z, r[2], 0u);
+    }
+  }
+}
+
+// One frame of env `env` by the calling workgroup (COLLIDE: with the
+// instrument collision pass).
+template <int BLOCK, bool CARRY, bool COLLIDE>
 __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_params& prm, int env, int refit,
                                           unsigned long long* lds) {
   const ffsrl_env_desc e = b.envs[env];
@@ -310,6 +343,9 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
   __syncthreads();
 
   const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
+  constexpr bool collide = COLLIDE;
+  __shared__ int contacts;  // surface triangles in contact during the frame
+  if (tid == 0) contacts = 0;
   const bool do_tets = prm.vol_mode != FFSRL_OFF || prm.nh_on;
   const bool do_edges = prm.dist_mode != FFSRL_OFF;
   for (int s = 0; s < prm.substeps; ++s) {
@@ -327,6 +363,20 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
       // ---- apply (+ grasp; + ground, integrate and next predict) ----
       const bool last_it = (it == prm.iterations - 1);
       const bool next_sub = (s + 1 < prm.substeps);
+      // ground clamp, integrate and the next substep's predict of vertex i
+      auto finish = [&](int i, f3 p, float w) {
+        if (p.y < vs.ground) p.y = vs.ground;
+        const float4 x4 = ld4(X, vb + i);
+        const f3 v = (p - xyz(x4)) * vs.inv_dt;
+        st4(X, vb + i, with_w(p, x4.w));
+        // v feeds the next substep's predict here; only the frame's last is state
+        if (next_sub)
+          p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
+        else
+          st4(Vv, vb + i, with_w(v, 0.0f));
+        st4(P, vb + i, with_w(p, w));
+      };
+      const bool fuse = last_it && !collide;  // with collision, the integrate follows it
       for (int i = tid, k = 0; i < nv; i += BLOCK, ++k) {
         const unsigned long long w0 = c.acc[2 * i], w1 = c.acc[2 * i + 1];
         c.acc[2 * i] = 0ull;
@@ -341,24 +391,37 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
           p = p + mk((float)fx, (float)fy, (float)fz) * s;
         }
         if ((grasped >> k) & 1ull) p = grasp_pull<true>(p, tip, b.drag[vb + i], vs.ks);
-        if (!last_it) {
+        if (!fuse) {
           st4(P, vb + i, with_w(p, p4.w));
           continue;
         }
-        if (p.y < vs.ground) p.y = vs.ground;
-        const float4 x4 = ld4(X, vb + i);
-        const f3 v = (p - xyz(x4)) * vs.inv_dt;
-        st4(X, vb + i, with_w(p, x4.w));
-        // v feeds the next substep's predict here; only the frame's last is state
-        if (next_sub)
-          p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
-        else
-          st4(Vv, vb + i, with_w(v, 0.0f));
-        st4(P, vb + i, with_w(p, p4.w));
+        finish(i, p, p4.w);
       }
       __syncthreads();
+      if (collide) {
+        // ---- instrument collision (collideLaparoscopeDO), summed in LDS ----
+        collide_env(b, prm, e, env, c, P, vb, &contacts, BLOCK);
+        __syncthreads();
+        for (int i = tid; i < nv; i += BLOCK) {
+          const unsigned long long w0 = c.acc[2 * i], w1 = c.acc[2 * i + 1];
+          c.acc[2 * i] = 0ull;
+          c.acc[2 * i + 1] = 0ull;
+          if (!last_it && (w0 | w1) == 0ull) continue;
+          int32_t fx, fy, fz, n;
+          unpack2(w0, &fx, &fy);
+          unpack2(w1, &fz, &n);
+          const float4 p4 = ld4(P, vb + i);
+          const f3 p = xyz(p4) + mk((float)fx, (float)fy, (float)fz) * c.q;
+          if (last_it)
+            finish(i, p, p4.w);
+          else
+            st4(P, vb + i, with_w(p, p4.w));
+        }
+        __syncthreads();
+      }
     }
   }
+  if (collide && tid == 0) b.in_collision[env] += contacts;  // counts accumulate until the caller clears them
   if (refit) refit_env<BLOCK>(b, e);
 }
 
@@ -368,12 +431,12 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
 // largest-first order, with no workgroup dispatch between envs.  The last
 // workgroup to finish (sched[1]) resets both counters for the next launch.
 // With sched == NULL each workgroup takes env order[blockIdx.x].
-template <int BLOCK, bool CARRY>
+template <int BLOCK, bool CARRY, bool COLLIDE>
 __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm, const int32_t* order, int n,
                                                  int refit, uint32_t* sched) {
   extern __shared__ unsigned long long lds[];
   if (!sched) {
-    frame_env<BLOCK, CARRY>(b, prm, order ? order[blockIdx.x] : (int)blockIdx.x, refit, lds);
+    frame_env<BLOCK, CARRY, COLLIDE>(b, prm, order ? order[blockIdx.x] : (int)blockIdx.x, refit, lds);
     return;
   }
   __shared__ int next;
@@ -383,7 +446,7 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
     const int k = next;
     __syncthreads();
     if (k >= n) break;  // uniform: every thread read the same k
-    frame_env<BLOCK, CARRY>(b, prm, order ? order[k] : k, refit, lds);
+    frame_env<BLOCK, CARRY, COLLIDE>(b, prm, order ? order[k] : k, refit, lds);
     __syncthreads();  // the env's LDS and phases are done before the next env
   }
   if (threadIdx.x == 0 && atomicAdd(&sched[1], 1u) == gridDim.x - 1) {
@@ -395,7 +458,8 @@ __global__ __launch_bounds__(BLOCK) void k_frame(ffsrl_batch b, ffsrl_params prm
 template <int BLOCK>
 static int launch_frame(const ffsrl_batch& b, const ffsrl_params& prm, const int32_t* order, int n, int refit,
                         bool carry, int lds, uint32_t* sched, hipStream_t st) {
-  auto kern = carry ? k_frame<BLOCK, true> : k_frame<BLOCK, false>;
+  auto kern = prm.collide ? (carry ? k_frame<BLOCK, true, true> : k_frame<BLOCK, false, true>)
+                          : (carry ? k_frame<BLOCK, true, false> : k_frame<BLOCK, false, false>);
   FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   int grid = n;
@@ -542,6 +606,45 @@ __global__ void k_grasp(ffsrl_batch b, ffsrl_params prm, int env, int64_t vb, in
   st4(b.p, vb + i, with_w(grasp_pull<false>(xyz(p), mk(t[0], t[1], t[2]), dr, prm.drag_ks), p.w));
 }
 
+// collideLaparoscopeDO (integrator.py:981-1050), plain form: one thread per
+// surface triangle of env `env`; responses summed into dp (no count), then
+// k_collide_apply subtracts them, so every triangle reads the positions of
+// the start of the pass.
+__global__ void k_collide(ffsrl_batch b, ffsrl_params prm, int env, ffsrl_env_desc e, int64_t V) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= e.nf) return;
+  const int64_t vb = e.vbase;
+  const int4 t = ldi4(b.tris, (int64_t)e.fbase + f);
+  const float4 A = ld4(b.p, vb + t.x), B = ld4(b.p, vb + t.y), Cc = ld4(b.p, vb + t.z);
+  const f3 pA = xyz(A), pB = xyz(B), pC = xyz(Cc);
+  const f3 centre = ((pA + pB) + pC) * 0.3333f;
+  const f3 n = normalize(cross(pB - pA, pC - pA));
+  f3 r[3] = {mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f)};
+  float len = 0.0f;
+  for (int i = 0; i < 3; ++i) {
+    const int ci = env * 4 + i;
+    const f3 tip = mk(b.tips[3 * ci], b.tips[3 * ci + 1], b.tips[3 * ci + 2]);
+    const f3 base = mk(b.caps_base[3 * ci], b.caps_base[3 * ci + 1], b.caps_base[3 * ci + 2]);
+    len = len + capsule_response(tip, base, b.caps_radius[ci], n, centre, pA, pB, pC, prm.contact_k, r);
+  }
+  if (!(len > 0.0f)) return;
+  atomicAdd(&b.in_collision[env], 1);
+  if (A.w > 0.0f) g_add(b, V, vb + t.x, r[0]);
+  if (B.w > 0.0f) g_add(b, V, vb + t.y, r[1]);
+  if (Cc.w > 0.0f) g_add(b, V, vb + t.z, r[2]);
+}
+
+__global__ void k_collide_apply(ffsrl_batch b, int64_t vb, int nv, int64_t V) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int64_t g = vb + i;
+  const float4 p = ld4(b.p, g);
+  st4(b.p, g, with_w(xyz(p) - mk(b.dp[g], b.dp[V + g], b.dp[2 * V + g]), p.w));
+  b.dp[g] = 0.0f;
+  b.dp[V + g] = 0.0f;
+  b.dp[2 * V + g] = 0.0f;
+}
+
 __global__ void k_bounds_integrate(ffsrl_batch b, ffsrl_params prm, int64_t vb, int nv) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
@@ -564,10 +667,13 @@ __global__ void k_move_tools(ffsrl_batch b, const float* delta, const float* lo,
   const int env = i >> 2;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    float t = b.tips[i * 3 + k] + delta[env * 3 + k];
+    const float t0 = b.tips[i * 3 + k];
+    float t = t0 + delta[env * 3 + k];
     if (lo) t = fmax_(t, lo[k]);
     if (hi) t = fmin_(t, hi[k]);
     b.tips[i * 3 + k] = t;
+    // capsule bases follow their tips (the instrument is translated rigidly)
+    if (b.caps_base) b.caps_base[i * 3 + k] += t - t0;
   }
 }
 
@@ -635,6 +741,8 @@ static int frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* e
     return fail(FFSRL_E_ARG, "refit requested without BVH arrays");
   if ((prm->vol_mode || prm->nh_on) && !b->tets16) return fail(FFSRL_E_ARG, "batched frame needs packed tet ids");
   if (prm->dist_mode && !b->edges16) return fail(FFSRL_E_ARG, "batched frame needs packed edge ids");
+  if (prm->collide && (!b->tris || !b->caps_base || !b->caps_radius || !b->in_collision))
+    return fail(FFSRL_E_ARG, "collision needs triangles, capsule bases / radii and the in_collision counters");
   if (n < 0 || n > b->n_env) return fail(FFSRL_E_ARG, "%d environments listed, batch has %d", n, b->n_env);
   if (n == 0) return 0;
   const int lds = lds_bytes_for(b->max_nv < 1 ? 1 : b->max_nv);
@@ -751,6 +859,26 @@ int ffsrl_grasp(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nv) continue;
     hipLaunchKernelGGL(k_grasp, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, *prm, i, (int64_t)h[i].vbase, h[i].nv);
+  }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+int ffsrl_collide(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {
+  int rc = check_batch(b, prm);
+  if (rc) return rc;
+  if (!b->dp) return fail(FFSRL_E_ARG, "plain form needs dp/cnt scratch");
+  if (!b->tris || !b->caps_base || !b->caps_radius || !b->in_collision)
+    return fail(FFSRL_E_ARG, "collision needs triangles, capsule bases / radii and the in_collision counters");
+  const ffsrl_env_desc* h;
+  int64_t V;
+  hipStream_t st = as_stream(stream);
+  if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
+  for (int i = eb; i < ee; ++i) {
+    if (!h[i].nv || !h[i].nf) continue;
+    hipLaunchKernelGGL(k_collide, dim3(nblk(h[i].nf, 256)), dim3(256), 0, st, *b, *prm, i, h[i], V);
+    hipLaunchKernelGGL(k_collide_apply, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, (int64_t)h[i].vbase, h[i].nv,
+                       V);
   }
   FFSRL_HIP(hipGetLastError());
   return 0;

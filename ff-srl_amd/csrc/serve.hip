@@ -35,6 +35,7 @@ __global__ __launch_bounds__(kServeBlock) void k_apply_requests(ffsrl_batch b, f
       b.drag[vb + v] = 0.0f;
     }
     if (tid < 12) b.tips[env * 12 + tid] = r.tips0[env * 12 + tid];
+    if (tid < 12 && r.base0 && b.caps_base) b.caps_base[env * 12 + tid] = r.base0[env * 12 + tid];
     __syncthreads();
     if (b.bvh_lo && b.level_nodes && b.level_info) refit_env<kServeBlock>(b, e);
     return;
@@ -43,10 +44,13 @@ __global__ __launch_bounds__(kServeBlock) void k_apply_requests(ffsrl_batch b, f
   const float* a = r.action + 4 * (int64_t)i;
   if (tid < 12) {
     const int k = tid % 3;
-    float t = b.tips[env * 12 + tid] + a[k];
+    const float t0 = b.tips[env * 12 + tid];
+    float t = t0 + a[k];
     if (r.lo) t = fmax_(t, r.lo[k]);
     if (r.hi) t = fmin_(t, r.hi[k]);
     b.tips[env * 12 + tid] = t;
+    // capsule bases follow their tips (the instrument is translated rigidly)
+    if (b.caps_base) b.caps_base[env * 12 + tid] += t - t0;
     if (tid >= 9) tip3[k] = t;
   }
   const float clamp = a[3];

@@ -2,8 +2,8 @@
 
 ``stepModel`` mirrors ``SimIntegratorDO.stepModel`` (``FF_SRL/FF_SRL/
 integrator.py:1640-1744``): S substeps of predict, I iterations of the
-constraint projection with Jacobi averaging and the grasp pull, the ground
-clamp and the integrate.  Two launch forms run the same arithmetic:
+constraint projection with Jacobi averaging, the grasp pull and (with
+``collision``) the instrument collision, the ground clamp and the integrate.  Two launch forms run the same arithmetic:
 
 * batched (default): ``ffsrl_frame``, one launch for all environments and
   the whole substep loop (and optionally the BVH refit);
@@ -61,4 +61,6 @@ class SimIntegratorDO:
                 lib.ffsrl_project_neohookean(b, prm, eb, ee, s)
                 lib.ffsrl_apply(b, eb, ee, s)
                 lib.ffsrl_grasp(b, prm, eb, ee, s)
+                if m.collision:
+                    lib.ffsrl_collide(b, prm, eb, ee, s)
             lib.ffsrl_bounds_integrate(b, prm, eb, ee, s)

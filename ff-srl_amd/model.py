@@ -183,7 +183,8 @@ class SimModelDO:
                  volumeComplianceXPBD=0.0, globalVolumeCompliance=1.0, globalDeviatoricCompliance=1.0,
                  globalLaparoscopeDragLookupRadius=0.15, lambdaLegacy=False, bins=8,
                  initialVelocity: Optional[np.ndarray] = None, tips: Optional[np.ndarray] = None,
-                 constraint_order: str = "local"):
+                 constraint_order: str = "local", globalKsContact=1.0, collision: bool = False,
+                 capsuleBases: Optional[np.ndarray] = None, capsuleRadii: Optional[np.ndarray] = None):
         self.device = torch.device(device)
         self.meshes = list(meshes)
         env_mesh = list(range(len(self.meshes))) if env_mesh is None else list(env_mesh)
@@ -208,13 +209,15 @@ class SimModelDO:
         self.globalVolumeComplianceNeoHookean = globalVolumeCompliance
         self.gloabalDeviatoricComplianceNeoHookean = globalDeviatoricCompliance
         self.lambda_legacy = bool(lambdaLegacy)
+        self.globalKsContact = globalKsContact
+        self.collision = bool(collision)
         self.params = self._make_params()
 
         self.constraint_order = constraint_order
         self.topo = _parallel(lambda m: MeshTopology(m, bins, constraint_order), self.meshes)
         self._build_topology()
         self._build_envs()
-        self._build_state(initialVelocity, tips)
+        self._build_state(initialVelocity, tips, capsuleBases, capsuleRadii)
         self._batch = self._make_batch()
         self._order = None
 
@@ -236,6 +239,8 @@ class SimModelDO:
         p.nh_dev_c = self.gloabalDeviatoricComplianceNeoHookean
         p.drag_ks = self.globalKsDrag
         p.lambda_legacy = int(self.lambda_legacy)
+        p.collide = int(self.collision)
+        p.contact_k = self.globalKsContact
         return p
 
     @property
@@ -325,7 +330,7 @@ class SimModelDO:
         work = np.array([self.meshes[k].num_tets * 4 + self.meshes[k].num_edges for k in self.env_mesh])
         self.work = work
 
-    def _build_state(self, initial_velocity, tips):
+    def _build_state(self, initial_velocity, tips, bases, radii):
         dev = self.device
         V = self.numVertices
         x0 = np.zeros((V, 4), dtype=np.float32)
@@ -350,6 +355,14 @@ class SimModelDO:
         t0 = np.zeros((self.numEnvs, 4, 3), dtype=np.float32) if tips is None else np.asarray(tips, np.float32)
         self.initialTips = torch.from_numpy(t0.reshape(self.numEnvs, 4, 3).copy()).to(dev)
         self.laparoscopeTip = self.initialTips.clone()
+        # instrument capsules (laparoscope.py:375-386): segment base -> tip of
+        # points 0..2 with a radius; point 3 is the drag point (radius 0)
+        c0 = self.initialTips.cpu().numpy() if bases is None else np.asarray(bases, np.float32)
+        self.initialBases = torch.from_numpy(c0.reshape(self.numEnvs, 4, 3).copy()).to(dev)
+        self.laparoscopeBase = self.initialBases.clone()
+        r0 = np.zeros((self.numEnvs, 4), np.float32) if radii is None else np.asarray(radii, np.float32)
+        self.laparoscopeRadius = torch.from_numpy(r0.reshape(self.numEnvs, 4).copy()).to(dev)
+        self.laparoscopeInCollision = torch.zeros(max(self.numEnvs, 1), dtype=torch.int32, device=dev)
         vc = np.concatenate([self.meshes[int(k)].colors for k in self.env_mesh]) if self.numEnvs else np.zeros((0, 3))
         self.vertexColors = torch.from_numpy(vc.astype(np.float32)).to(dev)
         self.normals = torch.zeros((V, 4), dtype=torch.float32, device=dev)
@@ -388,6 +401,9 @@ class SimModelDO:
         b.bvh_lo = _ptr(self.bvh_lo)
         b.bvh_hi = _ptr(self.bvh_hi)
         b.tips = _ptr(self.laparoscopeTip)
+        b.caps_base = _ptr(self.laparoscopeBase)
+        b.caps_radius = _ptr(self.laparoscopeRadius)
+        b.in_collision = _ptr(self.laparoscopeInCollision)
         b.sched = _ptr(self.sched)
         b.dp = _ptr(self.dp)
         b.cnt = _ptr(self.cnt)
@@ -479,6 +495,15 @@ class SimModelDO:
     def getLaparoscopeClampVertex(self, vertexId):
         return self.drag[vertexId]
 
+    def getLaparoscopeInCollision(self):
+        """``modelDO.py:1513-1517``: per env, the surface triangles found in
+        contact, summed over every collision pass since the last reset."""
+        return self.laparoscopeInCollision[:self.numEnvs]
+
+    def resetCollisionInfo(self):
+        """``modelDO.py:1519-1521`` (cleared in place: the batch keeps its pointer)."""
+        self.laparoscopeInCollision.zero_()
+
     # ------------------------------------------------------------------ reset
     def resetFixedModel(self, envNums):
         """``modelDO.py:1589-1622``: restore positions, zero predicted positions,
@@ -491,6 +516,7 @@ class SimModelDO:
             self.v[s] = 0.0
             self.drag[s] = 0.0
             self.laparoscopeTip[int(i)] = self.initialTips[int(i)]
+            self.laparoscopeBase[int(i)] = self.initialBases[int(i)]
 
     def set_grasp(self, env: int, local_ids, weight: float = 1.0):
         """Force grasp weights on env-local vertices (``forceClampsDragKernel``,

@@ -120,6 +120,7 @@ class SceneServer:
         self.obs = torch.zeros((self.n_slots, N_SCALARS), dtype=torch.float32, device=self.device)
         self.x0 = m.initialVertex
         self.tips0 = m.initialTips.reshape(-1).contiguous()
+        self.base0 = m.initialBases.reshape(-1).contiguous()
         self.block = int(block)
         self.radius = float(grasp_radius)
         self.max_batch = int(max_batch)
@@ -281,7 +282,7 @@ class SceneServer:
             s = stream.cuda_stream
             r = Requests()
             r.n, r.envs, r.kind, r.action = n, p_env, p_kind, p_act
-            r.x0, r.tips0 = self.x0.data_ptr(), self.tips0.data_ptr()
+            r.x0, r.tips0, r.base0 = self.x0.data_ptr(), self.tips0.data_ptr(), self.base0.data_ptr()
             r.lo = None if self._lo is None else self._lo.data_ptr()
             r.hi = None if self._hi is None else self._hi.data_ptr()
             lib.ffsrl_apply_requests(model.batch, r, self.radius, s)

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 6
+#define FFSRL_ABI_VERSION 7
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -94,6 +94,9 @@ typedef struct ffsrl_batch {
   float* bvh_lo;                  /* float4 [sum N]: aabb min, .w = int bits (first tri or left child) */
   float* bvh_hi;                  /* float4 [sum N]: aabb max, .w = int bits (triangle count) */
   float* tips;                    /* [n_env*4*3] instrument points; grasp pulls to point 2 */
+  float* caps_base;               /* [n_env*4*3] instrument capsule bases (tips are `tips`), or NULL */
+  const float* caps_radius;       /* [n_env*4] instrument capsule radii */
+  int32_t* in_collision;          /* [n_env] surface-triangle contacts, summed over passes until cleared */
   uint32_t* sched;                /* [2] work counters of ffsrl_frame (NULL: one workgroup per env);
                                      zero before the first frame, left zero by every frame */
   /* plain-form scratch (per-kernel launches with global atomics) */
@@ -120,6 +123,8 @@ typedef struct ffsrl_params {
   float nh_vol_c, nh_dev_c;       /* hydrostatic / deviatoric compliance (1/lambda, 1/mu) */
   float drag_ks;                  /* globalKsDrag */
   int32_t lambda_legacy;          /* 1: zero lambdas only when iterations > 1 */
+  int32_t collide;                /* 1: instrument collision after the grasp (collideLaparoscopeDO) */
+  float contact_k;                /* globalKsContact */
 } ffsrl_params;
 
 /* Camera of the ray-cast renderer: one origin and a precomputed direction per
@@ -145,6 +150,7 @@ typedef struct ffsrl_requests {
   const float* tips0;             /* float [n_env*4*3]: initial instrument points */
   const float* lo;                /* float[3] workspace lower corner, or NULL */
   const float* hi;                /* float[3] workspace upper corner, or NULL */
+  const float* base0;             /* float [n_env*4*3]: initial capsule bases (reset), or NULL */
 } ffsrl_requests;
 
 int ffsrl_abi_version(void);
@@ -202,7 +208,8 @@ int ffsrl_vertex_normals(const ffsrl_batch* b, int32_t env_begin, int32_t env_en
                          void* stream);
 
 /* Per-env tool update: adds delta[env] (3 floats) to the 4 instrument points
- * of each env and clamps them to [lo, hi].  Stands in for
+ * of each env and clamps them to [lo, hi]; capsule bases, when present, move
+ * by the same clamped displacement as their tips.  Stands in for
  * applyCartesianActionsInWorkspace (modelDO.py:1385-1404) with the tool tip
  * driven directly (no remote-centre-of-motion kinematics). */
 int ffsrl_move_tools(const ffsrl_batch* b, const float* delta, const float* lo, const float* hi,
@@ -225,6 +232,8 @@ int ffsrl_grasp_select(const ffsrl_batch* b, float radius, const int32_t* state,
  * (modelDO.py:247-279): drag = |x - tip[env*4+3]| < radius when closed, 0 when
  * open. */
 int ffsrl_apply_requests(const ffsrl_batch* b, const ffsrl_requests* r, float radius, void* stream);
+/* (With capsule bases in the batch, a step moves each base by its tip's
+ * clamped displacement and a reset restores base0 when given.) */
 
 /* 32 observation scalars per listed env, written to out[env*32 + k]:
  *   0-11 the 4 instrument points, 12-14 centroid, 15-17 / 18-20 bounding box
@@ -246,6 +255,11 @@ int ffsrl_project_distance(const ffsrl_batch* b, const ffsrl_params* prm, int32_
 int ffsrl_project_neohookean(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end, void* stream);
 int ffsrl_apply(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, void* stream);
 int ffsrl_grasp(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end, void* stream);
+/* collideLaparoscopeDO (integrator.py:981-1050, launched at :1713-1726):
+ * every surface triangle against instrument capsules 0..2 of its env, the
+ * responses subtracted from p for vertices with inverse mass > 0 after all
+ * triangles have read p; in_collision[env] += triangles with a response. */
+int ffsrl_collide(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end, void* stream);
 int ffsrl_bounds_integrate(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end, void* stream);
 int ffsrl_zero_lambdas(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, void* stream);
 /* One launch per env and BVH level, deepest level first (refitBVHLoop). */

@@ -51,6 +51,7 @@ def _cfg_struct(real):
             ("dt", real), ("gravity", real * 3), ("damping", real), ("ground", real),
             ("dist_mode", C.c_int), ("vol_mode", C.c_int), ("nh_on", C.c_int), ("lambda_legacy", C.c_int),
             ("dist_k", real), ("vol_k", real), ("nh_vol_c", real), ("nh_dev_c", real), ("drag_ks", real),
+            ("collide", C.c_int), ("nF", C.c_long), ("contact_k", real),
         ]
     return Cfg
 
@@ -79,7 +80,8 @@ class StepConfig:
 
     def __init__(self, substeps=10, iterations=1, fps=60.0, gravity=(0.0, -9.81, 0.0), damping=0.1, ground=0.0,
                  dist_mode=0, dist_k=0.0, vol_mode=0, vol_k=0.0, nh_on=0, nh_vol_c=1.0, nh_dev_c=1.0,
-                 drag_ks=1.0, lambda_legacy=0, dist_compliance=None, vol_compliance=None):
+                 drag_ks=1.0, lambda_legacy=0, dist_compliance=None, vol_compliance=None, collide=0,
+                 contact_k=1.0):
         self.substeps, self.iterations = substeps, iterations
         self.dt = (1.0 / fps) / substeps
         self.gravity, self.damping, self.ground = gravity, damping, ground
@@ -89,8 +91,9 @@ class StepConfig:
         self.vol_k = vol_k if vol_compliance is None else vol_compliance / (self.dt * self.dt)
         self.nh_vol_c, self.nh_dev_c = nh_vol_c, nh_dev_c
         self.drag_ks, self.lambda_legacy = drag_ks, lambda_legacy
+        self.collide, self.contact_k = collide, contact_k
 
-    def cstruct(self, dtype, nV, nE, nT):
+    def cstruct(self, dtype, nV, nE, nT, nF=0):
         S = CfgF32 if dtype == np.float32 else CfgF64
         c = S()
         c.nV, c.nE, c.nT = nV, nE, nT
@@ -103,16 +106,18 @@ class StepConfig:
         c.dist_k, c.vol_k = self.dist_k, self.vol_k
         c.nh_vol_c, c.nh_dev_c = self.nh_vol_c, self.nh_dev_c
         c.drag_ks = self.drag_ks
+        c.collide, c.nF, c.contact_k = int(self.collide), nF, self.contact_k
         return c
 
 
 class OracleSim:
     """Concatenated multi-env state in the reference layout."""
 
-    def __init__(self, meshes, env_mesh, dtype=np.float32, initial_velocity=None, tips=None):
+    def __init__(self, meshes, env_mesh, dtype=np.float32, initial_velocity=None, tips=None, caps_base=None,
+                 caps_radius=None):
         self.dtype = dtype
         vb = 0
-        edges, rest, tets, vol, inv, x, w, v2e = [], [], [], [], [], [], [], []
+        edges, rest, tets, vol, inv, x, w, v2e, tri, t2e = [], [], [], [], [], [], [], [], [], []
         self.vbase = [0]
         for i, k in enumerate(env_mesh):
             m = meshes[k]
@@ -124,6 +129,8 @@ class OracleSim:
             x.append(m.verts)
             w.append(m.inv_mass)
             v2e.append(np.full(m.num_vertices, i, np.int32))
+            tri.append(m.tris.astype(np.int32) + vb)
+            t2e.append(np.full(m.num_tris, i, np.int32))
             vb += m.num_vertices
             self.vbase.append(vb)
         cat = np.concatenate
@@ -144,19 +151,28 @@ class OracleSim:
         self.lam_t = np.zeros(max(self.nT, 1), dtype)
         self.drag = np.zeros(self.nV, dtype)
         self.v2e = cat(v2e).astype(np.int32)
+        self.tri = np.ascontiguousarray(cat(tri).reshape(-1)).astype(np.int32)
+        self.t2e = cat(t2e).astype(np.int32)
+        self.nF = len(self.t2e)
         n_env = len(env_mesh)
         self.tips = np.zeros((n_env * 4, 3), dtype) if tips is None else np.ascontiguousarray(
             np.asarray(tips).reshape(-1, 3)).astype(dtype)
+        self.caps_base = np.zeros((n_env * 4, 3), dtype) if caps_base is None else np.ascontiguousarray(
+            np.asarray(caps_base).reshape(-1, 3)).astype(dtype)
+        self.caps_radius = np.zeros(n_env * 4, dtype) if caps_radius is None else np.ascontiguousarray(
+            np.asarray(caps_radius).reshape(-1)).astype(dtype)
+        self.in_collision = np.zeros(n_env, np.int32)
 
     def step(self, cfg: StepConfig, frames: int = 1, use_drag: bool = True):
         L = lib()
         sfx = "_f32" if self.dtype == np.float32 else "_f64"
         fn = getattr(L, "oracle_step_frame" + sfx)
-        c = cfg.cstruct(self.dtype, self.nV, self.nE, self.nT)
+        c = cfg.cstruct(self.dtype, self.nV, self.nE, self.nT, self.nF)
         for _ in range(frames):
             fn(C.byref(c), _p(self.x), _p(self.p), _p(self.v), _p(self.inv_mass), _p(self.dP), _p(self.cnt),
                _p(self.edge), _p(self.edge_rest), _p(self.tet), _p(self.rest_vol), _p(self.inv_rest),
-               _p(self.lam_e), _p(self.lam_t), _p(self.drag) if use_drag else None, _p(self.v2e), _p(self.tips))
+               _p(self.lam_e), _p(self.lam_t), _p(self.drag) if use_drag else None, _p(self.v2e), _p(self.tips),
+               _p(self.tri), _p(self.t2e), _p(self.caps_base), _p(self.caps_radius), _p(self.in_collision))
 
 
 def call(name, dtype, *args):
@@ -332,3 +348,21 @@ def bvh_build(vertex, faces, bins=8):
     return {"left": left[:n].astype(np.int32), "first": first[:n].astype(np.int32),
             "count": count[:n].astype(np.int32), "level": level[:n].astype(np.int32),
             "tri_map": tri.astype(np.int32), "max_level": int(level[:n].max())}
+
+
+def collide(pred, tri, tri_env, inv_mass, caps_base, caps_tip, caps_radius, k=1.0, dtype=np.float64):
+    """Restated collideLaparoscopeDO (integrator.py:981-1050) for one pass.
+    Returns the updated positions and the per-env triangle-in-collision count."""
+    pred = np.ascontiguousarray(pred, dtype).copy()
+    tri = np.ascontiguousarray(np.asarray(tri).reshape(-1), np.int32)
+    tri_env = np.ascontiguousarray(tri_env, np.int32)
+    n_env = int(np.asarray(caps_radius).size // 4)
+    inc = np.zeros(n_env, np.int32)
+    acc = np.zeros_like(pred)
+    f = lambda a: np.ascontiguousarray(a, dtype)
+    real = C.c_float if dtype == np.float32 else C.c_double
+    sfx = "_f32" if dtype == np.float32 else "_f64"
+    getattr(lib(), "oracle_collide" + sfx)(C.c_long(tri.size // 3), _p(pred), _p(tri), _p(tri_env), _p(f(inv_mass)),
+                                           _p(f(caps_base)), _p(f(caps_tip)), _p(f(caps_radius)), _p(inc), real(k),
+                                           C.c_long(pred.shape[0]), _p(acc))
+    return pred, inc

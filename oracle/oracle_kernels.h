@@ -333,6 +333,120 @@ void FN(oracle_grasp_select)(long n, const REAL* pos, const int* vertexToEnv, co
   }
 }
 
+/* closestPointOnEdge (integrator.py:1521-1534) */
+static inline V3 FN(closest_on_edge)(V3 p, V3 ea, V3 eb) {
+  V3 d = FN(sub)(eb, ea);
+  REAL t;
+  if (FN(length)(d) < FEPS) {
+    t = R(0.5);
+  } else {
+    REAL d2 = FN(length)(d) * FN(length)(d);
+    t = FN(dot)(d, FN(sub)(p, ea)) / d2;
+    if (t < R(0)) t = R(0);
+    else if (t > R(1)) t = R(1);
+  }
+  return FN(add)(ea, FN(mul)(d, t));
+}
+
+/* triangle_closest_point_barycentric (integrator.py:1536-1582) */
+static inline V3 FN(closest_bary)(V3 a, V3 b, V3 c, V3 p) {
+  V3 ab = FN(sub)(b, a), ac = FN(sub)(c, a), ap = FN(sub)(p, a);
+  REAL d1 = FN(dot)(ab, ap), d2 = FN(dot)(ac, ap);
+  if (d1 <= R(0) && d2 <= R(0)) return FN(v3)(R(1), R(0), R(0));
+  V3 bp = FN(sub)(p, b);
+  REAL d3 = FN(dot)(ab, bp), d4 = FN(dot)(ac, bp);
+  if (d3 >= R(0) && d4 <= d3) return FN(v3)(R(0), R(1), R(0));
+  REAL vc = d1 * d4 - d3 * d2;
+  REAL v = d1 / (d1 - d3);
+  if (vc <= R(0) && d1 >= R(0) && d3 <= R(0)) return FN(v3)(R(1) - v, v, R(0));
+  V3 cp = FN(sub)(p, c);
+  REAL d5 = FN(dot)(ab, cp), d6 = FN(dot)(ac, cp);
+  if (d6 >= R(0) && d5 <= d6) return FN(v3)(R(0), R(0), R(1));
+  REAL vb = d5 * d2 - d1 * d6;
+  REAL w = d2 / (d2 - d6);
+  if (vb <= R(0) && d2 >= R(0) && d6 <= R(0)) return FN(v3)(R(1) - w, R(0), w);
+  REAL va = d3 * d6 - d5 * d4;
+  w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+  if (va <= R(0) && (d4 - d3) >= R(0) && (d5 - d6) >= R(0)) return FN(v3)(R(0), w, R(1) - w);
+  REAL denom = R(1) / (va + vb + vc);
+  v = vb * denom;
+  w = vc * denom;
+  return FN(v3)(R(1) - v - w, v, w);
+}
+
+/* checkSdfCapsuleCollision (integrator.py:1303-1355): the response of each
+ * triangle vertex to one capsule (r[0..2]); returns 0 when there is none.
+ * As in the reference, the barycentric coordinates of the line/plane
+ * intersection (not the point) are what closestPointOnEdge receives. */
+static inline int FN(capsule_response)(V3 tip, V3 base, REAL radius, V3 triNormal, V3 triCentre, V3 pA, V3 pB,
+                                       V3 pC, REAL kS, V3 r[3]) {
+  V3 cn = FN(normalize)(FN(sub)(tip, base));
+  V3 off = FN(mul)(cn, radius);
+  V3 posA = FN(add)(base, off), posB = FN(sub)(tip, off);
+  REAL denom = FN(dot)(triNormal, cn);
+  if (FN(absr)(denom) < FEPS) return 0;
+  REAL t = FN(dot)(triNormal, FN(sub)(triCentre, base)) / denom;
+  V3 lpi = FN(add)(base, FN(mul)(cn, t));
+  V3 refp = FN(closest_bary)(pA, pB, pC, lpi);
+  V3 sc = FN(closest_on_edge)(refp, posA, posB);
+  V3 bary = FN(closest_bary)(pA, pB, pC, sc);
+  V3 contact = FN(add)(FN(add)(FN(mul)(pA, bary.x), FN(mul)(pB, bary.y)), FN(mul)(pC, bary.z));
+  V3 sv = FN(sub)(sc, contact);
+  REAL distance = FN(length)(sv);
+  REAL pen = radius / FN(length)(sv) - R(1.0);
+  V3 pv = FN(v3)(pen * sv.x, pen * sv.y, pen * sv.z);
+  REAL plen = FN(length)(pv);
+  V3 pn = FN(normalize)(pv);
+  REAL s = bary.x * bary.x + bary.y * bary.y + bary.z * bary.z;
+  if (s < FEPS) return 0;
+  V3 resp = FN(mul)(FN(mul)(pn, plen / s), kS);
+  if (distance > radius) return 0;
+  r[0] = FN(mul)(resp, bary.x);
+  r[1] = FN(mul)(resp, bary.y);
+  r[2] = FN(mul)(resp, bary.z);
+  return 1;
+}
+
+/* collideLaparoscopeDO (integrator.py:981-1050): every surface triangle
+ * against the first three capsules of its env's instrument (base, tip,
+ * radius at env*4 + i); inCollision[env] counts the triangles with a
+ * response.  The responses are computed from the positions at the start of
+ * the pass and then subtracted from the vertices with inverse mass > 0 (the
+ * reference subtracts with unordered atomics on the GPU, so no triangle sees
+ * another's update in a defined order; this is that order-free form).
+ * tri: F x 3 global vertex ids; acc: nV x 3 scratch, left zero. */
+void FN(oracle_collide)(long nF, REAL* pred, const int* tri, const int* triToEnv, const REAL* invMass,
+                        const REAL* capsBase, const REAL* capsTip, const REAL* capsRadius, int* inCollision,
+                        REAL kS, long nV, REAL* acc) {
+  for (long v = 0; v < nV; ++v) FN(st)(acc, v, FN(v3)(R(0), R(0), R(0)));
+  for (long tid = 0; tid < nF; ++tid) {
+    int env = triToEnv[tid];
+    int ia = tri[3 * tid], ib = tri[3 * tid + 1], ic = tri[3 * tid + 2];
+    V3 pA = FN(ld)(pred, ia), pB = FN(ld)(pred, ib), pC = FN(ld)(pred, ic);
+    V3 centre = FN(mul)(FN(add)(FN(add)(pA, pB), pC), R(0.3333));
+    V3 n = FN(normalize)(FN(cross)(FN(sub)(pB, pA), FN(sub)(pC, pA)));
+    V3 tA = FN(v3)(R(0), R(0), R(0)), tB = tA, tC = tA;
+    REAL len = R(0);
+    for (int i = 0; i < 3; ++i) {
+      V3 r[3] = {FN(v3)(R(0), R(0), R(0)), FN(v3)(R(0), R(0), R(0)), FN(v3)(R(0), R(0), R(0))};
+      FN(capsule_response)(FN(ld)(capsTip, env * 4 + i), FN(ld)(capsBase, env * 4 + i), capsRadius[env * 4 + i], n,
+                           centre, pA, pB, pC, kS, r);
+      tA = FN(add)(tA, r[0]);
+      tB = FN(add)(tB, r[1]);
+      tC = FN(add)(tC, r[2]);
+      len = len + FN(length)(r[0]) + FN(length)(r[1]) + FN(length)(r[2]);
+    }
+    if (len > R(0)) inCollision[env] += 1;
+    if (invMass[ia] > R(0)) FN(st)(acc, ia, FN(add)(FN(ld)(acc, ia), tA));
+    if (invMass[ib] > R(0)) FN(st)(acc, ib, FN(add)(FN(ld)(acc, ib), tB));
+    if (invMass[ic] > R(0)) FN(st)(acc, ic, FN(add)(FN(ld)(acc, ic), tC));
+  }
+  for (long v = 0; v < nV; ++v) {
+    FN(st)(pred, v, FN(sub)(FN(ld)(pred, v), FN(ld)(acc, v)));
+    FN(st)(acc, v, FN(v3)(R(0), R(0), R(0)));
+  }
+}
+
 /* Constraint set of one step (the arguments SimModelDO hands to the kernels). */
 typedef struct {
   long nV, nE, nT;
@@ -346,6 +460,9 @@ typedef struct {
   REAL dist_k, vol_k; /* kS or alpha~ depending on mode */
   REAL nh_vol_c, nh_dev_c;
   REAL drag_ks;
+  int collide;        /* 1: collideLaparoscopeDO after the grasp */
+  long nF;            /* surface triangles (collision) */
+  REAL contact_k;     /* globalKsContact */
 } FN(oracle_step_cfg);
 
 /* SimIntegratorDO.stepModel (integrator.py:1640-1744), one frame.  Lambdas are
@@ -355,7 +472,8 @@ void FN(oracle_step_frame)(const FN(oracle_step_cfg)* c, REAL* vertex, REAL* pre
                            const REAL* invMass, REAL* dP, int* cnt, const int* edge,
                            const REAL* edgeRest, const int* tet, const REAL* restVol,
                            const REAL* invRest, REAL* lamE, REAL* lamT, const REAL* drag,
-                           const int* vertexToEnv, const REAL* tips) {
+                           const int* vertexToEnv, const REAL* tips, const int* tri, const int* triToEnv,
+                           const REAL* capsBase, const REAL* capsRadius, int* inCollision) {
   for (int s = 0; s < c->substeps; ++s) {
     FN(oracle_gravity)(c->nV, vertex, pred, vel, invMass, c->gravity, c->damping, c->dt);
     if (!c->lambda_legacy || c->iterations > 1) {
@@ -370,6 +488,9 @@ void FN(oracle_step_frame)(const FN(oracle_step_cfg)* c, REAL* vertex, REAL* pre
       if (c->nh_on) FN(oracle_neohookean)(c->nT, pred, dP, cnt, tet, restVol, invMass, invRest, c->nh_vol_c, c->nh_dev_c, c->dt);
       FN(oracle_apply)(c->nV, pred, dP, cnt);
       if (drag) FN(oracle_drag)(c->nV, pred, drag, vertexToEnv, tips, c->drag_ks);
+      if (c->collide)
+        FN(oracle_collide)(c->nF, pred, tri, triToEnv, invMass, capsBase, tips, capsRadius, inCollision,
+                           c->contact_k, c->nV, dP);
     }
     FN(oracle_bounds)(c->nV, pred, c->ground);
     FN(oracle_pbd_step)(c->nV, vertex, pred, vel, c->dt);
