@@ -61,24 +61,35 @@ def parse():
     ap.add_argument("--serve-inflight", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy before timing (HBM counter calibration)")
+    ap.add_argument("--collision", action="store_true",
+                    help="add the instrument collision pass to every constraint iteration (not the headline)")
     return ap.parse_args()
 
 
 def make_workload(wl, args):
     if args.workload == "mixed":
-        return wl.mixed(args.envs or 1024, seed=args.seed)
-    if args.workload == "ellipsoids":
-        return wl.ellipsoids(args.envs or 64, seed=args.seed)
-    if args.workload == "large":
-        return wl.large(args.envs or 8192, seed=args.seed)
-    return wl.cube(args.seed)
+        w = wl.mixed(args.envs or 1024, seed=args.seed)
+    elif args.workload == "ellipsoids":
+        w = wl.ellipsoids(args.envs or 64, seed=args.seed)
+    elif args.workload == "large":
+        w = wl.large(args.envs or 8192, seed=args.seed)
+    else:
+        w = wl.cube(args.seed)
+    if args.collision:
+        if w.caps_base is None:
+            raise SystemExit(f"workload {w.name} has no instrument capsules")
+        w.model_kwargs = dict(w.model_kwargs, collision=True, globalKsContact=1.0)
+        w.name += "+collision"
+    return w
 
 
 def algorithmic_bytes(w, model):
     """Compulsory bytes of one frame kernel launch (all envs), per env:
     S x [predict 40 V + I x (28 V + 12 E + T x (56 with Neo-Hookean, else 20))
     + integrate 48 V] + 8 B per lambda-carrying constraint per extra iteration
-    + refit (12 V_render + 16 F + 40 N)."""
+    + refit (12 V_render + 16 F + 40 N); with the instrument collision,
+    I x (24 F + 32 V) more per substep (triangle ids, surface positions, and
+    one more read and write of every position)."""
     kw = w.model_kwargs
     S, I = kw["simSubsteps"], kw["simConstraintsSteps"]
     nh = kw.get("neoHookean", False)
@@ -94,6 +105,8 @@ def algorithmic_bytes(w, model):
         lam = 0
         if I > 1:
             lam = 8 * ((E if kw.get("distance") == "xpbd" else 0) + (T if kw.get("volume") == "xpbd" else 0))
+        if kw.get("collision"):
+            proj += 24 * F + 32 * V
         per_sub = 40 * V + I * proj + (I - 1) * lam + 48 * V
         vr = m.surface_vertices().size
         total += S * per_sub + 12 * vr + 16 * F + 40 * N
@@ -120,7 +133,8 @@ def cpu_baseline(pkg, orc, w, model, ren, seconds, seed, threads):
                          nh_vol_c=kw.get("globalVolumeCompliance", 1.0),
                          nh_dev_c=kw.get("globalDeviatoricCompliance", 1.0),
                          dist_compliance=kw.get("distanceCompliance", 0.0),
-                         vol_compliance=kw.get("volumeComplianceXPBD", 0.0), drag_ks=kw.get("globalKsDrag", 1.0))
+                         vol_compliance=kw.get("volumeComplianceXPBD", 0.0), drag_ks=kw.get("globalKsDrag", 1.0),
+                         collide=int(kw.get("collision", False)), contact_k=kw.get("globalKsContact", 1.0))
     dirs = ren.dirs.cpu().numpy()
     origin = np.asarray(ren.camera.origin[:], np.float32)
     light = np.asarray(ren.camera.light_dir[:], np.float32)
@@ -133,7 +147,9 @@ def cpu_baseline(pkg, orc, w, model, ren, seconds, seed, threads):
         k = w.env_mesh[e]
         m = w.meshes[k]
         sim = orc.OracleSim([m], [0], np.float32, tips=w.tips[e:e + 1] if w.tips is not None else None,
-                            initial_velocity=w.initial_velocity)
+                            initial_velocity=w.initial_velocity,
+                            caps_base=w.caps_base[e:e + 1] if w.caps_base is not None else None,
+                            caps_radius=w.caps_radius[e:e + 1] if w.caps_radius is not None else None)
         if w.grasp is not None and w.grasp[e] is not None:
             sim.drag[w.grasp[e]] = 1.0
         sim.step(cfg, 1)
@@ -330,7 +346,8 @@ def main():
                              f"(2x2 supersampled)", "block": integ.block,
                    "dispatch": "lpt" if integ.dispatch_sorted else "table",
                    "chunk": model.topo[0].chunk, "autotuned": tuned is not None,
-                   "form": "batched (1 frame kernel + 1 render kernel per step)"},
+                   "form": "batched (1 frame kernel + 1 render kernel per step)",
+                   **({"collision": True} if w.model_kwargs.get("collision") else {})},
         "p50_step_ms": float(np.percentile(step_ms, 50)), "p99_step_ms": float(np.percentile(step_ms, 99)),
         "frame_kernel_ms": float(frame_ms.mean()), "render_kernel_ms": float(render_ms.mean()),
         "rays_per_s": float((2 if w.rgb else 1) * 4 * w.resolution ** 2 * n / (render_ms.mean() * 1e-3)),

@@ -321,6 +321,31 @@ __device__ __forceinline__ float capsule_response(f3 tip, f3 base, float radius,
   return (length(ra) + length(rb)) + length(rc);
 }
 
+// Box around the points a capsule response can come from: the segment
+// posA..posB of capsule_response grown by the radius (and a relative 1e-5
+// plus 1e-6 absolute margin over rounding).  A triangle whose box misses it
+// is farther than the radius from every point of the segment, so its
+// response is zero: skipping it changes no result.
+struct CapsuleBox {
+  f3 lo, hi;
+};
+
+__device__ __forceinline__ CapsuleBox capsule_box(f3 tip, f3 base, float radius) {
+  const f3 cn = normalize(tip - base);
+  const f3 off = cn * radius;
+  const f3 a = base + off, b = tip - off;
+  const float m = radius * 1.00001f + 1e-6f;
+  return {mk(fminf(a.x, b.x) - m, fminf(a.y, b.y) - m, fminf(a.z, b.z) - m),
+          mk(fmaxf(a.x, b.x) + m, fmaxf(a.y, b.y) + m, fmaxf(a.z, b.z) + m)};
+}
+
+__device__ __forceinline__ bool tri_meets_box(const CapsuleBox& c, f3 pA, f3 pB, f3 pC) {
+  const float x0 = fminf(fminf(pA.x, pB.x), pC.x), x1 = fmaxf(fmaxf(pA.x, pB.x), pC.x);
+  const float y0 = fminf(fminf(pA.y, pB.y), pC.y), y1 = fmaxf(fmaxf(pA.y, pB.y), pC.y);
+  const float z0 = fminf(fminf(pA.z, pB.z), pC.z), z1 = fmaxf(fmaxf(pA.z, pB.z), pC.z);
+  return x1 >= c.lo.x && x0 <= c.hi.x && y1 >= c.lo.y && y0 <= c.hi.y && z1 >= c.lo.z && z0 <= c.hi.z;
+}
+
 // dragConstraintsDO (integrator.py:412-442): p -= |p-tip| * n * drag * ks / 2.
 template <bool FAST>
 __device__ __forceinline__ f3 grasp_pull(f3 p, f3 tip, float dr, float ks) {

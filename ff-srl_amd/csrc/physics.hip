@@ -281,16 +281,28 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
 __device__ __forceinline__ void collide_env(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
                                             int env, const FrameCtx& c, const float* P, int64_t vb, int* contacts,
                                             int block) {
+  CapsuleBox box[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int ci = env * 4 + i;
+    box[i] = capsule_box(mk(b.tips[3 * ci], b.tips[3 * ci + 1], b.tips[3 * ci + 2]),
+                         mk(b.caps_base[3 * ci], b.caps_base[3 * ci + 1], b.caps_base[3 * ci + 2]),
+                         b.caps_radius[ci]);
+  }
   for (int f = threadIdx.x; f < e.nf; f += block) {
     const int4 t = ldi4(b.tris, (int64_t)e.fbase + f);
     const float4 A = ld4(P, vb + t.x), B = ld4(P, vb + t.y), Cc = ld4(P, vb + t.z);
     const f3 pA = xyz(A), pB = xyz(B), pC = xyz(Cc);
+    const bool near0 = tri_meets_box(box[0], pA, pB, pC), near1 = tri_meets_box(box[1], pA, pB, pC),
+               near2 = tri_meets_box(box[2], pA, pB, pC);
+    if (!(near0 || near1 || near2)) continue;  // every response would be zero
     const f3 centre = ((pA + pB) + pC) * 0.3333f;
     const f3 n = normalize(cross(pB - pA, pC - pA));
     f3 r[3] = {mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 0.0f)};
     float len = 0.0f;
 #pragma unroll 1
     for (int i = 0; i < 3; ++i) {
+      if (!(i == 0 ? near0 : i == 1 ? near1 : near2)) continue;  // zero response
       const int ci = env * 4 + i;
       const f3 tip = mk(b.tips[3 * ci], b.tips[3 * ci + 1], b.tips[3 * ci + 2]);
       const f3 base = mk(b.caps_base[3 * ci], b.caps_base[3 * ci + 1], b.caps_base[3 * ci + 2]);
@@ -625,6 +637,7 @@ __global__ void k_collide(ffsrl_batch b, ffsrl_params prm, int env, ffsrl_env_de
     const int ci = env * 4 + i;
     const f3 tip = mk(b.tips[3 * ci], b.tips[3 * ci + 1], b.tips[3 * ci + 2]);
     const f3 base = mk(b.caps_base[3 * ci], b.caps_base[3 * ci + 1], b.caps_base[3 * ci + 2]);
+    if (!tri_meets_box(capsule_box(tip, base, b.caps_radius[ci]), pA, pB, pC)) continue;  // zero response
     len = len + capsule_response(tip, base, b.caps_radius[ci], n, centre, pA, pB, pC, prm.contact_k, r);
   }
   if (!(len > 0.0f)) return;

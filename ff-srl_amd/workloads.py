@@ -47,6 +47,8 @@ class Workload:
     initial_velocity: Optional[np.ndarray] = None
     walk_sigma: float = 0.002
     rgb: bool = False
+    caps_base: Optional[np.ndarray] = None     # (n_env, 4, 3) instrument capsule bases
+    caps_radius: Optional[np.ndarray] = None   # (n_env, 4) capsule radii
 
     @property
     def num_envs(self):
@@ -86,6 +88,20 @@ def _tool(mesh, rng_offset=0.02, k=8):
     return np.repeat(tip[None], 4, axis=0).astype(np.float32), sc.nearest_surface_vertices(mesh, tip, k)
 
 
+# Synthetic instrument around the tool point t (the four instrument points
+# all sit at t): a rod of radius 3 cm reaching 0.5 up from t and two jaws of
+# radius 1.5 cm closing on t from 3 cm to either side; point 3 (the drag
+# point) is a capsule of radius 0, as in laparoscope.py:375-381.
+INSTRUMENT_BASE_OFFSET = np.array([[0.0, 0.5, 0.0], [-0.03, 0.08, 0.0], [0.03, 0.08, 0.0], [0.0, 0.0, 0.0]],
+                                  np.float32)
+INSTRUMENT_RADII = np.array([0.03, 0.015, 0.015, 0.0], np.float32)
+
+
+def _capsules(tips: np.ndarray):
+    tips = np.asarray(tips, np.float32).reshape(-1, 4, 3)
+    return tips + INSTRUMENT_BASE_OFFSET[None], np.tile(INSTRUMENT_RADII, (tips.shape[0], 1))
+
+
 def cube(seed: int = 0) -> Workload:
     mesh = sc.cube_grid(10)
     rng = np.random.default_rng(seed)
@@ -108,8 +124,10 @@ def ellipsoids(n_env: int = 64, tets: int = 20000, seed: int = 0, substeps=10, i
     if volume:
         kw.update(volume="xpbd", volumeComplianceXPBD=0.0)
     env_mesh = [0] * n_env if shared else list(range(n_env))
+    cb, cr = _capsules(np.stack(tips))
     return Workload(f"ellipsoid-{n_env}env-{meshes[0].num_tets}tets", meshes, env_mesh, kw, resolution,
-                    (0.0, 1.6, 5.5), (0.0, 0.45, 0.0), tips=np.stack(tips), grasp=grasp, rgb=True)
+                    (0.0, 1.6, 5.5), (0.0, 0.45, 0.0), tips=np.stack(tips), grasp=grasp, rgb=True,
+                    caps_base=cb, caps_radius=cr)
 
 
 def mixed(n_env: int = 1024, lo: int = 1000, hi: int = 40000, seed: int = 0, substeps=10, iterations=1,
@@ -123,8 +141,9 @@ def mixed(n_env: int = 1024, lo: int = 1000, hi: int = 40000, seed: int = 0, sub
         tips.append(t)
         grasp.append(g if grasp_on[i] else None)
     kw = dict(BASE, simSubsteps=substeps, simConstraintsSteps=iterations, **XPBD_DV, **NH_PARAMS)
+    cb, cr = _capsules(np.stack(tips))
     return Workload(f"mixed-{n_env}env-logU[{lo},{hi}]tets", meshes, list(range(n_env)), kw, resolution,
-                    (0.0, 1.6, 5.5), (0.0, 0.45, 0.0), tips=np.stack(tips), grasp=grasp)
+                    (0.0, 1.6, 5.5), (0.0, 0.45, 0.0), tips=np.stack(tips), grasp=grasp, caps_base=cb, caps_radius=cr)
 
 
 def large(n_env: int = 8192, tets: int = 20000, seed: int = 0) -> Workload:
@@ -161,7 +180,7 @@ def serving(actors: int = 4096, rate: float = 20000.0, tets=(2000, 8000, 20000, 
 def build_model(pkg, w: Workload, device="cuda:0"):
     """SimModelDO for a workload, with the grasp weights set."""
     model = pkg.SimModelDO(w.meshes, w.env_mesh, device=device, tips=w.tips, initialVelocity=w.initial_velocity,
-                           **w.model_kwargs)
+                           capsuleBases=w.caps_base, capsuleRadii=w.caps_radius, **w.model_kwargs)
     if w.grasp is not None:
         import torch
         ids = [np.asarray(g, np.int64) + int(model.env_vbase[i]) for i, g in enumerate(w.grasp) if g is not None]
