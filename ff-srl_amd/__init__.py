@@ -6,6 +6,7 @@ identifier; import it with ``importlib.import_module("ff-srl_amd")``.
 """
 from ._lib import FFSRLError, lib
 from .bvh import SimBVH
+from .instrument import Laparoscope
 from .integrator import SimIntegratorDO
 from .model import SimModelDO
 from .render import RaycastRendererDO, RenderMode, WarpRaycastRendererDO
@@ -14,6 +15,6 @@ from . import scenes
 from . import serve
 
 __all__ = [
-    "FFSRLError", "lib", "SimBVH", "SimIntegratorDO", "SimModelDO", "RaycastRendererDO", "RenderMode",
+    "FFSRLError", "lib", "Laparoscope", "SimBVH", "SimIntegratorDO", "SimModelDO", "RaycastRendererDO", "RenderMode",
     "WarpRaycastRendererDO", "autotune", "scenes", "serve",
 ]

@@ -70,6 +70,13 @@ class Requests(C.Structure):
     ]
 
 
+class Instrument(C.Structure):
+    _fields_ = [
+        ("n_env", C.c_int32), ("xform", _vp), ("height", _vp), ("drag_cut_heat", _vp),
+        ("drag_cut_heat_update", _vp), ("violation", _vp),
+    ]
+
+
 REQ_STEP, REQ_RESET = 0, 1
 
 _i32, _f32, _i64 = C.c_int32, C.c_float, C.c_int64
@@ -94,6 +101,10 @@ SIGNATURES = {
     "ffsrl_apply": [_P(Batch), _i32, _i32, _vp],
     "ffsrl_grasp": [_P(Batch), _P(Params), _i32, _i32, _vp],
     "ffsrl_collide": [_P(Batch), _P(Params), _i32, _i32, _vp],
+    "ffsrl_instrument_actions": [_P(Batch), _P(Instrument), _vp, _f32, _vp],
+    "ffsrl_instrument_cartesian": [_P(Batch), _P(Instrument), _vp, _vp, _vp, _vp],
+    "ffsrl_instrument_set_rcm": [_P(Batch), _P(Instrument), _vp, _vp, _vp],
+    "ffsrl_instrument_update": [_P(Batch), _P(Instrument), _vp],
     "ffsrl_bounds_integrate": [_P(Batch), _P(Params), _i32, _i32, _vp],
     "ffsrl_zero_lambdas": [_P(Batch), _i32, _i32, _vp],
     "ffsrl_bvh_refit_plain": [_P(Batch), _i32, _i32, _vp],

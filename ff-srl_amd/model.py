@@ -25,7 +25,7 @@ from typing import Optional, Sequence
 import numpy as np
 import torch
 
-from ._lib import Batch, EnvDesc, ENV_DESC_FIELDS, OFF, PBD, XPBD, Params, lib
+from ._lib import Batch, EnvDesc, ENV_DESC_FIELDS, Instrument, OFF, PBD, XPBD, Params, lib
 from .scenes import TetMesh
 
 MAX_STACK = 32  # traversal stack entries per ray (render.py:424, 924)
@@ -184,7 +184,8 @@ class SimModelDO:
                  globalLaparoscopeDragLookupRadius=0.15, lambdaLegacy=False, bins=8,
                  initialVelocity: Optional[np.ndarray] = None, tips: Optional[np.ndarray] = None,
                  constraint_order: str = "local", globalKsContact=1.0, collision: bool = False,
-                 capsuleBases: Optional[np.ndarray] = None, capsuleRadii: Optional[np.ndarray] = None):
+                 capsuleBases: Optional[np.ndarray] = None, capsuleRadii: Optional[np.ndarray] = None,
+                 laparoscopes=None, workspaceLow=(-1e5, -1e5, -1e5), workspaceHigh=(1e5, 1e5, 1e5)):
         self.device = torch.device(device)
         self.meshes = list(meshes)
         env_mesh = list(range(len(self.meshes))) if env_mesh is None else list(env_mesh)
@@ -211,7 +212,18 @@ class SimModelDO:
         self.lambda_legacy = bool(lambdaLegacy)
         self.globalKsContact = globalKsContact
         self.collision = bool(collision)
+        self.workspaceLow = tuple(float(v) for v in workspaceLow)
+        self.workspaceHigh = tuple(float(v) for v in workspaceHigh)
         self.params = self._make_params()
+        if laparoscopes is not None:
+            # instrument points, capsules and heights from the laparoscope geometry
+            laps = list(laparoscopes) if isinstance(laparoscopes, (list, tuple)) else [laparoscopes] * len(env_mesh)
+            if len(laps) != len(env_mesh):
+                raise ValueError(f"{len(laps)} laparoscopes for {len(env_mesh)} environments")
+            pts = [lap.initial_points() for lap in laps]
+            tips = np.stack([p[0] for p in pts]).astype(np.float32)
+            capsuleBases = np.stack([p[1] for p in pts]).astype(np.float32)
+            capsuleRadii = np.stack([p[2] for p in pts]).astype(np.float32)
 
         self.constraint_order = constraint_order
         self.topo = _parallel(lambda m: MeshTopology(m, bins, constraint_order), self.meshes)
@@ -219,6 +231,9 @@ class SimModelDO:
         self._build_envs()
         self._build_state(initialVelocity, tips, capsuleBases, capsuleRadii)
         self._batch = self._make_batch()
+        self._instrument = None
+        if laparoscopes is not None:
+            self._build_instrument(laps)
         self._order = None
 
     # ------------------------------------------------------------------ setup
@@ -411,6 +426,94 @@ class SimModelDO:
         b.host_level_off = self.level_off_host.ctypes.data
         return b
 
+    def _build_instrument(self, laps):
+        """Transforms, heights, clamp state and workspace flags of every env's
+        laparoscope (``modelDO.py:1208-1225``)."""
+        dev = self.device
+        xf = np.concatenate([lap.stored_transforms() for lap in laps])           # (n*7, 4, 4) transposed
+        self.laparoscopeXForm = torch.from_numpy(xf).to(dev)
+        self.laparoscopeInitialXForm = self.laparoscopeXForm.clone()
+        h = np.stack([lap.initial_points()[3] for lap in laps]).astype(np.float32)
+        self.laparoscopeHeight = torch.from_numpy(h).to(dev)
+        n = self.numEnvs
+        self.laparoscopeDragCutHeat = torch.zeros(3 * n, dtype=torch.int32, device=dev)
+        self.laparoscopeDragCutHeatUpdate = torch.zeros(3 * n, dtype=torch.int32, device=dev)
+        self.laparoscopeWorkspaceViolation = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        self._workspace = torch.tensor([self.workspaceLow, self.workspaceHigh], dtype=torch.float32, device=dev)
+        ins = Instrument()
+        ins.n_env = n
+        ins.xform = _ptr(self.laparoscopeXForm)
+        ins.height = _ptr(self.laparoscopeHeight)
+        ins.drag_cut_heat = _ptr(self.laparoscopeDragCutHeat)
+        ins.drag_cut_heat_update = _ptr(self.laparoscopeDragCutHeatUpdate)
+        ins.violation = _ptr(self.laparoscopeWorkspaceViolation)
+        self._instrument = ins
+
+    @property
+    def instrument(self) -> Instrument:
+        if self._instrument is None:
+            raise RuntimeError("this model has no laparoscopes (pass laparoscopes= to SimModelDO)")
+        return self._instrument
+
+    def _actions(self, a, width: int) -> torch.Tensor:
+        t = torch.as_tensor(a, dtype=torch.float32, device=self.device).reshape(-1)
+        if t.numel() != width * self.numEnvs:
+            raise ValueError(f"expected {width} action values per environment, got {t.numel()}")
+        return t.contiguous()
+
+    # ------------------------------------------------------------- instrument
+    def applyActions(self, actions):
+        """``modelDO.py:1331-1362``: per env (rod insertion, rod roll, rotation-
+        centre pitch, yaw, jaw opening in radians); moves the instruments and
+        reselects the grasp of envs whose clamp state changed."""
+        a = self._actions(actions, 5)
+        lib.ffsrl_instrument_actions(self._batch, self.instrument, a.data_ptr(),
+                                     float(self.globalLaparoscopeDragLookupRadius), self.stream())
+
+    def applyCartesianActions(self, cartesianActions):
+        """``modelDO.py:1364-1383``: move each env's drag point by (dx, dy, dz)
+        about its rotation centre."""
+        a = self._actions(cartesianActions, 3)
+        lib.ffsrl_instrument_cartesian(self._batch, self.instrument, a.data_ptr(), None, None, self.stream())
+
+    def applyCartesianActionsInWorkspace(self, cartesianActions):
+        """``modelDO.py:1385-1404``: as applyCartesianActions, dropping the
+        components that would leave [workspaceLow, workspaceHigh]."""
+        a = self._actions(cartesianActions, 3)
+        w = self._workspace
+        lib.ffsrl_instrument_cartesian(self._batch, self.instrument, a.data_ptr(), w[0].data_ptr(),
+                                       w[1].data_ptr(), self.stream())
+
+    def setRotationCenter(self, laparoscopeTranslation, laparoscopeRotation):
+        """``modelDO.py:1570-1587``: place each env's rotation centre at
+        T(translation) Rx Ry Rz(rotation)."""
+        t = self._actions(laparoscopeTranslation, 3)
+        r = self._actions(laparoscopeRotation, 3)
+        lib.ffsrl_instrument_set_rcm(self._batch, self.instrument, t.data_ptr(), r.data_ptr(), self.stream())
+
+    def setRotationCenterInitialPosition(self, laparoscopeTranslation, laparoscopeRotation):
+        """``modelDO.py:1416-1419``."""
+        self.setRotationCenter(laparoscopeTranslation, laparoscopeRotation)
+        self.laparoscopeInitialXForm.copy_(self.laparoscopeXForm)
+
+    def setEffectorInitialPosition(self, cartesianActions):
+        """``modelDO.py:1406-1414``."""
+        self.applyCartesianActions(cartesianActions)
+        self.laparoscopeInitialXForm.copy_(self.laparoscopeXForm)
+        self.initialBases.copy_(self.laparoscopeBase)
+        self.initialTips.copy_(self.laparoscopeTip)
+
+    def forceLaparoscopeClamp(self, envs):
+        """``modelDO.py:1524-1531``: close the jaws of the envs with envs[i] > 0."""
+        e = torch.clamp(torch.as_tensor(envs, dtype=torch.float32, device=self.device).reshape(-1), 0.0, 1.0)
+        a = torch.zeros((self.numEnvs, 5), dtype=torch.float32, device=self.device)
+        a[:, 4] = -e
+        self.applyActions(a)
+
+    def getLaparoscopeWorkspaceViolation(self):
+        """``modelDO.py:1508-1512``."""
+        return self.laparoscopeWorkspaceViolation[:self.numEnvs]
+
     def enable_plain(self):
         """Allocate the dP / count scratch of the plain (per-kernel) form."""
         if self.dp is None:
@@ -517,6 +620,11 @@ class SimModelDO:
             self.drag[s] = 0.0
             self.laparoscopeTip[int(i)] = self.initialTips[int(i)]
             self.laparoscopeBase[int(i)] = self.initialBases[int(i)]
+            if self._instrument is not None:
+                k = int(i)
+                self.laparoscopeXForm[7 * k:7 * k + 7] = self.laparoscopeInitialXForm[7 * k:7 * k + 7]
+                self.laparoscopeDragCutHeat[3 * k:3 * k + 3] = 0
+                self.laparoscopeDragCutHeatUpdate[3 * k:3 * k + 3] = 0
 
     def set_grasp(self, env: int, local_ids, weight: float = 1.0):
         """Force grasp weights on env-local vertices (``forceClampsDragKernel``,

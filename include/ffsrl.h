@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 7
+#define FFSRL_ABI_VERSION 8
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -220,6 +220,57 @@ int ffsrl_move_tools(const ffsrl_batch* b, const float* delta, const float* lo, 
  * modelDO.py:247-279).  state/update are int32 [n_env*3] device arrays. */
 int ffsrl_grasp_select(const ffsrl_batch* b, float radius, const int32_t* state,
                        const int32_t* update, void* stream);
+
+/* ---- instrument kinematics ---------------------------------------------- */
+/* The instrument of each env (laparoscope.py:375-386): seven 4x4 transforms
+ * per env, stored transposed (row-vector convention, as the reference keeps
+ * them), at xform[(env*7 + k)*16]:
+ *   0 rod capsule, 1 left jaw, 2 right jaw, 3 rotation centre (the remote
+ *   centre of motion), 4 left jaw pivot, 5 right jaw pivot, 6 rod.
+ * A transform applied to 3 carries every other one, applied to 0 carries
+ * 1, 2, 4, 5, 6, applied to 4 carries 1 and to 5 carries 2
+ * (applyTransformationToChildren, modelDO.py:364-394).  height holds the
+ * capsule heights (rod, left jaw, right jaw, 0) per env; drag_cut_heat and
+ * drag_cut_heat_update the clamp state (3 ints per env, drag first);
+ * violation the workspace flags (NULL when unused).  The capsule points
+ * derived from the transforms are written to the batch's tips and
+ * caps_base. */
+typedef struct ffsrl_instrument {
+  int32_t n_env;
+  float* xform;                   /* [n_env*7*16] */
+  const float* height;            /* [n_env*4] */
+  int32_t* drag_cut_heat;         /* [n_env*3] */
+  int32_t* drag_cut_heat_update;  /* [n_env*3] */
+  int32_t* violation;             /* [n_env] or NULL */
+} ffsrl_instrument;
+
+/* applyActions (modelDO.py:1331-1362): actions float[n_env*5] per env (rod
+ * insertion, rod roll, rotation-centre pitch, yaw, jaw opening in radians);
+ * applyActionsKernel (:175-199) with its three transform threads of an env
+ * run in thread order (rod, rotation centre, jaws), the jaw state machine
+ * rotateClampsFunc (:301-362), updateLaparoscopeKernel (:218-245) into
+ * tips / caps_base, then updateClampsDragKernel (:247-279) with
+ * lookup_radius (lookup_radius < 0 skips it, as the jaw animation of
+ * forceLaparoscopeClampVertex does, :1533-1568). */
+int ffsrl_instrument_actions(const ffsrl_batch* b, const ffsrl_instrument* ins, const float* actions,
+                             float lookup_radius, void* stream);
+/* applyCartesianActions / applyCartesianActionsInWorkspace (modelDO.py:
+ * 1364-1404; transfromCartesianActionsKernel[InWorkspace], :55-154): moves
+ * the drag point (tip[env*4+3]) by actions[env*3..+2] about the rotation
+ * centre, then updates tips / caps_base.  With lo and hi (device float[3]
+ * each) the components that would take the drag point out of the box are
+ * dropped and violation[env] is set to 1 (0 otherwise) for envs with a
+ * nonzero action. */
+int ffsrl_instrument_cartesian(const ffsrl_batch* b, const ffsrl_instrument* ins, const float* actions,
+                               const float* lo, const float* hi, void* stream);
+/* setRotationCenter (modelDO.py:1570-1587; setRotationCenterTransformKernel,
+ * :156-173): the rotation centre of env e becomes
+ * T(translation[e]) Rx Ry Rz(rotation[e]) (radians), carrying the rest of the
+ * instrument; then updates tips / caps_base. */
+int ffsrl_instrument_set_rcm(const ffsrl_batch* b, const ffsrl_instrument* ins, const float* translation,
+                             const float* rotation, void* stream);
+/* updateLaparoscopeKernel (modelDO.py:218-245) alone. */
+int ffsrl_instrument_update(const ffsrl_batch* b, const ffsrl_instrument* ins, void* stream);
 
 /* ---- serving ------------------------------------------------------------ */
 

@@ -25,6 +25,18 @@ Parity status of each restated function (see the tests that pin them):
                                       SAH stop rule); exact float promotion of
                                       the reference's NumPy 1.x code is parity
                                       unpinned
+  instrument collision              - pinned: sphere-on-triangle closed form,
+                                      no-contact cases, the barycentric-point
+                                      quirk, summed responses
+                                      (tests/test_collision_oracle.py)
+  instrument kinematics             - pinned: float64 matrix algebra (inverse,
+                                      Rx Ry Rz products), rigid-motion
+                                      invariants of the actions, the
+                                      remote-centre property of Cartesian
+                                      actions, the jaw angle limits and clamp
+                                      state table (tests/test_instrument_oracle.py);
+                                      Warp's exact operation order for mat44
+                                      inverse / quaternions is parity unpinned
 """
 from __future__ import annotations
 
@@ -366,3 +378,72 @@ def collide(pred, tri, tri_env, inv_mass, caps_base, caps_tip, caps_radius, k=1.
                                            _p(f(caps_base)), _p(f(caps_tip)), _p(f(caps_radius)), _p(inc), real(k),
                                            C.c_long(pred.shape[0]), _p(acc))
     return pred, inc
+
+
+class OracleInstrument:
+    """Restated instrument kinematics of n envs (modelDO.py:55-289, 1331-1404,
+    1570-1587): seven transforms per env stored transposed (``xf``, n*7
+    matrices), capsule heights (n, 4), the clamp state ``dch`` / ``dchu``
+    (n*3 int32, drag first), workspace ``violation`` (n int32) and the
+    capsule points ``tips`` / ``bases`` (n*4, 3)."""
+
+    def __init__(self, xf, height, tips, bases, dtype=np.float32):
+        self.dtype = dtype
+        self.xf = np.ascontiguousarray(np.asarray(xf).reshape(-1, 4, 4), dtype).copy()
+        self.n = self.xf.shape[0] // 7
+        self.height = np.ascontiguousarray(np.asarray(height).reshape(-1), dtype).copy()
+        self.tips = np.ascontiguousarray(np.asarray(tips).reshape(-1, 3), dtype).copy()
+        self.bases = np.ascontiguousarray(np.asarray(bases).reshape(-1, 3), dtype).copy()
+        self.dch = np.zeros(3 * self.n, np.int32)
+        self.dchu = np.zeros(3 * self.n, np.int32)
+        self.violation = np.zeros(self.n, np.int32)
+
+    def _fn(self, name):
+        return getattr(lib(), name + ("_f32" if self.dtype == np.float32 else "_f64"))
+
+    def _f(self, a):
+        return np.ascontiguousarray(np.asarray(a).reshape(-1), self.dtype)
+
+    def actions(self, actions):
+        """applyActions without the grasp selection: (n, 5) actions."""
+        a = self._f(actions)
+        self._fn("oracle_instrument_actions")(C.c_int(self.n), _p(self.xf), _p(self.height), _p(self.dch),
+                                              _p(self.dchu), _p(a), _p(self.tips), _p(self.bases))
+
+    def cartesian(self, actions, lo=None, hi=None):
+        a = self._f(actions)
+        lo = None if lo is None else self._f(lo)
+        hi = None if hi is None else self._f(hi)
+        self._fn("oracle_instrument_cartesian")(C.c_int(self.n), _p(self.xf), _p(self.height), _p(a),
+                                                None if lo is None else _p(lo), None if hi is None else _p(hi),
+                                                _p(self.violation), _p(self.tips), _p(self.bases))
+
+    def set_rcm(self, translation, rotation):
+        t, r = self._f(translation), self._f(rotation)
+        self._fn("oracle_instrument_set_rcm")(C.c_int(self.n), _p(self.xf), _p(self.height), _p(t), _p(r),
+                                              _p(self.tips), _p(self.bases))
+
+    def update(self):
+        self._fn("oracle_instrument_update")(C.c_int(self.n), _p(self.xf), _p(self.height), _p(self.tips),
+                                             _p(self.bases))
+
+    def matrix(self, env, k):
+        """Column-vector form of transform k of env."""
+        return self.xf[env * 7 + k].T.copy()
+
+
+def m4_inverse(m, dtype=np.float64):
+    m = np.ascontiguousarray(m, dtype)
+    out = np.zeros((4, 4), dtype)
+    sfx = "_f32" if dtype == np.float32 else "_f64"
+    getattr(lib(), "oracle_m4_inverse" + sfx)(_p(m), _p(out))
+    return out
+
+
+def m4_transformation(sx, sy, sz, ax, ay, az, dtype=np.float64):
+    """getTransformationMatrix (modelDO.py:396-416), column-vector form."""
+    real = C.c_float if dtype == np.float32 else C.c_double
+    out = np.zeros((4, 4), dtype)
+    sfx = "_f32" if dtype == np.float32 else "_f64"
+    getattr(lib(), "oracle_m4_transformation" + sfx)(*(real(v) for v in (sx, sy, sz, ax, ay, az)), _p(out))
+    return out

@@ -705,6 +705,255 @@ void FN(oracle_render_rgb)(int H, int W, int mode, const REAL* origin, const REA
   free(ray);
 }
 
+/* ---- instrument kinematics (modelDO.py:55-289, 1331-1404, 1570-1587) ----
+ * Transforms are 4x4, stored transposed as the reference stores them
+ * (xf[(env*7+k)*16 + 4*row + col] is the transposed matrix).  A4 below is a
+ * column-vector matrix a[4*i + j] = row i, column j.  Products sum k = 0..3
+ * left to right.  Warp's mat44 inverse, quat_from_axis_angle, quaternion to
+ * matrix and asin are restated from their definitions; their exact operation
+ * order in warp-lang 0.10.1 is parity unpinned (tests pin the geometry). */
+#define A4 FN(mat4_t)
+typedef struct { REAL a[16]; } A4;
+static inline A4 FN(m4_load)(const REAL* xf, int id) {
+  A4 r;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) r.a[4 * i + j] = xf[id * 16 + 4 * j + i];
+  return r;
+}
+static inline void FN(m4_store)(REAL* xf, int id, A4 m) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) xf[id * 16 + 4 * j + i] = m.a[4 * i + j];
+}
+static inline A4 FN(m4_T)(A4 m) {
+  A4 r;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) r.a[4 * i + j] = m.a[4 * j + i];
+  return r;
+}
+static inline A4 FN(m4_mul)(A4 x, A4 y) {
+  A4 r;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      REAL s = R(0);
+      for (int k = 0; k < 4; ++k) s = s + x.a[4 * i + k] * y.a[4 * k + j];
+      r.a[4 * i + j] = s;
+    }
+  return r;
+}
+/* adjugate / determinant with the 2x2 minors of rows (0,1) and (2,3) */
+static inline A4 FN(m4_inv)(A4 x) {
+#define E(i, j) x.a[4 * (i) + (j)]
+  REAL s0 = E(0, 0) * E(1, 1) - E(1, 0) * E(0, 1), s1 = E(0, 0) * E(1, 2) - E(1, 0) * E(0, 2);
+  REAL s2 = E(0, 0) * E(1, 3) - E(1, 0) * E(0, 3), s3 = E(0, 1) * E(1, 2) - E(1, 1) * E(0, 2);
+  REAL s4 = E(0, 1) * E(1, 3) - E(1, 1) * E(0, 3), s5 = E(0, 2) * E(1, 3) - E(1, 2) * E(0, 3);
+  REAL c5 = E(2, 2) * E(3, 3) - E(3, 2) * E(2, 3), c4 = E(2, 1) * E(3, 3) - E(3, 1) * E(2, 3);
+  REAL c3 = E(2, 1) * E(3, 2) - E(3, 1) * E(2, 2), c2 = E(2, 0) * E(3, 3) - E(3, 0) * E(2, 3);
+  REAL c1 = E(2, 0) * E(3, 2) - E(3, 0) * E(2, 2), c0 = E(2, 0) * E(3, 1) - E(3, 0) * E(2, 1);
+  REAL det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+  A4 r;
+  if (det == R(0)) {
+    for (int i = 0; i < 16; ++i) r.a[i] = R(0);
+    return r;
+  }
+  REAL id = R(1) / det;
+  r.a[0] = (E(1, 1) * c5 - E(1, 2) * c4 + E(1, 3) * c3) * id;
+  r.a[1] = (-E(0, 1) * c5 + E(0, 2) * c4 - E(0, 3) * c3) * id;
+  r.a[2] = (E(3, 1) * s5 - E(3, 2) * s4 + E(3, 3) * s3) * id;
+  r.a[3] = (-E(2, 1) * s5 + E(2, 2) * s4 - E(2, 3) * s3) * id;
+  r.a[4] = (-E(1, 0) * c5 + E(1, 2) * c2 - E(1, 3) * c1) * id;
+  r.a[5] = (E(0, 0) * c5 - E(0, 2) * c2 + E(0, 3) * c1) * id;
+  r.a[6] = (-E(3, 0) * s5 + E(3, 2) * s2 - E(3, 3) * s1) * id;
+  r.a[7] = (E(2, 0) * s5 - E(2, 2) * s2 + E(2, 3) * s1) * id;
+  r.a[8] = (E(1, 0) * c4 - E(1, 1) * c2 + E(1, 3) * c0) * id;
+  r.a[9] = (-E(0, 0) * c4 + E(0, 1) * c2 - E(0, 3) * c0) * id;
+  r.a[10] = (E(3, 0) * s4 - E(3, 1) * s2 + E(3, 3) * s0) * id;
+  r.a[11] = (-E(2, 0) * s4 + E(2, 1) * s2 - E(2, 3) * s0) * id;
+  r.a[12] = (-E(1, 0) * c3 + E(1, 1) * c1 - E(1, 2) * c0) * id;
+  r.a[13] = (E(0, 0) * c3 - E(0, 1) * c1 + E(0, 2) * c0) * id;
+  r.a[14] = (-E(3, 0) * s3 + E(3, 1) * s1 - E(3, 2) * s0) * id;
+  r.a[15] = (E(2, 0) * s3 - E(2, 1) * s1 + E(2, 2) * s0) * id;
+#undef E
+  return r;
+}
+static inline V3 FN(m4_point)(A4 m, V3 p) {
+  REAL o[3];
+  for (int i = 0; i < 3; ++i) o[i] = ((m.a[4 * i] * p.x + m.a[4 * i + 1] * p.y) + m.a[4 * i + 2] * p.z) + m.a[4 * i + 3];
+  return FN(v3)(o[0], o[1], o[2]);
+}
+/* quat_from_axis_angle: (axis sin(a/2), cos(a/2)); q[3] = w */
+static inline void FN(quat)(V3 axis, REAL angle, REAL q[4]) {
+  REAL h = angle * R(0.5), s = (REAL)SIN(h);
+  q[0] = axis.x * s; q[1] = axis.y * s; q[2] = axis.z * s; q[3] = (REAL)COS(h);
+}
+/* quat_rotate: x (2w^2 - 1) + (v x x) 2w + v 2(v . x) */
+static inline V3 FN(qrot)(const REAL q[4], V3 x) {
+  V3 v = FN(v3)(q[0], q[1], q[2]);
+  REAL c = R(2) * q[3] * q[3] - R(1), d = R(2) * FN(dot)(v, x);
+  return FN(add)(FN(add)(FN(mul)(x, c), FN(mul)(FN(cross)(v, x), R(2) * q[3])), FN(mul)(v, d));
+}
+/* wp.mat44(pos, rot, scale=1): rotation columns = rotated axes, then translation */
+static inline A4 FN(m4_rigid)(V3 t, const REAL q[4]) {
+  V3 c0 = FN(qrot)(q, FN(v3)(R(1), R(0), R(0))), c1 = FN(qrot)(q, FN(v3)(R(0), R(1), R(0)));
+  V3 c2 = FN(qrot)(q, FN(v3)(R(0), R(0), R(1)));
+  A4 r = {{c0.x, c1.x, c2.x, t.x, c0.y, c1.y, c2.y, t.y, c0.z, c1.z, c2.z, t.z, R(0), R(0), R(0), R(1)}};
+  return r;
+}
+/* getTransformationMatrix (modelDO.py:396-416) */
+static inline A4 FN(m4_transformation)(REAL sx, REAL sy, REAL sz, REAL ax, REAL ay, REAL az) {
+  V3 z = FN(v3)(R(0), R(0), R(0));
+  REAL q0[4], qx[4], qy[4], qz[4];
+  FN(quat)(z, R(0), q0);
+  FN(quat)(FN(v3)(R(1), R(0), R(0)), ax, qx);
+  FN(quat)(FN(v3)(R(0), R(1), R(0)), ay, qy);
+  FN(quat)(FN(v3)(R(0), R(0), R(1)), az, qz);
+  A4 S = FN(m4_rigid)(FN(v3)(sx, sy, sz), q0), X = FN(m4_rigid)(z, qx), Y = FN(m4_rigid)(z, qy);
+  A4 Z = FN(m4_rigid)(z, qz);
+  return FN(m4_mul)(S, FN(m4_mul)(X, FN(m4_mul)(Y, Z)));
+}
+/* applyTransformationToChild / ...Children (modelDO.py:364-394) */
+static void FN(carry_child)(REAL* xf, A4 m, int parent, int child) {
+  A4 P = FN(m4_load)(xf, parent), C = FN(m4_load)(xf, child);
+  FN(m4_store)(xf, child, FN(m4_mul)(P, FN(m4_mul)(m, FN(m4_mul)(FN(m4_inv)(P), C))));
+}
+static void FN(carry_children)(REAL* xf, A4 m, int parent) {
+  int env = parent / 7, kind = parent % 7;
+  if (kind == 4) FN(carry_child)(xf, m, parent, env * 7 + 1);
+  if (kind == 5) FN(carry_child)(xf, m, parent, env * 7 + 2);
+  if (kind == 3)
+    for (int i = 0; i < 7; ++i)
+      if (i != kind) FN(carry_child)(xf, m, parent, env * 7 + i);
+  if (kind == 0)
+    for (int i = 1; i < 7; ++i)
+      if (i != 3) FN(carry_child)(xf, m, parent, env * 7 + i);
+}
+/* transformFunc (modelDO.py:291-299) */
+static void FN(transform_func)(REAL* xf, int id, A4 X, A4 m) {
+  FN(carry_children)(xf, m, id);
+  FN(m4_store)(xf, id, FN(m4_mul)(X, m));
+}
+/* rotateClampsFunc (modelDO.py:301-362) */
+static void FN(rotate_clamps)(REAL* xf, int* dch, int* dchu, REAL action, int env) {
+  const REAL PI = R(3.14159265358979323846);
+  int rod = env * 7, left = env * 7 + 4, right = env * 7 + 5, drag = env * 3;
+  REAL diff = action * R(180.0) / PI;
+  if (diff == R(0)) return;
+  A4 L = FN(m4_load)(xf, left), Rt = FN(m4_load)(xf, right);
+  A4 local = FN(m4_mul)(FN(m4_inv)(FN(m4_load)(xf, rod)), Rt);
+  REAL angle = (REAL)ASIN(local.a[4 * 2 + 1]) * R(180.0) / PI;
+  int apply = 0;
+  if (dch[drag] == 1 && dchu[drag] == 1) dchu[drag] = 0;
+  if (action > R(0)) {
+    if (angle < R(85)) {
+      if (angle + diff > R(85)) diff = R(85) - angle;
+      apply = 1;
+    }
+    if (angle > R(43) && dch[drag] == 1) { dch[drag] = 0; dchu[drag] = 1; }
+  } else {
+    if (angle > R(40)) {
+      if (angle + diff < R(40)) diff = R(40) - angle;
+      apply = 1;
+    }
+    if (angle < R(43) && dch[drag] == 0) { dch[drag] = 1; dchu[drag] = 1; }
+  }
+  if (!apply) return;
+  REAL rad = diff * PI / R(180.0);
+  A4 m = FN(m4_transformation)(R(0), R(0), R(0), rad, R(0), R(0));
+  A4 mi = FN(m4_transformation)(R(0), R(0), R(0), -rad, R(0), R(0));
+  FN(m4_store)(xf, left, FN(m4_mul)(L, mi));
+  FN(carry_children)(xf, mi, left);
+  FN(m4_store)(xf, right, FN(m4_mul)(Rt, m));
+  FN(carry_children)(xf, m, right);
+}
+/* updateLaparoscopeKernel (modelDO.py:218-245) */
+void FN(oracle_instrument_update)(int nEnv, const REAL* xf, const REAL* height, REAL* tips, REAL* bases) {
+  for (int tid = 0; tid < 4 * nEnv; ++tid) {
+    int env = tid / 4, kind = tid % 4;
+    A4 X;
+    V3 lo, hi;
+    if (kind < 3) {
+      X = FN(m4_load)(xf, env * 7 + kind);
+      REAL h = height[env * 4 + kind];
+      lo = FN(v3)(R(0), R(0), R(0) - h / R(1.85));
+      hi = FN(v3)(R(0), R(0), R(0) + h / R(1.85));
+    } else {
+      X = FN(m4_load)(xf, env * 7);
+      REAL rod = height[env * 4], clamp = height[env * 4 + 2];
+      lo = FN(v3)(R(0), R(0), R(0) - rod * R(0.5) - clamp * R(1.0));
+      hi = FN(v3)(R(0), R(0), R(0) + rod * R(0.5) + clamp * R(1.0));
+    }
+    FN(st)(bases, tid, FN(m4_point)(X, lo));
+    FN(st)(tips, tid, FN(m4_point)(X, hi));
+  }
+}
+/* applyActionsKernel (modelDO.py:175-199), threads in tid order, then
+ * updateLaparoscopeKernel; actions [nEnv*5] */
+void FN(oracle_instrument_actions)(int nEnv, REAL* xf, const REAL* height, int* dch, int* dchu,
+                                   const REAL* actions, REAL* tips, REAL* bases) {
+  for (int tid = 0; tid < 7 * nEnv; ++tid) {
+    int env = tid / 7, kind = tid % 7;
+    A4 X = FN(m4_load)(xf, tid);
+    const REAL* a = actions + 5 * env;
+    if (kind == 0)
+      FN(transform_func)(xf, tid, X, FN(m4_transformation)(R(0), R(0), a[0], R(0), R(0), a[1]));
+    else if (kind == 3)
+      FN(transform_func)(xf, tid, X, FN(m4_transformation)(R(0), R(0), R(0), a[2], a[3], R(0)));
+    else if (kind == 4)
+      FN(rotate_clamps)(xf, dch, dchu, a[4], env);
+  }
+  FN(oracle_instrument_update)(nEnv, xf, height, tips, bases);
+}
+/* transfromCartesianActionsKernel[InWorkspace] (modelDO.py:55-154), then
+ * updateLaparoscopeKernel; lo/hi NULL for the unbounded form */
+void FN(oracle_instrument_cartesian)(int nEnv, REAL* xf, const REAL* height, const REAL* actions,
+                                     const REAL* lo, const REAL* hi, int* violation, REAL* tips, REAL* bases) {
+  for (int env = 0; env < nEnv; ++env) {
+    REAL av[3] = {actions[3 * env], actions[3 * env + 1], actions[3 * env + 2]};
+    if (av[0] == R(0) && av[1] == R(0) && av[2] == R(0)) continue;
+    V3 cur = FN(ld)(tips, 4 * env + 3);
+    if (lo && hi) {
+      violation[env] = 0;
+      REAL nw[3] = {cur.x + av[0], cur.y + av[1], cur.z + av[2]};
+      for (int c = 0; c < 3; ++c)
+        if (nw[c] < lo[c] || nw[c] > hi[c]) { av[c] = R(0); violation[env] = 1; }
+    }
+    A4 rodS = FN(m4_T)(FN(m4_load)(xf, 7 * env));      /* stored (transposed) forms, as the kernel reads them */
+    A4 rcS = FN(m4_T)(FN(m4_load)(xf, 7 * env + 3));
+    V3 rel = FN(m4_point)(FN(m4_T)(FN(m4_inv)(rcS)), cur);
+    V3 delta = FN(m4_point)(rcS, FN(v3)(av[0], av[1], av[2]));
+    V3 nrel = FN(add)(rel, delta);
+    REAL r0 = FN(length)(rel), r1 = FN(length)(nrel);
+    V3 vc = FN(cross)(rel, nrel);
+    REAL sinT = FN(length)(vc) / (FN(length)(rel) * FN(length)(nrel));
+    REAL q[4];
+    FN(quat)(FN(normalize)(vc), (REAL)ASIN(sinT), q);
+    FN(transform_func)(xf, 7 * env, FN(m4_T)(rodS), FN(m4_transformation)(R(0), R(0), r1 - r0, R(0), R(0), R(0)));
+    FN(transform_func)(xf, 7 * env + 3, FN(m4_T)(rcS), FN(m4_rigid)(FN(v3)(R(0), R(0), R(0)), q));
+  }
+  FN(oracle_instrument_update)(nEnv, xf, height, tips, bases);
+}
+/* setRotationCenterTransformKernel (modelDO.py:156-173), then updateLaparoscopeKernel */
+void FN(oracle_instrument_set_rcm)(int nEnv, REAL* xf, const REAL* height, const REAL* t, const REAL* r,
+                                   REAL* tips, REAL* bases) {
+  for (int env = 0; env < nEnv; ++env) {
+    A4 X = FN(m4_load)(xf, 7 * env + 3);
+    A4 m = FN(m4_transformation)(t[3 * env], t[3 * env + 1], t[3 * env + 2], r[3 * env], r[3 * env + 1], r[3 * env + 2]);
+    FN(transform_func)(xf, 7 * env + 3, X, FN(m4_mul)(FN(m4_inv)(X), m));
+  }
+  FN(oracle_instrument_update)(nEnv, xf, height, tips, bases);
+}
+/* 4x4 helpers exported for the tests */
+void FN(oracle_m4_inverse)(const REAL* m, REAL* out) {
+  A4 x;
+  for (int i = 0; i < 16; ++i) x.a[i] = m[i];
+  A4 r = FN(m4_inv)(x);
+  for (int i = 0; i < 16; ++i) out[i] = r.a[i];
+}
+void FN(oracle_m4_transformation)(REAL sx, REAL sy, REAL sz, REAL ax, REAL ay, REAL az, REAL* out) {
+  A4 r = FN(m4_transformation)(sx, sy, sz, ax, ay, az);
+  for (int i = 0; i < 16; ++i) out[i] = r.a[i];
+}
+
+#undef A4
 #undef V3
 #undef FEPS
 #undef R

@@ -33,7 +33,7 @@ def test_every_declared_symbol_is_exported(pkg):
 
 
 def test_abi_version(pkg):
-    assert pkg.lib.ffsrl_abi_version() == 7
+    assert pkg.lib.ffsrl_abi_version() == 8
     assert pkg.lib.ffsrl_frame_max_vertices() >= 10000
 
 
@@ -48,6 +48,7 @@ int main(void) {
   printf("ffsrl_params size %zu\n", sizeof(ffsrl_params));
   printf("ffsrl_camera size %zu\n", sizeof(ffsrl_camera));
   printf("ffsrl_requests size %zu\n", sizeof(ffsrl_requests));
+  printf("ffsrl_instrument size %zu\n", sizeof(ffsrl_instrument));
   %FIELDS%
   return 0;
 }
@@ -57,7 +58,7 @@ int main(void) {
 def test_struct_layouts_match_header(pkg, tmp_path):
     L = pkg._lib
     structs = {"ffsrl_batch": L.Batch, "ffsrl_params": L.Params, "ffsrl_camera": L.Camera,
-               "ffsrl_env_desc": L.EnvDesc, "ffsrl_requests": L.Requests}
+               "ffsrl_env_desc": L.EnvDesc, "ffsrl_requests": L.Requests, "ffsrl_instrument": L.Instrument}
     fields = "\n".join(f"F({n}, {f[0]})" for n, S in structs.items() for f in S._fields_)
     src = tmp_path / "layout.c"
     src.write_text(_LAYOUT_C.replace("%FIELDS%", fields))
