@@ -19,7 +19,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import Tuple
+from typing import Optional, Tuple
 
 import numpy as np
 
@@ -61,6 +61,20 @@ class Laparoscope:
     jaw_height: float = 0.06
     jaw_radius: float = 0.008
     jaw_angle: float = 60.0
+    # a scene's own transforms and per-jaw capsules (from_transforms)
+    xf: Optional[np.ndarray] = dataclasses.field(default=None, repr=False)
+    left_jaw: Optional[Tuple[float, float]] = None     # (height, radius)
+    right_jaw: Optional[Tuple[float, float]] = None
+
+    @classmethod
+    def from_transforms(cls, xf, rod_height, rod_radius, left_height, left_radius, right_height,
+                        right_radius) -> "Laparoscope":
+        """An instrument given by its seven local-to-world transforms (the
+        order above, column-vector form) and capsule sizes, as a USD scene
+        defines it."""
+        xf = np.asarray(xf, np.float64).reshape(7, 4, 4)
+        return cls(rod_height=rod_height, rod_radius=rod_radius, jaw_height=right_height, jaw_radius=right_radius,
+                   xf=xf.copy(), left_jaw=(left_height, left_radius), right_jaw=(right_height, right_radius))
 
     @classmethod
     def aimed_at(cls, point, rcm_height: float = 1.0, **kw) -> "Laparoscope":
@@ -76,6 +90,8 @@ class Laparoscope:
 
     def transforms(self) -> np.ndarray:
         """(7, 4, 4) local-to-world matrices, column-vector convention, float64."""
+        if self.xf is not None:
+            return self.xf.copy()
         rc = transformation(*self.rcm, *self.rcm_rotation)
         rod = rc @ _shift(0.0, 0.0, self.insertion)
         end = rod @ _shift(0.0, 0.0, self.rod_height / 2.0)
@@ -91,15 +107,17 @@ class Laparoscope:
         = twice the capsule radii (0 for the drag point) and heights (4,)."""
         x = self.transforms()
         rod, left, right = x[6], x[1], x[2]
-        hr, hj = self.rod_height, self.jaw_height
+        hr = self.rod_height
+        hl, rl = self.left_jaw or (self.jaw_height, self.jaw_radius)
+        hrj, rr = self.right_jaw or (self.jaw_height, self.jaw_radius)
         rod_base, rod_tip = _point(rod, (0, 0, -hr / 2.0)), _point(rod, (0, 0, hr / 2.0))
-        lb, lt = _point(left, (0, 0, -hj / 1.9)), _point(left, (0, 0, hj / 1.9))
-        rb, rt = _point(right, (0, 0, -hj / 1.9)), _point(right, (0, 0, hj / 1.9))
-        drag = _point(rod, (0, 0, hr / 2.0 + 1.2 * hj))
+        lb, lt = _point(left, (0, 0, -hl / 1.9)), _point(left, (0, 0, hl / 1.9))
+        rb, rt = _point(right, (0, 0, -hrj / 1.9)), _point(right, (0, 0, hrj / 1.9))
+        drag = _point(rod, (0, 0, hr / 2.0 + 1.2 * hrj))
         tips = np.stack([rod_tip, lt, rt, drag])
         bases = np.stack([rod_base, lb, rb, rod_base])
-        radii = np.array([2 * self.rod_radius, 2 * self.jaw_radius, 2 * self.jaw_radius, 0.0])
-        heights = np.array([hr, hj, hj, 0.0])
+        radii = np.array([2 * self.rod_radius, 2 * rl, 2 * rr, 0.0])
+        heights = np.array([hr, hl, hrj, 0.0])
         return tips, bases, radii, heights
 
     def stored_transforms(self) -> np.ndarray:

@@ -236,6 +236,19 @@ class SimModelDO:
             self._build_instrument(laps)
         self._order = None
 
+    @classmethod
+    def from_usd(cls, path: str, numEnvs: int, device="cuda:0", **kw) -> "SimModelDO":
+        """``SimModelDO(stage, numEnvs, device, ...)`` of the reference
+        (``modelDO.py:774-1266``) for a USD crate scene or its converted
+        ``.npz``: every env holds the scene's simulated meshes (merged into one
+        mesh, lock-box pins applied) and its laparoscope."""
+        from .usd_scene import merged_mesh, scene
+        sc = scene(path)
+        laps = None if sc.laparoscope is None else [sc.laparoscope] * int(numEnvs)
+        model = cls([merged_mesh(sc)], [0] * int(numEnvs), device=device, laparoscopes=laps, **kw)
+        model.scene = sc
+        return model
+
     # ------------------------------------------------------------------ setup
     def _make_params(self) -> Params:
         p = Params()

@@ -1,0 +1,139 @@
+"""Scene ingestion: the USD crate reader and the scene loader.
+
+The reader is pinned on the reference's own liver scene
+(FF_SRL/FF_SRL/scenes/liverRetraction.usd, when the reference checkout is
+next to this one): the decoded vertices, tets, edges and rest data must
+satisfy the geometry they describe (rest volume = |det| / 6, rest length =
+edge length, inverse rest matrix = Dm^-1), which a wrong decoding of any
+compressed table or value would break.  The committed fixture
+(tests/golden/liver_retraction.npz, from tests/golden/make_liver_fixture.py)
+must equal a fresh load.  The LZ4 and integer decoders are also checked on
+hand-made inputs.
+"""
+import importlib
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+pkg = importlib.import_module("ff-srl_amd")
+usdc = importlib.import_module("ff-srl_amd.usdc")
+us = importlib.import_module("ff-srl_amd.usd_scene")
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "liver_retraction.npz")
+SRC = "/root/reference/FF_SRL/FF_SRL/scenes/liverRetraction.usd"
+need_src = pytest.mark.skipif(not os.path.exists(SRC), reason="reference checkout not present")
+
+
+def test_lz4_block_with_overlapping_match():
+    # "abc", then a 9-byte match at offset 3 (overlapping), then the literals "xy"
+    block = bytes([0x35]) + b"abc" + bytes([3, 0]) + bytes([0x20]) + b"xy"
+    assert usdc.decompress(b"\x00" + block) == b"abcabcabcabcxy"
+    # two chunks: chunk count, then (int32 size, block) per chunk
+    two = b"\x02" + struct.pack("<i", len(block)) + block + struct.pack("<i", 2) + bytes([0x10]) + b"z"
+    assert usdc.decompress(two) == b"abcabcabcabcxyz"
+
+
+def test_integer_coding():
+    # common delta 2; codes: common, 1-byte, 2-byte, 4-byte, common
+    deltas = [2, -9, 300, -70000, 2]
+    vals = list(np.cumsum(deltas))
+    codes = [0, 1, 2, 3, 0]
+    code_bytes = bytearray((5 * 2 + 7) // 8)
+    for i, c in enumerate(codes):
+        code_bytes[i >> 2] |= c << (2 * (i & 3))
+    data = struct.pack("<i", 2) + bytes(code_bytes) + struct.pack("<b", -9) + struct.pack("<h", 300) + \
+        struct.pack("<i", -70000)
+    assert usdc.decode_ints(data, 5).tolist() == vals
+
+
+def _check_geometry(m):
+    V = m.verts.astype(np.float64)
+    A, B, C, D = (V[m.tets[:, i]] for i in range(4))
+    det = np.einsum("ij,ij->i", np.cross(B - A, C - A), D - A)
+    # float32 world coordinates (|x| ~ 10) round the smallest tets' volumes by
+    # up to ~2e-4 relative; a wrong decoding is off by O(1)
+    assert np.allclose(np.abs(det) / 6.0, m.tet_rest_volume, rtol=1e-4, atol=2e-5)
+    assert np.allclose(np.linalg.norm(V[m.edges[:, 1]] - V[m.edges[:, 0]], axis=1), m.edge_rest, rtol=1e-4,
+                       atol=2e-5)
+    # render triangles are exactly the boundary faces of the tets
+    faces = np.concatenate([m.tets[:, [1, 2, 3]], m.tets[:, [0, 3, 2]], m.tets[:, [0, 1, 3]], m.tets[:, [0, 2, 1]]])
+    u, cnt = np.unique(np.sort(faces, axis=1), axis=0, return_counts=True)
+    assert set(map(tuple, u[cnt == 1])) == set(map(tuple, np.sort(m.tris, axis=1)))
+    # each is wound outward (away from the fourth vertex of its tet)
+    owner = {}
+    for t in m.tets:
+        for f in ((1, 2, 3), (0, 3, 2), (0, 1, 3), (0, 2, 1)):
+            owner[tuple(sorted(t[list(f)]))] = int(t[[k for k in range(4) if k not in f][0]])
+    n = np.cross(V[m.tris[:, 1]] - V[m.tris[:, 0]], V[m.tris[:, 2]] - V[m.tris[:, 0]])
+    opp = np.array([owner[tuple(sorted(t))] for t in m.tris])
+    assert (np.einsum("ij,ij->i", n, V[opp] - V[m.tris[:, 0]]) < 0).all()
+    return det
+
+
+@need_src
+def test_reference_liver_scene_decodes_consistently():
+    crate = usdc.CrateFile(SRC)
+    assert crate.version >= (0, 4, 0)
+    assert "/World/Tissue/Scene_01" in crate.prim_paths()
+    p = "/World/Tissue/Scene_01"
+    local = crate.get(p, "extMesh:vertex").astype(np.float64)
+    T = crate.get(p, "extMesh:elem")
+    A, B, C, D = (local[T[:, i]] for i in range(4))
+    Dm = np.stack([B - A, C - A, D - A], axis=2)
+    M = crate.get(p, "extMesh:inverseRestPosition").reshape(-1, 3, 3).astype(np.float64)
+    # inverse rest matrices as stored (float32, in the mesh's local frame)
+    rel = np.abs(M - np.linalg.inv(Dm)).max() / np.abs(M).max()
+    assert rel < 1e-3
+    assert crate.fields_of("/")["upAxis"] == "Y"
+    sc = us.load_usd_scene(SRC)
+    assert len(sc.meshes) == 1
+    det = _check_geometry(sc.meshes[0])
+    assert (det < 0).all()                 # every tet of this scene is negatively oriented
+    # the committed fixture is this scene
+    fx = us.load_scene(FIXTURE)
+    for a in us._MESH_ARRAYS:
+        assert np.array_equal(getattr(fx.meshes[0], a), getattr(sc.meshes[0], a)), a
+    assert np.array_equal(fx.laparoscope.transforms(), sc.laparoscope.transforms())
+    assert fx.pinned == sc.pinned and fx.params == sc.params
+
+
+def test_liver_fixture_scene():
+    sc = us.load_scene(FIXTURE)
+    m = sc.meshes[0]
+    assert (m.num_vertices, m.num_tets, m.num_edges, m.num_tris) == (545, 1717, 2757, 992)
+    _check_geometry(m)
+    # the lock box pins 491 vertices; the liver rests on the ground inside the
+    # workspace of the reference's retraction test (testLiverRetraction.py:29-30)
+    assert sc.pinned == [491] and int((m.inv_mass == 0).sum()) == 491
+    lo, hi = m.verts.min(axis=0), m.verts.max(axis=0)
+    assert lo[1] > 0.0 and hi[1] < 15.0 and np.all(np.abs(lo[[0, 2]]) < 12) and np.all(np.abs(hi[[0, 2]]) < 12)
+    assert sc.meters_per_unit == 0.01 and sc.up_axis == "Y"
+    assert sc.params[0]["ksDistance"] == 1.0 and sc.params[0]["density"] == 1000.0
+    # the laparoscope: the rod's axis passes through its rotation centre, the
+    # jaw opening is inside the clamp range, capsule sizes from the Capsule prims
+    lap = sc.laparoscope
+    x = lap.transforms()
+    rel = x[3][:3, 3] - x[6][:3, 3]
+    assert np.linalg.norm(np.cross(x[6][:3, 2], rel)) < 1e-9 * np.linalg.norm(rel)
+    jaw = math.degrees(math.asin((np.linalg.inv(x[0]) @ x[5])[2, 1]))
+    assert 40.0 < jaw < 85.0
+    tips, bases, radii, heights = lap.initial_points()
+    assert radii.tolist() == pytest.approx([0.8, 0.2, 0.2, 0.0])
+    assert heights.tolist() == pytest.approx([30.0, 0.8, 0.8, 0.0])
+    # merged mesh of a one-mesh scene is the mesh itself
+    assert us.merged_mesh(sc) is m
+
+
+def test_merged_mesh_offsets_ids():
+    sc = us.load_scene(FIXTURE)
+    two = us.UsdScene(meshes=[sc.meshes[0], sc.meshes[0]], params=sc.params * 2, laparoscope=None,
+                      meters_per_unit=1.0, up_axis="Y", pinned=[0, 0])
+    m = us.merged_mesh(two)
+    n = sc.meshes[0].num_vertices
+    assert m.num_vertices == 2 * n and m.num_tets == 2 * sc.meshes[0].num_tets
+    assert np.array_equal(m.tets[sc.meshes[0].num_tets:], sc.meshes[0].tets + n)
+    assert np.array_equal(m.tris[sc.meshes[0].num_tris:], sc.meshes[0].tris + n)
