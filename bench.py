@@ -39,7 +39,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "ellipsoids", "cube", "large", "serve"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "ellipsoids", "cube", "large", "serve", "liver"])
+    ap.add_argument("--scene", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                                     "liver_retraction.npz"),
+                    help="USD crate scene or converted .npz for --workload liver")
+    ap.add_argument("--resolution", type=int, default=None, help="render resolution override")
     ap.add_argument("--envs", type=int, default=None)
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--no-autotune", action="store_true",
@@ -73,10 +77,14 @@ def make_workload(wl, args):
         w = wl.ellipsoids(args.envs or 64, seed=args.seed)
     elif args.workload == "large":
         w = wl.large(args.envs or 8192, seed=args.seed)
+    elif args.workload == "liver":
+        w = wl.liver(args.scene, args.envs or 1024)
     else:
         w = wl.cube(args.seed)
+    if args.resolution:
+        w.resolution = args.resolution
     if args.collision:
-        if w.caps_base is None:
+        if w.caps_base is None and w.laparoscopes is None:
             raise SystemExit(f"workload {w.name} has no instrument capsules")
         w.model_kwargs = dict(w.model_kwargs, collision=True, globalKsContact=1.0)
         w.name += "+collision"
@@ -125,15 +133,17 @@ def cpu_baseline(pkg, orc, w, model, ren, seconds, seed, threads):
     order = list(rng.permutation(w.num_envs))
     kw = w.model_kwargs
     cfg = orc.StepConfig(substeps=kw["simSubsteps"], iterations=kw["simConstraintsSteps"],
-                         fps=kw["simFrameRate"], gravity=kw["environmentGravity"],
-                         damping=kw["environmentVelocityDampening"],
+                         fps=kw["simFrameRate"], gravity=kw.get("environmentGravity", (0.0, -9.8, 0.0)),
+                         damping=kw.get("environmentVelocityDampening", 0.1),
                          dist_mode={None: 0, "pbd": 1, "xpbd": 2}[kw.get("distance")],
                          vol_mode={None: 0, "pbd": 1, "xpbd": 2}[kw.get("volume")],
                          nh_on=int(kw.get("neoHookean", False)),
                          nh_vol_c=kw.get("globalVolumeCompliance", 1.0),
                          nh_dev_c=kw.get("globalDeviatoricCompliance", 1.0),
-                         dist_compliance=kw.get("distanceCompliance", 0.0),
-                         vol_compliance=kw.get("volumeComplianceXPBD", 0.0), drag_ks=kw.get("globalKsDrag", 1.0),
+                         dist_k=kw.get("globalKsDistance", 1.0), vol_k=kw.get("globalKsVolume", 1.0),
+                         dist_compliance=kw.get("distanceCompliance", 0.0) if kw.get("distance") == "xpbd" else None,
+                         vol_compliance=kw.get("volumeComplianceXPBD", 0.0) if kw.get("volume") == "xpbd" else None,
+                         drag_ks=kw.get("globalKsDrag", 1.0),
                          collide=int(kw.get("collision", False)), contact_k=kw.get("globalKsContact", 1.0))
     dirs = ren.dirs.cpu().numpy()
     origin = np.asarray(ren.camera.origin[:], np.float32)
@@ -279,8 +289,16 @@ def main():
             lib.ffsrl_render(model.batch, cam_rgb, eb, ee, None, ren.pixels[eb:].data_ptr(), model.normals.data_ptr(),
                              model.vertexColors.data_ptr(), model.t_tri_colors.data_ptr(), None, None, sh)
 
+    def move(f, sh):
+        """The tool random walk: instrument Cartesian actions for a model
+        with laparoscopes, else a direct move of the instrument points."""
+        if w.laparoscopes is not None:
+            lib.ffsrl_instrument_cartesian(model.batch, model.instrument, walk[f].data_ptr(), None, None, sh)
+        else:
+            lib.ffsrl_move_tools(model.batch, walk[f].data_ptr(), None, None, sh)
+
     def step(f, sh):
-        lib.ffsrl_move_tools(model.batch, walk[f].data_ptr(), None, None, sh)
+        move(f, sh)
         integ.stepModel(model, refit=True)       # on torch's current stream, which is sh
         render(sh)
 
@@ -303,8 +321,9 @@ def main():
         step(f, s)
     torch.cuda.synchronize()
     # every timed form (eager, graph, plain) starts from this state
-    state = [t for t in (model.x, model.p, model.v, model.drag, model.laparoscopeTip, model.bvh_lo, model.bvh_hi,
-                         model.lam_t, model.lam_e) if t is not None]
+    state = [t for t in (model.x, model.p, model.v, model.drag, model.laparoscopeTip, model.laparoscopeBase,
+                         model.bvh_lo, model.bvh_hi, model.lam_t, model.lam_e,
+                         getattr(model, "laparoscopeXForm", None)) if t is not None]
     snapshot = [(t, t.clone()) for t in state]
 
     def restore():
@@ -320,7 +339,7 @@ def main():
         f = args.warmup + i
         e0, e1, e2, e3 = ev[i]
         e0.record(stream)
-        lib.ffsrl_move_tools(model.batch, walk[f].data_ptr(), None, None, s)
+        move(f, s)
         e1.record(stream)
         integ.stepModel(model, refit=True)
         e2.record(stream)
@@ -339,7 +358,7 @@ def main():
         "metric": "env-steps/s @1024 mixed-size envs + p50/p99 step latency, batched vs plain",
         "value": value, "unit": "env-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded meshes, seeded tool random walk)",
+        "vs_baseline": None, "dtype": "f32", "data": w.data,
         "config": {"workload": w.name, "envs": n, **{k: int(v) for k, v in w.totals().items()},
                    "substeps": w.model_kwargs["simSubsteps"], "iterations": w.model_kwargs["simConstraintsSteps"],
                    "render": f"{w.resolution}x{w.resolution} depth{' + RGB (per-triangle colours)' if w.rgb else ''} "
@@ -383,7 +402,7 @@ def main():
         model.enable_plain()
 
         def plain_step(f):
-            lib.ffsrl_move_tools(model.batch, walk[f % total_frames].data_ptr(), None, None, s)
+            move(f % total_frames, s)
             integ.stepModelPlain(model)
             lib.ffsrl_bvh_refit_plain(model.batch, 0, n, s)
             for e in range(n):

@@ -49,6 +49,9 @@ class Workload:
     rgb: bool = False
     caps_base: Optional[np.ndarray] = None     # (n_env, 4, 3) instrument capsule bases
     caps_radius: Optional[np.ndarray] = None   # (n_env, 4) capsule radii
+    laparoscopes: Optional[list] = None        # instrument kinematics per env (replaces tips / capsules)
+    data: str = "synthetic (seeded meshes, seeded tool random walk)"
+    setup: Optional[object] = None             # callable(model) run once after the model is built
 
     @property
     def num_envs(self):
@@ -146,6 +149,39 @@ def mixed(n_env: int = 1024, lo: int = 1000, hi: int = 40000, seed: int = 0, sub
                     (0.0, 1.6, 5.5), (0.0, 0.45, 0.0), tips=np.stack(tips), grasp=grasp, caps_base=cb, caps_radius=cr)
 
 
+# SimModelDO arguments and set-up of the reference's liver retraction test
+# (tests/testLiverRetraction.py:29-58); its DO step runs the PBD forms.
+LIVER_KW = dict(simFrameRate=30, simSubsteps=8, simConstraintsSteps=1, globalKsDrag=1.0,
+                globalLaparoscopeDragLookupRadius=1.0, globalKsDistance=1.0, globalKsVolume=1.0, distance="pbd",
+                volume="pbd", environmentGravity=(0.0, -9.8, 0.0), workspaceLow=(-7.0, 0.7, -10.6),
+                workspaceHigh=(7.0, 9.0, 7.4))
+LIVER_RCM, LIVER_RCM_ROT, LIVER_EFFECTOR = (9.036183, 35.260103, 1.567807), (0.0, 0.0, 1.0), (0.0, 10.0, 4.0)
+
+
+def liver(path: str, n_env: int = 1024, resolution: int = 64) -> Workload:
+    """n_env copies of a USD scene (the reference's liver retraction scene, or
+    its converted .npz), with the instrument placed as in the reference's
+    test: rotation centre, then the drag point moved to (0, 10, 4).  The tool
+    random walk drives the instrument through Cartesian actions (0.05 per
+    frame)."""
+    from . import usd_scene
+    sc = usd_scene.scene(path)
+    mesh = usd_scene.merged_mesh(sc)
+
+    def setup(model):
+        import torch
+        n = model.numEnvs
+        dev = model.device
+        model.setRotationCenterInitialPosition(torch.tensor([LIVER_RCM] * n, device=dev),
+                                               torch.tensor([LIVER_RCM_ROT] * n, device=dev))
+        model.setEffectorInitialPosition(torch.tensor(LIVER_EFFECTOR, device=dev) - model.getLaparoscopePositionsTensor())
+
+    name = f"liver-retraction-{n_env}env-{mesh.num_tets}tets"
+    return Workload(name, [mesh], [0] * n_env, dict(LIVER_KW), resolution, (0.0, 50.0, 50.0), (0.0, 10.0, 0.0),
+                    laparoscopes=[sc.laparoscope] * n_env, walk_sigma=0.05, setup=setup,
+                    data=f"the reference's scene ({os.path.basename(path)}), seeded Cartesian tool random walk")
+
+
 def large(n_env: int = 8192, tets: int = 20000, seed: int = 0) -> Workload:
     w = ellipsoids(n_env, tets, seed, substeps=20, iterations=2, resolution=128, volume=True, shared=True)
     w.name = f"ellipsoid-{n_env}env-{w.meshes[0].num_tets}tets-S20I2"
@@ -180,7 +216,10 @@ def serving(actors: int = 4096, rate: float = 20000.0, tets=(2000, 8000, 20000, 
 def build_model(pkg, w: Workload, device="cuda:0"):
     """SimModelDO for a workload, with the grasp weights set."""
     model = pkg.SimModelDO(w.meshes, w.env_mesh, device=device, tips=w.tips, initialVelocity=w.initial_velocity,
-                           capsuleBases=w.caps_base, capsuleRadii=w.caps_radius, **w.model_kwargs)
+                           capsuleBases=w.caps_base, capsuleRadii=w.caps_radius, laparoscopes=w.laparoscopes,
+                           **w.model_kwargs)
+    if w.setup is not None:
+        w.setup(model)
     if w.grasp is not None:
         import torch
         ids = [np.asarray(g, np.int64) + int(model.env_vbase[i]) for i, g in enumerate(w.grasp) if g is not None]
