@@ -77,6 +77,13 @@ class Instrument(C.Structure):
     ]
 
 
+class BvhJob(C.Structure):
+    _fields_ = [
+        ("verts", _vp), ("faces", _vp), ("nv", C.c_int32), ("nf", C.c_int32), ("left", _vp), ("first", _vp),
+        ("count", _vp), ("level", _vp), ("tri_map", _vp), ("info", _vp), ("scratch", _vp),
+    ]
+
+
 REQ_STEP, REQ_RESET = 0, 1
 
 _i32, _f32, _i64 = C.c_int32, C.c_float, C.c_int64
@@ -105,6 +112,7 @@ SIGNATURES = {
     "ffsrl_instrument_cartesian": [_P(Batch), _P(Instrument), _vp, _vp, _vp, _vp],
     "ffsrl_instrument_set_rcm": [_P(Batch), _P(Instrument), _vp, _vp, _vp],
     "ffsrl_instrument_update": [_P(Batch), _P(Instrument), _vp],
+    "ffsrl_bvh_build_device": [_vp, _i32, _i32, _vp],
     "ffsrl_bounds_integrate": [_P(Batch), _P(Params), _i32, _i32, _vp],
     "ffsrl_zero_lambdas": [_P(Batch), _i32, _i32, _vp],
     "ffsrl_bvh_refit_plain": [_P(Batch), _i32, _i32, _vp],
@@ -122,6 +130,7 @@ PLAIN_SIGNATURES = {
     "ffsrl_abi_version": ([], C.c_int),
     "ffsrl_last_error": ([], C.c_char_p),
     "ffsrl_frame_max_vertices": ([], C.c_int),
+    "ffsrl_bvh_scratch_bytes": ([_i32], C.c_int64),
 }
 
 

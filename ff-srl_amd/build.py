@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libffsrl.so")
-SOURCES = ["physics.hip", "bvh.hip", "render.hip", "meshprep.hip", "serve.hip", "instrument.hip"]
+SOURCES = ["physics.hip", "bvh.hip", "render.hip", "meshprep.hip", "serve.hip", "instrument.hip", "bvh_build.hip"]
 ARCH = os.environ.get("FFSRL_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

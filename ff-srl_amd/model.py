@@ -25,7 +25,7 @@ from typing import Optional, Sequence
 import numpy as np
 import torch
 
-from ._lib import Batch, EnvDesc, ENV_DESC_FIELDS, Instrument, OFF, PBD, XPBD, Params, lib
+from ._lib import Batch, BvhJob, EnvDesc, ENV_DESC_FIELDS, Instrument, OFF, PBD, XPBD, Params, lib
 from .scenes import TetMesh
 
 MAX_STACK = 32  # traversal stack entries per ray (render.py:424, 924)
@@ -63,29 +63,83 @@ def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
     return None if t is None else t.data_ptr()
 
 
+def surface_inputs(mesh: TetMesh):
+    """The BVH build's inputs: the surface vertices (the render points) and the
+    render faces in their numbering."""
+    surf = mesh.surface_vertices()
+    local = np.full(mesh.num_vertices, -1, dtype=np.int32)
+    local[surf] = np.arange(surf.size, dtype=np.int32)
+    faces_s = np.ascontiguousarray(local[mesh.tris]).astype(np.int32)
+    verts_s = np.ascontiguousarray(mesh.verts[surf]).astype(np.float32)
+    return verts_s, faces_s
+
+
+def build_bvh_host(mesh: TetMesh, bins: int = 8) -> dict:
+    """SAH topology of one mesh on the host (``ffsrl_bvh_build``)."""
+    verts_s, faces_s = surface_inputs(mesh)
+    nf = mesh.num_tris
+    out = {k: np.empty(2 * nf, np.int32) for k in ("left", "first", "count", "level")}
+    out["tri_map"] = np.empty(nf, np.int32)
+    maxl, nn = C.c_int32(0), C.c_int32(0)
+    lib.ffsrl_bvh_build(verts_s.ctypes.data, verts_s.shape[0], faces_s.ctypes.data, nf, bins,
+                        out["left"].ctypes.data, out["first"].ctypes.data, out["count"].ctypes.data,
+                        out["level"].ctypes.data, out["tri_map"].ctypes.data, C.byref(nn), C.byref(maxl))
+    out["num_nodes"], out["max_level"] = int(nn.value), int(maxl.value)
+    return out
+
+
+def build_bvhs_device(meshes: Sequence[TetMesh], device="cuda:0", bins: int = 8) -> list:
+    """SAH topologies of many meshes in one launch (``ffsrl_bvh_build_device``),
+    node for node the host build's.  Returns one dict per mesh with the arrays
+    MeshTopology takes."""
+    dev = torch.device(device)
+    inputs = [surface_inputs(m) for m in meshes]
+    sizes = [(v.shape[0], f.shape[0]) for v, f in inputs]
+    align = lambda x: (x + 255) // 256 * 256
+    scratch_sz = [align(int(lib.ffsrl_bvh_scratch_bytes(nf))) for _, nf in sizes]
+    scratch = torch.empty(max(sum(scratch_sz), 1), dtype=torch.uint8, device=dev)
+    verts = [torch.from_numpy(v.reshape(-1)).to(dev) for v, _ in inputs]
+    faces = [torch.from_numpy(f.reshape(-1)).to(dev) for _, f in inputs]
+    outs = [torch.empty(8 * nf + nf + 2, dtype=torch.int32, device=dev) for _, nf in sizes]
+    jobs = (BvhJob * len(meshes))()
+    off = 0
+    for i, ((nv, nf), o) in enumerate(zip(sizes, outs)):
+        j = jobs[i]
+        j.verts, j.faces, j.nv, j.nf = verts[i].data_ptr(), faces[i].data_ptr(), nv, nf
+        base = o.data_ptr()
+        j.left, j.first, j.count, j.level = base, base + 8 * nf, base + 16 * nf, base + 24 * nf
+        j.tri_map, j.info = base + 32 * nf, base + 36 * nf
+        j.scratch = scratch.data_ptr() + off
+        off += scratch_sz[i]
+    jobs_dev = torch.frombuffer(bytearray(jobs), dtype=torch.uint8).to(dev)
+    lib.ffsrl_bvh_build_device(jobs_dev.data_ptr(), len(meshes), int(bins), torch.cuda.current_stream().cuda_stream)
+    res = []
+    for (nv, nf), o in zip(sizes, outs):
+        a = o.cpu().numpy()
+        n, max_level = int(a[9 * nf]), int(a[9 * nf + 1])
+        if n < 0:
+            raise ValueError("device BVH build: tree deeper than 64 levels")
+        res.append({"left": a[:2 * nf], "first": a[2 * nf:4 * nf], "count": a[4 * nf:6 * nf],
+                    "level": a[6 * nf:8 * nf], "tri_map": a[8 * nf:9 * nf].copy(), "num_nodes": n,
+                    "max_level": max_level})
+    return res
+
+
 class MeshTopology:
     """Host-side topology of one mesh plus its BVH (built once)."""
 
-    def __init__(self, mesh: TetMesh, bins: int = 8, constraint_order: str = "local", chunk: int = LOCAL_CHUNK):
+    def __init__(self, mesh: TetMesh, bins: int = 8, constraint_order: str = "local", chunk: int = LOCAL_CHUNK,
+                 bvh: Optional[dict] = None):
+        """`bvh`: the SAH topology already built (``build_bvhs_device``);
+        otherwise it is built here on the host (``ffsrl_bvh_build``)."""
         self.mesh = mesh
-        surf = mesh.surface_vertices()
-        local = np.full(mesh.num_vertices, -1, dtype=np.int32)
-        local[surf] = np.arange(surf.size, dtype=np.int32)
-        faces_s = np.ascontiguousarray(local[mesh.tris]).astype(np.int32)
-        verts_s = np.ascontiguousarray(mesh.verts[surf]).astype(np.float32)
         nf = mesh.num_tris
-        left = np.empty(2 * nf, np.int32)
-        first = np.empty(2 * nf, np.int32)
-        count = np.empty(2 * nf, np.int32)
-        level = np.empty(2 * nf, np.int32)
-        tri_map = np.empty(nf, np.int32)
-        maxl, nn = C.c_int32(0), C.c_int32(0)
-        lib.ffsrl_bvh_build(verts_s.ctypes.data, verts_s.shape[0], faces_s.ctypes.data, nf, bins,
-                            left.ctypes.data, first.ctypes.data, count.ctypes.data, level.ctypes.data,
-                            tri_map.ctypes.data, C.byref(nn), C.byref(maxl))
-        n = int(nn.value)
+        if bvh is None:
+            bvh = build_bvh_host(mesh, bins)
+        left, first, count, level, tri_map = (bvh[k] for k in ("left", "first", "count", "level", "tri_map"))
+        n, max_level = int(bvh["num_nodes"]), int(bvh["max_level"])
         self.num_nodes = n
-        self.max_level = int(maxl.value)
+        self.max_level = max_level
         if self.max_level + 1 > MAX_STACK:
             raise ValueError(f"BVH of {mesh.name} has {self.max_level + 1} levels; the ray stack holds {MAX_STACK}")
         self.left = left[:n].copy()
@@ -185,7 +239,8 @@ class SimModelDO:
                  initialVelocity: Optional[np.ndarray] = None, tips: Optional[np.ndarray] = None,
                  constraint_order: str = "local", globalKsContact=1.0, collision: bool = False,
                  capsuleBases: Optional[np.ndarray] = None, capsuleRadii: Optional[np.ndarray] = None,
-                 laparoscopes=None, workspaceLow=(-1e5, -1e5, -1e5), workspaceHigh=(1e5, 1e5, 1e5)):
+                 laparoscopes=None, workspaceLow=(-1e5, -1e5, -1e5), workspaceHigh=(1e5, 1e5, 1e5),
+                 bvh_on_device: bool = False):
         self.device = torch.device(device)
         self.meshes = list(meshes)
         env_mesh = list(range(len(self.meshes))) if env_mesh is None else list(env_mesh)
@@ -226,7 +281,12 @@ class SimModelDO:
             capsuleRadii = np.stack([p[2] for p in pts]).astype(np.float32)
 
         self.constraint_order = constraint_order
-        self.topo = _parallel(lambda m: MeshTopology(m, bins, constraint_order), self.meshes)
+        if bvh_on_device:   # every mesh's SAH topology in one device launch
+            bvhs = build_bvhs_device(self.meshes, self.device, bins)
+            self.topo = _parallel(lambda mb: MeshTopology(mb[0], bins, constraint_order, bvh=mb[1]),
+                                  list(zip(self.meshes, bvhs)))
+        else:
+            self.topo = _parallel(lambda m: MeshTopology(m, bins, constraint_order), self.meshes)
         self._build_topology()
         self._build_envs()
         self._build_state(initialVelocity, tips, capsuleBases, capsuleRadii)

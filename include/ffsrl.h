@@ -329,6 +329,24 @@ int ffsrl_bvh_build(const float* verts, int32_t nv, const int32_t* faces, int32_
                     int32_t* left, int32_t* first, int32_t* count, int32_t* level, int32_t* tri_map,
                     int32_t* num_nodes, int32_t* max_level);
 
+/* Device build of the same topology (ffsrl_bvh_build's outputs, node for
+ * node), one workgroup per job; all pointers of a job are device memory and
+ * `jobs` itself is a device array.  info[0] = node count (-1 if the tree was
+ * deeper than 64 levels), info[1] = deepest level.  scratch holds
+ * ffsrl_bvh_scratch_bytes(nf) bytes.  Replaces SimBVH.buildBVH
+ * (bvh.py:456-518) for many meshes at once (scene admission). */
+typedef struct ffsrl_bvh_job {
+  const float* verts;             /* [nv*3] */
+  const int32_t* faces;           /* [nf*3] */
+  int32_t nv, nf;
+  int32_t *left, *first, *count, *level;  /* [2*nf] each */
+  int32_t* tri_map;               /* [nf] */
+  int32_t* info;                  /* [2] */
+  void* scratch;
+} ffsrl_bvh_job;
+int64_t ffsrl_bvh_scratch_bytes(int32_t nf);
+int ffsrl_bvh_build_device(const ffsrl_bvh_job* jobs, int32_t n_jobs, int32_t bins, void* stream);
+
 /* Greedy colouring of n constraints of `arity` vertices (host memory): two
  * constraints that share a vertex get different colours.  No reference
  * counterpart (the reference accumulates with global atomics in element

@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "ffsrl.h")
 def _declared():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ffsrl_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(ffsrl_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_every_declared_symbol_is_exported(pkg):
@@ -49,6 +49,7 @@ int main(void) {
   printf("ffsrl_camera size %zu\n", sizeof(ffsrl_camera));
   printf("ffsrl_requests size %zu\n", sizeof(ffsrl_requests));
   printf("ffsrl_instrument size %zu\n", sizeof(ffsrl_instrument));
+  printf("ffsrl_bvh_job size %zu\n", sizeof(ffsrl_bvh_job));
   %FIELDS%
   return 0;
 }
@@ -58,7 +59,8 @@ int main(void) {
 def test_struct_layouts_match_header(pkg, tmp_path):
     L = pkg._lib
     structs = {"ffsrl_batch": L.Batch, "ffsrl_params": L.Params, "ffsrl_camera": L.Camera,
-               "ffsrl_env_desc": L.EnvDesc, "ffsrl_requests": L.Requests, "ffsrl_instrument": L.Instrument}
+               "ffsrl_env_desc": L.EnvDesc, "ffsrl_requests": L.Requests, "ffsrl_instrument": L.Instrument,
+               "ffsrl_bvh_job": L.BvhJob}
     fields = "\n".join(f"F({n}, {f[0]})" for n, S in structs.items() for f in S._fields_)
     src = tmp_path / "layout.c"
     src.write_text(_LAYOUT_C.replace("%FIELDS%", fields))
