@@ -51,8 +51,10 @@ __device__ __forceinline__ bool visit_node(f3 o, f3 d, f3 inv, f3 lo, f3 hi, flo
   const float tmin = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
   const float tmax = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
   const bool finite = __builtin_isfinite((((t1 + t2) + (t3 + t4)) + (t5 + t6)));
-  const bool robust = finite && fabsf(tmax) > 1e-30f && !close_rel(tmax, tmin) && !close_rel(tmin, tMin);
-  if (robust) return (tmax >= 0.0f) && (tmax >= tmin) && (tmin < tMin);
+  // every condition is evaluated (bitwise &, no short circuit), so the whole
+  // fast path is one basic block with a single branch
+  const bool robust = finite & (fabsf(tmax) > 1e-30f) & !close_rel(tmax, tmin) & !close_rel(tmin, tMin);
+  if (robust) return (tmax >= 0.0f) & (tmax >= tmin) & (tmin < tMin);
   float tb;
   return isect_aabb(o, d, lo, hi, &tb) && tb < tMin;
 }
