@@ -282,12 +282,15 @@ def main():
     cam_rgb.mode = pkg.RenderMode.texture              # per-triangle colours, Lambert-lit
 
     def render(sh, eb=0, ee=n):
-        lib.ffsrl_render(model.batch, ren.camera, eb, ee, depth_out[eb:].data_ptr(), None, None, None, None, None,
-                         None, sh)
         if w.rgb:
+            # RGB-D: normals, then colour and depth from one traversal
             lib.ffsrl_vertex_normals(model.batch, eb, ee, model.normals.data_ptr(), sh)
-            lib.ffsrl_render(model.batch, cam_rgb, eb, ee, None, ren.pixels[eb:].data_ptr(), model.normals.data_ptr(),
-                             model.vertexColors.data_ptr(), model.t_tri_colors.data_ptr(), None, None, sh)
+            lib.ffsrl_render(model.batch, cam_rgb, eb, ee, depth_out[eb:].data_ptr(), ren.pixels[eb:].data_ptr(),
+                             model.normals.data_ptr(), model.vertexColors.data_ptr(), model.t_tri_colors.data_ptr(),
+                             None, None, sh)
+        else:
+            lib.ffsrl_render(model.batch, ren.camera, eb, ee, depth_out[eb:].data_ptr(), None, None, None, None, None,
+                             None, sh)
 
     def move(f, sh):
         """The tool random walk: instrument Cartesian actions for a model
@@ -369,7 +372,7 @@ def main():
                    **({"collision": True} if w.model_kwargs.get("collision") else {})},
         "p50_step_ms": float(np.percentile(step_ms, 50)), "p99_step_ms": float(np.percentile(step_ms, 99)),
         "frame_kernel_ms": float(frame_ms.mean()), "render_kernel_ms": float(render_ms.mean()),
-        "rays_per_s": float((2 if w.rgb else 1) * 4 * w.resolution ** 2 * n / (render_ms.mean() * 1e-3)),
+        "rays_per_s": float(4 * w.resolution ** 2 * n / (render_ms.mean() * 1e-3)),
         "finite": finite, "setup_s": setup_s,
     }
     alg = algorithmic_bytes(w, model)

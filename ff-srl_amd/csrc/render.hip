@@ -209,6 +209,16 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
       const float c01 = __shfl_down(ch[k], 1), c10 = __shfl_down(ch[k], 8), c11 = __shfl_down(ch[k], 9);
       if (owner && out.rgb) out.rgb[3 * pix + k] = (((ch[k] + c01) + c10) + c11) / 4.0f;
     }
+    if (out.depth) {
+      // RGB-D: the depth image of the same hits, computed as depth mode does
+      float dz = 0.0f;
+      if (hitp) {
+        dz = tMin / 50.0f * 1.0f;
+        if (dz > 1.0f) dz = 1.0f;
+      }
+      const float d01 = __shfl_down(dz, 1), d10 = __shfl_down(dz, 8), d11 = __shfl_down(dz, 9);
+      if (owner) out.depth[pix] = (((dz + d01) + d10) + d11) / 4.0f;
+    }
   }
 }
 
@@ -247,6 +257,8 @@ static int render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t eb, con
   const int m = cam->mode;
   if (m != 0 && m != 1 && m != 2 && m != 3 && m != 4) return fail(FFSRL_E_ARG, "render mode %d", m);
   if (m == 4 && !out.depth && !out.hit) return fail(FFSRL_E_ARG, "depth mode needs a depth (or hit) output");
+  // a colour mode also writes the depth image of the same traversal when a
+  // depth output is given (RGB-D in one pass)
   if (m != 4 && (!out.rgb || !out.normals)) return fail(FFSRL_E_ARG, "colour modes need rgb output and normals");
   if (m == 3 && !out.vcolor) return fail(FFSRL_E_ARG, "vertex colour mode needs vertex colours");
   if (m == 1 && !out.tcolor) return fail(FFSRL_E_ARG, "triangle colour mode needs triangle colours");

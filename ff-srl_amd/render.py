@@ -137,6 +137,19 @@ class RaycastRendererDO:
                          m.vertexColors.data_ptr(), m.t_tri_colors.data_ptr(), None, None, m.stream())
         return out
 
+    def render_rgbd(self, env_begin: int = 0, env_end: int = None):
+        """Colour and depth images from one traversal: ``(pixels, depth)``,
+        each equal to what ``render_rgb`` and ``render_depth`` return."""
+        m = self.simModel
+        ee = m.numEnvs if env_end is None else env_end
+        if self.render_mode == RenderMode.depth_map:
+            raise ValueError("render_rgbd needs a colour render mode")
+        lib.ffsrl_vertex_normals(m.batch, env_begin, ee, m.normals.data_ptr(), m.stream())
+        pix, dep = self.pixels[env_begin:ee], self.depth[env_begin:ee]
+        lib.ffsrl_render(m.batch, self.camera, env_begin, ee, dep.data_ptr(), pix.data_ptr(), m.normals.data_ptr(),
+                         m.vertexColors.data_ptr(), m.t_tri_colors.data_ptr(), None, None, m.stream())
+        return pix, dep
+
     def renderNew(self, simBVH=None, simModel=None) -> torch.Tensor:
         """``(numEnvs, H, W, 3)`` image of every environment."""
         if self.render_mode == RenderMode.depth_map:

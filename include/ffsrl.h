@@ -183,6 +183,8 @@ int ffsrl_bvh_refit(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, vo
 
 /* Ray-cast render of environments [env_begin, env_end) with the 2x2 box
  * filter.  depth: [n][H][W] (mode 4) ; rgb: [n][H][W][3] (other modes);
+ * a colour mode given a depth output too writes both from one traversal
+ * (RGB-D), each equal to what its own mode writes;
  * hit / thit (optional, may be NULL): per-ray face ids and t, [n][2H][2W].
  * Replaces drawSimBVHKernelManyEnvs + meshQueryRayManyEnvs + downsample_kernel
  * (render.py:276-498, 699-724) launched by renderFunctionNew (render.py:975-1022).
