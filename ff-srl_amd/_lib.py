@@ -41,7 +41,7 @@ class Batch(C.Structure):
         ("x", _vp), ("p", _vp), ("v", _vp), ("drag", _vp), ("lam_e", _vp), ("lam_t", _vp),
         ("bvh_lo", _vp), ("bvh_hi", _vp), ("tips", _vp), ("caps_base", _vp), ("caps_radius", _vp),
         ("in_collision", _vp), ("sched", _vp),
-        ("dp", _vp), ("cnt", _vp), ("host_envs", _vp), ("host_level_off", _vp),
+        ("dp", _vp), ("cnt", _vp), ("split_acc", _vp), ("split_ctr", _vp), ("host_envs", _vp), ("host_level_off", _vp),
     ]
 
 
@@ -93,6 +93,7 @@ _P = C.POINTER
 SIGNATURES = {
     "ffsrl_frame": [_P(Batch), _P(Params), _vp, _i32, _i32, _vp],
     "ffsrl_frame_envs": [_P(Batch), _P(Params), _vp, _i32, _i32, _i32, _vp],
+    "ffsrl_frame_shared": [_P(Batch), _P(Params), _vp, _i32, _i32, _i32, _vp],
     "ffsrl_render_envs": [_P(Batch), _P(Camera), _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ffsrl_apply_requests": [_P(Batch), _P(Requests), _f32, _vp],
     "ffsrl_observe_envs": [_P(Batch), _vp, _i32, _vp, _f32, _vp, _vp],
@@ -131,6 +132,7 @@ PLAIN_SIGNATURES = {
     "ffsrl_last_error": ([], C.c_char_p),
     "ffsrl_frame_max_vertices": ([], C.c_int),
     "ffsrl_bvh_scratch_bytes": ([_i32], C.c_int64),
+    "ffsrl_xcd_count": ([], C.c_int),          # count > 0, or an error code < 0
 }
 
 

@@ -69,6 +69,31 @@ __device__ inline void st4(float* base, int64_t i, float4 v) { reinterpret_cast<
 __device__ inline int4 ldi4(const int32_t* base, int64_t i) { return reinterpret_cast<const int4*>(base)[i]; }
 __device__ inline int2 ldi2(const int32_t* base, int64_t i) { return reinterpret_cast<const int2*>(base)[i]; }
 
+// Loads of state that other workgroups on the same XCD wrote during the
+// kernel: nontemporal loads skip the CU's L1, which is not coherent between
+// CUs, and read the XCD's L2, where stores land once `s_waitcnt vmcnt(0)`
+// has returned.  (An agent-scope L2 invalidate would also make them visible,
+// at the cost of the L2's contents.)
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ inline float4 ld4c(const float* base, int64_t i) {
+  const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(base) + i);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline float ldfc(const float* base, int64_t i) { return __builtin_nontemporal_load(base + i); }
+__device__ inline unsigned long long ldu64c(const unsigned long long* base, int64_t i) {
+  return __builtin_nontemporal_load(base + i);
+}
+// COH: state written by other workgroups of a shared env (ld4c), else ld4.
+template <bool COH>
+__device__ inline float4 ldp(const float* P, int64_t i) {
+  return COH ? ld4c(P, i) : ld4(P, i);
+}
+// Waits until this wavefront's vector memory operations (loads, stores and
+// atomics) have completed: s_waitcnt vmcnt(0).
+__device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// Hardware id of the XCD (XCC) the calling wavefront runs on.
+__device__ inline int xcc_id() { return (int)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20); }
+
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 }  // namespace ffsrl

@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "constraints.hpp"
 #include "refit.hpp"
@@ -130,6 +132,7 @@ struct TetG {
   float M[9];
 };
 
+template <bool COH>
 __device__ __forceinline__ TetG gather_tet(const ffsrl_batch& b, const float* P, int64_t vb, const TetIn& t,
                                            int64_t ti, int64_t T, int nh) {
   TetG g;
@@ -137,24 +140,27 @@ __device__ __forceinline__ TetG gather_tet(const ffsrl_batch& b, const float* P,
 #pragma unroll
     for (int k = 0; k < 9; ++k) g.M[k] = b.tet_inv_rest[k * T + ti];
   }
-  g.P[0] = ld4(P, vb + t.q.x);
-  g.P[1] = ld4(P, vb + t.q.y);
-  g.P[2] = ld4(P, vb + t.q.z);
-  g.P[3] = ld4(P, vb + t.q.w);
+  g.P[0] = ldp<COH>(P, vb + t.q.x);
+  g.P[1] = ldp<COH>(P, vb + t.q.y);
+  g.P[2] = ldp<COH>(P, vb + t.q.z);
+  g.P[3] = ldp<COH>(P, vb + t.q.w);
   return g;
 }
 
-// Volume + Neo-Hookean deltas of one tet, summed into LDS.
-template <bool CARRY>
+// Volume + Neo-Hookean deltas of one tet, summed into LDS.  fresh: the
+// carried lambda starts from zero (first iteration of a substep) instead of
+// being read.
+template <bool CARRY, bool COH>
 __device__ __forceinline__ void process_tet(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
                                             const FrameCtx& c, const NHConst& nhk, const TetIn& tc, const TetG& g,
-                                            int j) {
+                                            int j, bool fresh) {
   f3 d[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
   uint32_t n = 0;
   if (prm.vol_mode == FFSRL_PBD) {
     n += volume_pbd<true>(g.P, tc.rest, prm.vol_k, d);
   } else if (prm.vol_mode == FFSRL_XPBD) {
-    float lam = CARRY ? b.lam_t[e.lam_tbase + j] : 0.0f;
+    float lam = 0.0f;
+    if (CARRY && !fresh) lam = COH ? ldfc(b.lam_t, e.lam_tbase + j) : b.lam_t[e.lam_tbase + j];
     n += volume_xpbd<true>(g.P, tc.rest, prm.vol_k, &lam, d);
     if (CARRY) b.lam_t[e.lam_tbase + j] = lam;
   }
@@ -175,17 +181,20 @@ __device__ __forceinline__ void process_tet(const ffsrl_batch& b, const ffsrl_pa
 }
 
 // Distance delta of one edge, summed into LDS.
-template <bool CARRY>
+template <bool CARRY, bool COH>
 __device__ __forceinline__ void process_edge(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
-                                             const FrameCtx& c, const EdgeIn& ec, float4 A, float4 B, int j) {
+                                             const FrameCtx& c, const EdgeIn& ec, float4 A, float4 B, int j,
+                                             bool fresh) {
   f3 d;
   int ok;
   if (prm.dist_mode == FFSRL_PBD) {
     ok = distance_pbd<true>(A, B, ec.rest, prm.dist_k, &d);
   } else {
-    float lam = CARRY ? b.lam_e[e.lam_ebase + j] : 0.0f;
+    float lam = 0.0f;
+    if (CARRY && !fresh) lam = COH ? ldfc(b.lam_e, e.lam_ebase + j) : b.lam_e[e.lam_ebase + j];
     ok = distance_xpbd<true>(A, B, ec.rest, prm.dist_k, &lam, &d);
-    if (CARRY && ok) b.lam_e[e.lam_ebase + j] = lam;
+    // an early-out leaves lambda unchanged: a fresh one is stored as zero
+    if (CARRY && (ok || fresh)) b.lam_e[e.lam_ebase + j] = lam;
   }
   if (ok) {
     lds_acc<-1>(c, ec.q.x, d * A.w, 1u);
@@ -210,26 +219,29 @@ struct Slot {
 //     strides ahead and positions / inverse rest matrices one stride ahead;
 //   BLOCK == 1024 (128 VGPRs per lane): two slots; indices are loaded one
 //     stride ahead and positions at use.
-template <int BLOCK, bool CARRY>
+//
+// Items jb .. je-1 (tets below nt, edges below ne); the batched frame passes
+// the whole range, a shared env one chunk of it.
+template <int BLOCK, bool CARRY, bool COH>
 __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
                                         const FrameCtx& c, const NHConst& nhk, const float* P, int64_t vb, int64_t T,
-                                        int nt, int ne) {
-  const int nmax = nt > ne ? nt : ne;
-  const int j0 = threadIdx.x;
+                                        int nt, int ne, int jb, int je, bool fresh) {
+  const int nmax = je;
+  const int j0 = jb + threadIdx.x;
   auto ids = [&](Slot& s, int jj) {
     if (jj < nt) s.t = load_tet(b, e.tbase + jj);
     if (jj < ne) s.e = load_edge(b, e.ebase + jj);
   };
   auto gat = [&](Slot& s, int jj) {
-    if (jj < nt) s.g = gather_tet(b, P, vb, s.t, e.tbase + jj, T, prm.nh_on);
+    if (jj < nt) s.g = gather_tet<COH>(b, P, vb, s.t, e.tbase + jj, T, prm.nh_on);
     if (jj < ne) {
-      s.A = ld4(P, vb + s.e.q.x);
-      s.B = ld4(P, vb + s.e.q.y);
+      s.A = ldp<COH>(P, vb + s.e.q.x);
+      s.B = ldp<COH>(P, vb + s.e.q.y);
     }
   };
   auto run = [&](const Slot& s, int jj) {
-    if (jj < nt) process_tet<CARRY>(b, prm, e, c, nhk, s.t, s.g, jj);
-    if (jj < ne) process_edge<CARRY>(b, prm, e, c, s.e, s.A, s.B, jj);
+    if (jj < nt) process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, s.g, jj, fresh);
+    if (jj < ne) process_edge<CARRY, COH>(b, prm, e, c, s.e, s.A, s.B, jj, fresh);
   };
   if constexpr (BLOCK <= 512) {
     Slot s0, s1, s2;
@@ -253,12 +265,12 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
     // Gather and compute one item at a time to stay within 128 VGPRs.
     auto gr = [&](const Slot& s, int jj) {
       if (jj < nt) {
-        TetG g = gather_tet(b, P, vb, s.t, e.tbase + jj, T, prm.nh_on);
-        process_tet<CARRY>(b, prm, e, c, nhk, s.t, g, jj);
+        TetG g = gather_tet<COH>(b, P, vb, s.t, e.tbase + jj, T, prm.nh_on);
+        process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, g, jj, fresh);
       }
       if (jj < ne) {
-        float4 A = ld4(P, vb + s.e.q.x), B = ld4(P, vb + s.e.q.y);
-        process_edge<CARRY>(b, prm, e, c, s.e, A, B, jj);
+        float4 A = ldp<COH>(P, vb + s.e.q.x), B = ldp<COH>(P, vb + s.e.q.y);
+        process_edge<CARRY, COH>(b, prm, e, c, s.e, A, B, jj, fresh);
       }
     };
     Slot s0, s1;
@@ -278,9 +290,11 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
 // the positions at the start of the pass and summed (negated) into the LDS
 // accumulators for the vertices with inverse mass > 0; *contacts counts the
 // triangles with a response.
+// Triangles f0 .. f1-1 (the whole surface, or one chunk of a shared env).
+template <bool COH>
 __device__ __forceinline__ void collide_env(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
                                             int env, const FrameCtx& c, const float* P, int64_t vb, int* contacts,
-                                            int block) {
+                                            int block, int f0, int f1) {
   CapsuleBox box[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -289,9 +303,9 @@ __device__ __forceinline__ void collide_env(const ffsrl_batch& b, const ffsrl_pa
                          mk(b.caps_base[3 * ci], b.caps_base[3 * ci + 1], b.caps_base[3 * ci + 2]),
                          b.caps_radius[ci]);
   }
-  for (int f = threadIdx.x; f < e.nf; f += block) {
+  for (int f = f0 + (int)threadIdx.x; f < f1; f += block) {
     const int4 t = ldi4(b.tris, (int64_t)e.fbase + f);
-    const float4 A = ld4(P, vb + t.x), B = ld4(P, vb + t.y), Cc = ld4(P, vb + t.z);
+    const float4 A = ldp<COH>(P, vb + t.x), B = ldp<COH>(P, vb + t.y), Cc = ldp<COH>(P, vb + t.z);
     const f3 pA = xyz(A), pB = xyz(B), pC = xyz(Cc);
     const bool near0 = tri_meets_box(box[0], pA, pB, pC), near1 = tri_meets_box(box[1], pA, pB, pC),
                near2 = tri_meets_box(box[2], pA, pB, pC);
@@ -370,7 +384,10 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
     }
     for (int it = 0; it < prm.iterations; ++it) {
       // ---- projection: tets (volume + Neo-Hookean) and edges (distance) ----
-      project<BLOCK, CARRY>(b, prm, e, c, nhk, P, vb, T, do_tets ? e.nt : 0, do_edges ? e.ne : 0);
+      {
+        const int nt = do_tets ? e.nt : 0, ne = do_edges ? e.ne : 0;
+        project<BLOCK, CARRY, false>(b, prm, e, c, nhk, P, vb, T, nt, ne, 0, nt > ne ? nt : ne, false);
+      }
       __syncthreads();
       // ---- apply (+ grasp; + ground, integrate and next predict) ----
       const bool last_it = (it == prm.iterations - 1);
@@ -412,7 +429,7 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
       __syncthreads();
       if (collide) {
         // ---- instrument collision (collideLaparoscopeDO), summed in LDS ----
-        collide_env(b, prm, e, env, c, P, vb, &contacts, BLOCK);
+        collide_env<false>(b, prm, e, env, c, P, vb, &contacts, BLOCK, 0, e.nf);
         __syncthreads();
         for (int i = tid; i < nv; i += BLOCK) {
           const unsigned long long w0 = c.acc[2 * i], w1 = c.acc[2 * i + 1];
@@ -484,6 +501,311 @@ static int launch_frame(const ffsrl_batch& b, const ffsrl_params& prm, const int
     grid = std::min(n, std::max(1, cus * std::max(1, per_cu)));
   }
   hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), lds, st, b, prm, order, n, refit, sched);
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Shared-env frame (ffsrl_frame_shared): the workgroups of one XCD share the
+// frames of that XCD's environments (list slot el belongs to XCD el % nxcd,
+// by the hardware id each workgroup reads).
+//
+// The frame of an env is a fixed sequence of passes (predict; then per
+// substep and iteration: projection, apply [, collision, collision apply]),
+// each cut into chunks numbered in pass order.  Workgroups take chunks from
+// the env's counter ctr[0]; a chunk of a pass starts once every chunk of the
+// passes before it is done (ctr[1] >= the pass's first chunk).  Projection
+// and collision chunks sum into the workgroup's LDS accumulators; the
+// workgroup adds its sums into the env's global sums (split_acc, the same
+// 64-bit fixed-point words as the LDS form, with integer atomics that the
+// XCD's L2 performs) when it leaves the pass, and only then counts those
+// chunks done.  An apply chunk reads and clears the global sums of its
+// vertices.  Integer sums do not depend on the order of the additions, so the
+// result equals the one-workgroup frame bit for bit.
+//
+// Progress: a workgroup waits only after it has counted every chunk it holds,
+// and only running workgroups hold chunks, so every pass completes whatever
+// number of workgroups is resident.  Waits are bounded anyway
+// (kShareSpinLimit); a frame that could not complete sets ctr[2].
+//
+// Coherence: all workgroups of an env run on one XCD and share its L2.
+// Stores and atomics have reached the L2 when `s_waitcnt vmcnt(0)` returns
+// (wait_vmem, before a chunk is counted done); state written during the
+// kernel (positions, velocities, carried lambdas, the global sums) is read
+// with nontemporal loads, which skip the CU's L1.
+constexpr int kShareItems = 2048;        // tets / edges (by index) per projection chunk
+constexpr int kShareVerts = 2048;        // vertices per apply chunk
+constexpr int kShareTris = 2048;         // surface triangles per collision chunk
+constexpr int kShareSpinLimit = 1 << 22; // polls of a pass wait before the frame gives up
+
+enum ShareKind { kPredictPass, kProjectPass, kApplyPass, kCollidePass, kCollideApplyPass };
+
+struct SharePlan {
+  int cv, cc, cf, unit, total;
+};
+
+__host__ __device__ inline SharePlan share_plan(const ffsrl_env_desc& e, const ffsrl_params& prm, bool collide) {
+  const bool do_tets = prm.vol_mode != FFSRL_OFF || prm.nh_on, do_edges = prm.dist_mode != FFSRL_OFF;
+  const int nt = do_tets ? e.nt : 0, ne = do_edges ? e.ne : 0, nmax = nt > ne ? nt : ne;
+  SharePlan p;
+  p.cv = (e.nv + kShareVerts - 1) / kShareVerts;
+  if (p.cv < 1) p.cv = 1;
+  p.cc = (nmax + kShareItems - 1) / kShareItems;
+  if (p.cc < 1) p.cc = 1;
+  p.cf = 0;
+  if (collide) {
+    p.cf = (e.nf + kShareTris - 1) / kShareTris;
+    if (p.cf < 1) p.cf = 1;
+  }
+  p.unit = p.cc + p.cv + (collide ? p.cf + p.cv : 0);
+  p.total = p.cv + prm.substeps * prm.iterations * p.unit;
+  return p;
+}
+
+struct ShareChunk {
+  int kind, base, local, s, it;  // pass kind, first chunk of the pass, index in the pass, substep, iteration
+};
+
+__device__ __forceinline__ ShareChunk share_chunk(const SharePlan& p, int c, int iterations) {
+  ShareChunk k;
+  if (c < p.cv) {
+    k.kind = kPredictPass;
+    k.base = 0;
+    k.local = c;
+    k.s = 0;
+    k.it = 0;
+    return k;
+  }
+  const int c1 = c - p.cv, u = c1 / p.unit, o = c1 - u * p.unit, ub = p.cv + u * p.unit;
+  k.s = u / iterations;
+  k.it = u - k.s * iterations;
+  if (o < p.cc) {
+    k.kind = kProjectPass;
+    k.base = ub;
+    k.local = o;
+  } else if (o < p.cc + p.cv) {
+    k.kind = kApplyPass;
+    k.base = ub + p.cc;
+    k.local = o - p.cc;
+  } else if (o < p.cc + p.cv + p.cf) {
+    k.kind = kCollidePass;
+    k.base = ub + p.cc + p.cv;
+    k.local = o - p.cc - p.cv;
+  } else {
+    k.kind = kCollideApplyPass;
+    k.base = ub + p.cc + p.cv + p.cf;
+    k.local = o - p.cc - p.cv - p.cf;
+  }
+  return k;
+}
+
+// Polls ctr[1] until it reaches target; false when the wait is given up
+// (limit reached, or another workgroup of the env gave up: ctr[2] set).
+__device__ __forceinline__ bool share_wait(uint32_t* ctr, uint32_t target) {
+  for (int k = 0; k < kShareSpinLimit; ++k) {
+    if (atomicAdd(&ctr[1], 0u) >= target) return true;
+    if ((k & 255) == 255 && atomicAdd(&ctr[2], 0u) != 0u) return false;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return false;
+}
+
+// The env's share of one workgroup: chunks until the env's counter runs out.
+template <int BLOCK, bool CARRY, bool COLLIDE>
+__device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, int env, int refit,
+                          unsigned long long* lds) {
+  __shared__ int s_chunk, s_ok, s_last, s_contacts;
+  const ffsrl_env_desc e = b.envs[env];
+  const SharePlan pl = share_plan(e, prm, COLLIDE);
+  uint32_t* ctr = b.split_ctr + 4 * (int64_t)env;
+  unsigned long long* gacc = reinterpret_cast<unsigned long long*>(b.split_acc) + 2 * (int64_t)e.vbase;
+  const int nv = e.nv, tid = threadIdx.x;
+  const int64_t vb = e.vbase;
+  FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
+  float* X = b.x;
+  float* P = b.p;
+  float* Vv = b.v;
+  VertexStep vs;
+  vs.g = mk(prm.gravity[0], prm.gravity[1], prm.gravity[2]);
+  vs.dt = prm.dt;
+  vs.damp1 = 1.0f - prm.damping;
+  vs.inv_dt = 1.0f / prm.dt;
+  vs.ground = prm.ground;
+  vs.ks = prm.drag_ks;
+  const NHConst nhk = nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt);
+  const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
+  const int64_t T = b.n_tet_total;
+  const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
+  const int nt = (prm.vol_mode != FFSRL_OFF || prm.nh_on) ? e.nt : 0;
+  const int ne = prm.dist_mode != FFSRL_OFF ? e.ne : 0;
+  const int nmax = nt > ne ? nt : ne;
+
+  // ground clamp, integrate and the next substep's predict of vertex i (as
+  // the one-workgroup frame)
+  auto finish = [&](int i, f3 p, float w, bool next_sub) {
+    if (p.y < vs.ground) p.y = vs.ground;
+    const float4 x4 = ld4c(X, vb + i);
+    const f3 v = (p - xyz(x4)) * vs.inv_dt;
+    st4(X, vb + i, with_w(p, x4.w));
+    if (next_sub)
+      p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
+    else
+      st4(Vv, vb + i, with_w(v, 0.0f));
+    st4(P, vb + i, with_w(p, w));
+  };
+
+  for (int i = tid; i < 2 * nv; i += BLOCK) lds[i] = 0ull;
+  if (tid == 0) s_contacts = 0;
+  int pend_base = -1, pend = 0;
+  for (;;) {
+    __syncthreads();  // LDS and the shared words of the previous chunk are done with
+    if (tid == 0) s_chunk = (int)atomicAdd(&ctr[0], 1u);
+    __syncthreads();
+    const int ci = s_chunk;
+    ShareChunk ch = {0, 0, 0, 0, 0};
+    if (ci < pl.total) ch = share_chunk(pl, ci, prm.iterations);
+    if (pend && (ci >= pl.total || ch.base != pend_base)) {
+      // leave the pass: add the LDS sums to the env's sums, then count the chunks
+      for (int i = tid; i < nv; i += BLOCK) {
+        const unsigned long long w0 = lds[2 * i], w1 = lds[2 * i + 1];
+        if (w0) {
+          atomicAdd(&gacc[2 * i], w0);
+          lds[2 * i] = 0ull;
+        }
+        if (w1) {
+          atomicAdd(&gacc[2 * i + 1], w1);
+          lds[2 * i + 1] = 0ull;
+        }
+      }
+      wait_vmem();
+      __syncthreads();
+      if (tid == 0) {
+        if (s_contacts) atomicAdd(&b.in_collision[env], s_contacts);
+        s_contacts = 0;
+        atomicAdd(&ctr[1], (uint32_t)pend);
+      }
+      pend = 0;
+    }
+    if (ci >= pl.total) break;
+    if (tid == 0) s_ok = ch.base == 0 || share_wait(ctr, (uint32_t)ch.base);
+    __syncthreads();
+    if (!s_ok) {
+      if (tid == 0) atomicOr(&ctr[2], 1u);
+      break;
+    }
+    const bool last_it = ch.it == prm.iterations - 1, next_sub = ch.s + 1 < prm.substeps;
+    if (ch.kind == kProjectPass || ch.kind == kCollidePass) {
+      if (ch.kind == kProjectPass) {
+        const int jb = ch.local * kShareItems, je = jb + kShareItems < nmax ? jb + kShareItems : nmax;
+        project<BLOCK, CARRY, true>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, reset_lambda && ch.it == 0);
+      } else {
+        const int f0 = ch.local * kShareTris, f1 = f0 + kShareTris < e.nf ? f0 + kShareTris : e.nf;
+        collide_env<true>(b, prm, e, env, c, P, vb, &s_contacts, BLOCK, f0, f1);
+      }
+      pend_base = ch.base;
+      pend += 1;
+      continue;
+    }
+    const int v0 = ch.local * kShareVerts, v1 = v0 + kShareVerts < nv ? v0 + kShareVerts : nv;
+    if (ch.kind == kPredictPass) {
+      for (int i = v0 + tid; i < v1; i += BLOCK) {
+        const float4 x = ld4c(X, vb + i), v = ld4c(Vv, vb + i);
+        st4(P, vb + i, with_w(predict(xyz(x), xyz(v), x.w, vs.g, vs.dt, vs.damp1), x.w));
+      }
+    } else if (ch.kind == kApplyPass) {
+      const bool fuse = last_it && !COLLIDE;
+      for (int i = v0 + tid; i < v1; i += BLOCK) {
+        const unsigned long long w0 = ldu64c(gacc, 2 * i), w1 = ldu64c(gacc, 2 * i + 1);
+        if (w0) gacc[2 * i] = 0ull;
+        if (w1) gacc[2 * i + 1] = 0ull;
+        int32_t fx, fy, fz, n;
+        unpack2(w0, &fx, &fy);
+        unpack2(w1, &fz, &n);
+        const float4 p4 = ld4c(P, vb + i);
+        f3 p = xyz(p4);
+        if (n > 0) {
+          const float sc = c.q * __builtin_amdgcn_rcpf((float)n);
+          p = p + mk((float)fx, (float)fy, (float)fz) * sc;
+        }
+        const float dr = b.drag[vb + i];
+        if (dr != 0.0f) p = grasp_pull<true>(p, tip, dr, vs.ks);
+        if (fuse)
+          finish(i, p, p4.w, next_sub);
+        else
+          st4(P, vb + i, with_w(p, p4.w));
+      }
+    } else {  // collision apply
+      for (int i = v0 + tid; i < v1; i += BLOCK) {
+        const unsigned long long w0 = ldu64c(gacc, 2 * i), w1 = ldu64c(gacc, 2 * i + 1);
+        if (w0) gacc[2 * i] = 0ull;
+        if (w1) gacc[2 * i + 1] = 0ull;
+        if (!last_it && (w0 | w1) == 0ull) continue;
+        int32_t fx, fy, fz, n;
+        unpack2(w0, &fx, &fy);
+        unpack2(w1, &fz, &n);
+        const float4 p4 = ld4c(P, vb + i);
+        const f3 p = xyz(p4) + mk((float)fx, (float)fy, (float)fz) * c.q;
+        if (last_it)
+          finish(i, p, p4.w, next_sub);
+        else
+          st4(P, vb + i, with_w(p, p4.w));
+      }
+    }
+    wait_vmem();
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(&ctr[1], 1u) + 1u == (uint32_t)pl.total;
+    __syncthreads();
+    if (refit && s_last) refit_env<BLOCK, true>(b, e);  // every chunk of the frame is done
+  }
+}
+
+template <int BLOCK, bool CARRY, bool COLLIDE>
+__global__ __launch_bounds__(BLOCK) void k_frame_shared(ffsrl_batch b, ffsrl_params prm, const int32_t* list, int n,
+                                                        int refit, int nxcd) {
+  extern __shared__ unsigned long long lds[];
+  const int x = xcc_id();
+  const int mine = (x < nxcd && x < n) ? (n - 1 - x) / nxcd + 1 : 0;  // slots x, x + nxcd, ...
+  const int rank = (int)blockIdx.x / nxcd;                           // spreads the XCD's workgroups
+  for (int k = 0; k < mine; ++k) {
+    const int el = x + nxcd * ((rank + k) % mine);
+    share_env<BLOCK, CARRY, COLLIDE>(b, prm, list ? list[el] : el, refit, lds);
+  }
+  // The last workgroup out checks that every env's frame completed and
+  // clears the chunk counters for the next launch.
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t* exits = b.split_ctr + 4 * (int64_t)(list ? list[0] : 0) + 3;
+    if (atomicAdd(exits, 1u) == gridDim.x - 1) {
+      for (int el = 0; el < n; ++el) {
+        const int env = list ? list[el] : el;
+        uint32_t* ctr = b.split_ctr + 4 * (int64_t)env;
+        const SharePlan p = share_plan(b.envs[env], prm, COLLIDE);
+        if (atomicAdd(&ctr[1], 0u) != (uint32_t)p.total) atomicOr(&ctr[2], 2u);
+        atomicExch(&ctr[0], 0u);
+        atomicExch(&ctr[1], 0u);
+      }
+      atomicExch(exits, 0u);
+    }
+  }
+}
+
+__global__ void k_xcc_probe(int32_t* out) {
+  if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
+}
+
+template <int BLOCK>
+static int launch_shared(const ffsrl_batch& b, const ffsrl_params& prm, const int32_t* list, int n, int refit,
+                         bool carry, int lds, int nxcd, hipStream_t st) {
+  auto kern = prm.collide ? (carry ? k_frame_shared<BLOCK, true, true> : k_frame_shared<BLOCK, false, true>)
+                          : (carry ? k_frame_shared<BLOCK, true, false> : k_frame_shared<BLOCK, false, false>);
+  FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  int dev = 0, cus = 0, per_cu = 0;
+  FFSRL_HIP(hipGetDevice(&dev));
+  FFSRL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  FFSRL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), BLOCK, lds));
+  const int grid = std::max(nxcd, cus * std::max(1, per_cu));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), lds, st, b, prm, list, n, refit, nxcd);
   FFSRL_HIP(hipGetLastError());
   return 0;
 }
@@ -739,8 +1061,8 @@ int ffsrl_abi_version(void) { return FFSRL_ABI_VERSION; }
 const char* ffsrl_last_error(void) { return g_err.c_str(); }
 int ffsrl_frame_max_vertices(void) { return kMaxFrameVertices; }
 
-static int frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
-                 int32_t refit, uint32_t* sched, void* stream) {
+// Argument checks shared by the batched frame entry points.
+static int frame_checks(const ffsrl_batch* b, const ffsrl_params* prm, int32_t n, int32_t refit) {
   if (!prm) return fail(FFSRL_E_ARG, "params NULL");
   int rc = check_batch(b, prm);
   if (rc) return rc;
@@ -757,7 +1079,15 @@ static int frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* e
   if (prm->collide && (!b->tris || !b->caps_base || !b->caps_radius || !b->in_collision))
     return fail(FFSRL_E_ARG, "collision needs triangles, capsule bases / radii and the in_collision counters");
   if (n < 0 || n > b->n_env) return fail(FFSRL_E_ARG, "%d environments listed, batch has %d", n, b->n_env);
+  return 0;
+}
+
+static int frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
+                 int32_t refit, uint32_t* sched, void* stream) {
+  const int rc = frame_checks(b, prm, n, refit);
+  if (rc) return rc;
   if (n == 0) return 0;
+  const bool carry = carries_lambda(*prm);
   const int lds = lds_bytes_for(b->max_nv < 1 ? 1 : b->max_nv);
   hipStream_t st = as_stream(stream);
   switch (block) {
@@ -778,6 +1108,58 @@ int ffsrl_frame_envs(const ffsrl_batch* b, const ffsrl_params* prm, const int32_
                      int32_t refit, void* stream) {
   if (n > 0 && !envs) return fail(FFSRL_E_ARG, "env list NULL");
   return frame(b, prm, envs, n, block, refit, nullptr, stream);
+}
+
+// XCD count of each device, probed once: the hardware XCD ids seen by one
+// workgroup per CU must be 0 .. count-1.
+static std::mutex g_xcd_mu;
+static int g_xcd[64];
+
+int ffsrl_xcd_count(void) {
+  int dev = 0, cus = 0;
+  FFSRL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(FFSRL_E_LIMIT, "device %d", dev);
+  std::lock_guard<std::mutex> lock(g_xcd_mu);
+  if (g_xcd[dev] > 0) return g_xcd[dev];
+  FFSRL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int32_t* d = nullptr;
+  FFSRL_HIP(hipMalloc(&d, sizeof(int32_t) * cus));
+  std::vector<int32_t> h(cus, -1);
+  hipLaunchKernelGGL(k_xcc_probe, dim3(cus), dim3(64), 0, 0, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(h.data(), d, sizeof(int32_t) * cus, hipMemcpyDeviceToHost);
+  hipFree(d);
+  FFSRL_HIP(e);
+  bool seen[16] = {};
+  int mx = -1;
+  for (int v : h) {
+    if (v < 0 || v >= 16) return fail(FFSRL_E_LIMIT, "XCD id %d outside 0..15", v);
+    seen[v] = true;
+    mx = std::max(mx, v);
+  }
+  for (int i = 0; i <= mx; ++i)
+    if (!seen[i]) return fail(FFSRL_E_LIMIT, "XCD ids are not 0..%d (no workgroup on XCD %d)", mx, i);
+  g_xcd[dev] = mx + 1;
+  return mx + 1;
+}
+
+int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
+                       int32_t refit, void* stream) {
+  int rc = frame_checks(b, prm, n, refit);
+  if (rc) return rc;
+  if (!b->split_acc || !b->split_ctr) return fail(FFSRL_E_ARG, "shared frame needs split_acc and split_ctr");
+  if (n == 0) return 0;
+  const int nxcd = ffsrl_xcd_count();
+  if (nxcd <= 0) return nxcd < 0 ? nxcd : fail(FFSRL_E_LIMIT, "no XCD found");
+  const bool carry = carries_lambda(*prm);
+  const int lds = lds_bytes_for(b->max_nv < 1 ? 1 : b->max_nv);
+  hipStream_t st = as_stream(stream);
+  switch (block) {
+    case 256: return launch_shared<256>(*b, *prm, envs, n, refit, carry, lds, nxcd, st);
+    case 512: return launch_shared<512>(*b, *prm, envs, n, refit, carry, lds, nxcd, st);
+    case 1024: return launch_shared<1024>(*b, *prm, envs, n, refit, carry, lds, nxcd, st);
+    default: return fail(FFSRL_E_ARG, "block must be 256, 512 or 1024 (got %d)", block);
+  }
 }
 
 int ffsrl_predict(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {

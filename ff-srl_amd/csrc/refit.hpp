@@ -16,7 +16,9 @@
 namespace ffsrl {
 
 // Refit of one node given its refit-group entry (node id; left child or
-// first triangle, count).
+// first triangle, count).  COH: positions are read with ld4c (written by
+// other workgroups during the same kernel).
+template <bool COH = false>
 __device__ __forceinline__ void refit_node_info(const ffsrl_batch& b, const ffsrl_env_desc& e, int node, int2 info) {
   const int64_t vb = e.vbase;
   const int n = info.y;
@@ -26,7 +28,7 @@ __device__ __forceinline__ void refit_node_info(const ffsrl_batch& b, const ffsr
     f3 mn = mk(1e6f, 1e6f, 1e6f), mx = mk(-1e6f, -1e6f, -1e6f);
     for (int k = 0; k < n; ++k) {
       const int4 t = ldi4(b.tris, (int64_t)e.fbase + first + k);
-      const f3 A = xyz(ld4(b.x, vb + t.x)), B = xyz(ld4(b.x, vb + t.y)), C = xyz(ld4(b.x, vb + t.z));
+      const f3 A = xyz(ldp<COH>(b.x, vb + t.x)), B = xyz(ldp<COH>(b.x, vb + t.y)), C = xyz(ldp<COH>(b.x, vb + t.z));
       mn = vmin(A, vmin(B, vmin(C, mn)));
       mx = vmax(A, vmax(B, vmax(C, mx)));
     }
@@ -46,16 +48,17 @@ __device__ __forceinline__ void refit_node_info(const ffsrl_batch& b, const ffsr
 }
 
 // Refit of entry k of the env's mesh's refit groups.
+template <bool COH = false>
 __device__ __forceinline__ void refit_node(const ffsrl_batch& b, const ffsrl_env_desc& e, int k) {
   const int64_t kb = (int64_t)e.nodebase + k;
-  refit_node_info(b, e, b.level_nodes[kb], ldi2(b.level_info, kb));
+  refit_node_info<COH>(b, e, b.level_nodes[kb], ldi2(b.level_info, kb));
 }
 
 // U entries of one group (k, k + stride, ...): every load of the U entries is
 // issued before the first store, so their latencies overlap.  A group holds
 // only leaves or only inner nodes; single-triangle leaves and inner nodes take
 // this path, anything else goes entry by entry through refit_node.
-template <int U>
+template <int U, bool COH = false>
 __device__ __forceinline__ void refit_entries(const ffsrl_batch& b, const ffsrl_env_desc& e, int k, int stride) {
   int node[U];
   int2 info[U];
@@ -79,7 +82,7 @@ __device__ __forceinline__ void refit_entries(const ffsrl_batch& b, const ffsrl_
     for (int u = 0; u < U; ++u) t[u] = ldi4(b.tris, (int64_t)e.fbase + info[u].x);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const f3 A = xyz(ld4(b.x, vb + t[u].x)), B = xyz(ld4(b.x, vb + t[u].y)), C = xyz(ld4(b.x, vb + t[u].z));
+      const f3 A = xyz(ldp<COH>(b.x, vb + t[u].x)), B = xyz(ldp<COH>(b.x, vb + t[u].y)), C = xyz(ldp<COH>(b.x, vb + t[u].z));
       const f3 mn = vmin(A, vmin(B, vmin(C, mk(1e6f, 1e6f, 1e6f))));
       const f3 mx = vmax(A, vmax(B, vmax(C, mk(-1e6f, -1e6f, -1e6f))));
       lo[u] = make_float4(mn.x, mn.y, mn.z, __int_as_float(info[u].x));
@@ -97,7 +100,7 @@ __device__ __forceinline__ void refit_entries(const ffsrl_batch& b, const ffsrl_
     }
   } else {
 #pragma unroll
-    for (int u = 0; u < U; ++u) refit_node(b, e, k + u * stride);
+    for (int u = 0; u < U; ++u) refit_node<COH>(b, e, k + u * stride);
     return;
   }
 #pragma unroll
@@ -113,14 +116,14 @@ constexpr int kMaxRefitGroups = 64;  // leaf group + inner levels (BVH depth <= 
 // once into LDS, and each thread loads its first entry of the next group
 // before the barrier (entries do not depend on bounds), so a group costs the
 // round trip of the child bounds and the barrier.
-template <int BLOCK>
+template <int BLOCK, bool COH = false>
 __device__ void refit_env(const ffsrl_batch& b, const ffsrl_env_desc& e) {
   __shared__ int off[kMaxRefitGroups + 1];
   const int ng = e.nlvl;
   if (ng > kMaxRefitGroups) {  // deeper trees than the renderer accepts: plain group walk
     for (int L = 0; L < ng; ++L) {
       const int beg = b.level_off[e.lvlbase + L], end = b.level_off[e.lvlbase + L + 1];
-      for (int k = beg + (int)threadIdx.x; k < end; k += BLOCK) refit_node(b, e, k);
+      for (int k = beg + (int)threadIdx.x; k < end; k += BLOCK) refit_node<COH>(b, e, k);
       __syncthreads();
     }
     return;
@@ -139,15 +142,15 @@ __device__ void refit_env(const ffsrl_batch& b, const ffsrl_env_desc& e) {
     const int beg = off[L], end = off[L + 1];
     int k = beg + tid;
     if (k < end) {
-      refit_node_info(b, e, node, info);
+      refit_node_info<COH>(b, e, node, info);
       k += BLOCK;
     }
-    for (; k + 3 * BLOCK < end; k += 4 * BLOCK) refit_entries<4>(b, e, k, BLOCK);
+    for (; k + 3 * BLOCK < end; k += 4 * BLOCK) refit_entries<4, COH>(b, e, k, BLOCK);
     if (k + BLOCK < end) {
-      refit_entries<2>(b, e, k, BLOCK);
+      refit_entries<2, COH>(b, e, k, BLOCK);
       k += 2 * BLOCK;
     }
-    if (k < end) refit_node(b, e, k);
+    if (k < end) refit_node<COH>(b, e, k);
     if (L + 1 < ng && off[L + 1] + tid < off[L + 2]) {  // the next group's first entry
       const int64_t kb = (int64_t)e.nodebase + off[L + 1] + tid;
       node = b.level_nodes[kb];

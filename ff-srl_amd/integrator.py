@@ -7,6 +7,10 @@ constraint projection with Jacobi averaging, the grasp pull and (with
 
 * batched (default): ``ffsrl_frame``, one launch for all environments and
   the whole substep loop (and optionally the BVH refit);
+* shared (``shared=True``): ``ffsrl_frame_shared``, the same launch with all
+  CUs sharing the environments (each env's frame cut into chunks that the
+  workgroups of one XCD take in turn), for batches with fewer environments
+  than CUs; bit-identical to the batched form;
 * plain: the reference's launch model with one launch per kernel per
   environment (``integrator.py:1779-1929`` is the reference's per-env
   precedent), used as the comparison baseline.
@@ -20,10 +24,11 @@ from .model import SimModelDO
 
 
 class SimIntegratorDO:
-    def __init__(self, device="cuda:0", block: int = 1024, dispatch_sorted: bool = True):
+    def __init__(self, device="cuda:0", block: int = 1024, dispatch_sorted: bool = True, shared: bool = False):
         self.device = device
         self.block = block
         self.dispatch_sorted = dispatch_sorted
+        self.shared = shared
 
     def configure(self, cfg: dict) -> None:
         """Launch shape from the autotuner: {block, dispatch ("lpt"/"table"), ...}."""
@@ -38,10 +43,15 @@ class SimIntegratorDO:
         return cfg
 
     def stepModel(self, simModel: SimModelDO, refit: bool = False) -> None:
-        """One frame of all environments in one launch (``ffsrl_frame``)."""
+        """One frame of all environments in one launch (``ffsrl_frame``, or
+        ``ffsrl_frame_shared`` when the integrator is shared)."""
         m = simModel
         order = m.dispatch_order().data_ptr() if self.dispatch_sorted else None
-        lib.ffsrl_frame(m.batch, m.params, order, self.block, int(bool(refit)), m.stream())
+        if self.shared:
+            m.enable_shared()
+            lib.ffsrl_frame_shared(m.batch, m.params, order, m.numEnvs, self.block, int(bool(refit)), m.stream())
+        else:
+            lib.ffsrl_frame(m.batch, m.params, order, self.block, int(bool(refit)), m.stream())
 
     def stepModelPlain(self, simModel: SimModelDO, env_begin: int = 0, env_end: int = None) -> None:
         """One frame with one launch per kernel per environment (the library

@@ -595,6 +595,24 @@ class SimModelDO:
             self._batch.dp = _ptr(self.dp)
             self._batch.cnt = _ptr(self.cnt)
 
+    def enable_shared(self):
+        """Allocate the global Jacobi sums and chunk counters of the shared-env
+        frame (``ffsrl_frame_shared``); both start at zero and every frame
+        leaves them at zero.  Probes the device's XCD count, so call it
+        before capturing a shared frame into a graph."""
+        if getattr(self, "split_acc", None) is None:
+            self.split_acc = torch.zeros((self.numVertices, 2), dtype=torch.int64, device=self.device)
+            self.split_ctr = torch.zeros((self.numEnvs, 4), dtype=torch.int32, device=self.device)
+            self._batch.split_acc = _ptr(self.split_acc)
+            self._batch.split_ctr = _ptr(self.split_ctr)
+            if lib.ffsrl_xcd_count() <= 0:
+                raise RuntimeError(f"ffsrl_xcd_count: {lib.ffsrl_last_error().decode()}")
+
+    def shared_status(self) -> torch.Tensor:
+        """Per-env status words of the shared-env frame (0: every frame
+        completed; 1: a pass wait gave up; 2: a frame ended incomplete)."""
+        return self.split_ctr[:, 2]
+
     @property
     def batch(self) -> Batch:
         return self._batch

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 8
+#define FFSRL_ABI_VERSION 9
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -102,6 +102,11 @@ typedef struct ffsrl_batch {
   /* plain-form scratch (per-kernel launches with global atomics) */
   float* dp;                      /* [3][V] planes (NULL unless the plain form is used) */
   int32_t* cnt;                   /* [V] */
+  /* shared-env frame scratch (ffsrl_frame_shared; NULL otherwise) */
+  uint64_t* split_acc;            /* [V][2] fixed-point Jacobi sums; zero before the first
+                                     frame, left zero by every frame */
+  uint32_t* split_ctr;            /* [n_env][4] chunk counters (taken, done, status, exits);
+                                     zero before the first frame, left zero except status */
   /* host mirrors needed by the plain form, which launches per env (and per
    * BVH level) from the host: the env table and the level offsets */
   const ffsrl_env_desc* host_envs;   /* [n_env] (host memory) */
@@ -173,6 +178,28 @@ int ffsrl_frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* or
  * pending step request.  Same reference counterpart as ffsrl_frame. */
 int ffsrl_frame_envs(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
                      int32_t refit, void* stream);
+
+/* ffsrl_frame_envs with every CU working on the n listed environments:
+ * each environment's frame is cut into chunks (constraint ranges of a
+ * projection pass, vertex ranges of the apply passes, triangle ranges of the
+ * collision pass) that the workgroups on one XCD take from the
+ * environment's counter, passes ordered by its completion counter.  For
+ * batches with fewer environments than CUs (serving ticks, few large
+ * scenes).  Results equal ffsrl_frame_envs bit for bit (integer Jacobi sums).
+ * envs NULL: environments 0 .. n-1.
+ * A workgroup only waits for chunks that running workgroups hold, so the
+ * launch needs no co-residency; waits are bounded, and a frame that could
+ * not complete sets split_ctr[4*env+2] (status) to nonzero.  Launches that
+ * overlap on one environment race.  Same reference counterpart as
+ * ffsrl_frame. */
+int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n,
+                       int32_t block, int32_t refit, void* stream);
+
+/* Number of XCDs of the current device (probed once with one workgroup per
+ * CU; the shared frame assigns environments to XCDs).  Allocates on its
+ * first call per device, so call it before capturing a shared frame into a
+ * graph.  No reference counterpart (launch-shape query). */
+int ffsrl_xcd_count(void);
 
 /* Largest nv that ffsrl_frame can hold in LDS (dP + counters). */
 int ffsrl_frame_max_vertices(void);

@@ -1,0 +1,137 @@
+"""The shared-env frame (ffsrl_frame_shared: every CU works on a few
+environments, each env's frame cut into chunks taken by the workgroups of
+one XCD) against the one-workgroup-per-env frame (ffsrl_frame_envs /
+ffsrl_frame): positions, velocities, predicted positions, carried lambdas,
+BVH bounds and contact counts equal bit for bit (both sum the Jacobi deltas
+as integers), for mixed sizes, XPBD with carried lambdas, the grasp, the
+instrument collision, env lists, every block size, and repeated frames (the
+chunk counters must be left at zero).  The physics itself is pinned against
+the oracle by the batched frame's own parity tests.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+NH_DV = dict(simSubsteps=10, simFrameRate=60, simConstraintsSteps=1, environmentGravity=(0.0, -9.81, 0.0),
+             environmentVelocityDampening=0.1, distance="xpbd", volume="xpbd", distanceCompliance=1e-6,
+             volumeComplianceXPBD=0.0, neoHookean=True, globalVolumeCompliance=1.0 / 5e4,
+             globalDeviatoricCompliance=1.0 / 1e4, globalKsDrag=1.0)
+XPBD_I2 = dict(NH_DV, neoHookean=False, simConstraintsSteps=2, simSubsteps=5)
+
+
+def _meshes(pkg, n, lo, hi, seed):
+    sc = pkg.scenes
+    specs = sc.mixed_sizes(n, lo, hi, seed=seed)[0]
+    return [sc.solid(shape, int(t), seed=seed + i) for i, (shape, t) in enumerate(specs)]
+
+
+def _pair(pkg, gpu, meshes, kw, grasp=True, **extra):
+    sc = pkg.scenes
+    tips = np.stack([np.repeat((sc.top_point(m) + np.array([0.0, 0.02, 0.0], np.float32))[None], 4, axis=0)
+                     for m in meshes]).astype(np.float32)
+    models = []
+    for _ in range(2):
+        m = pkg.SimModelDO(meshes, list(range(len(meshes))), device=gpu, tips=tips, **kw, **extra)
+        if grasp:
+            for e in range(1, len(meshes), 2):
+                m.set_grasp(e, sc.nearest_surface_vertices(meshes[e], tips[e, 2], 8))
+        models.append(m)
+    return models
+
+
+STATE = ("x", "v", "p", "bvh_lo", "bvh_hi")
+
+
+def _equal(a, b, extra=()):
+    torch.cuda.synchronize()
+    for k in STATE + tuple(extra):
+        ta, tb = getattr(a, k), getattr(b, k)
+        assert torch.equal(ta, tb), (k, float((ta - tb).abs().max()))
+
+
+def _clean(m):
+    torch.cuda.synchronize()
+    assert int(m.shared_status().abs().sum()) == 0, m.shared_status()
+    assert int(m.split_ctr.abs().sum()) == 0 and int(m.split_acc.abs().sum()) == 0
+
+
+@pytest.mark.parametrize("block", [1024, 512, 256])
+def test_shared_equals_batched_mixed(pkg, gpu, block):
+    meshes = _meshes(pkg, 12, 1000, 20000, seed=5)
+    a, b = _pair(pkg, gpu, meshes, NH_DV)
+    ref = pkg.SimIntegratorDO(gpu, block=block)
+    sh = pkg.SimIntegratorDO(gpu, block=block, shared=True)
+    for f in range(3):
+        ref.stepModel(a, refit=True)
+        sh.stepModel(b, refit=True)
+        _equal(a, b)
+        _clean(b)
+    assert bool(torch.isfinite(b.x).all())
+    print(f"\n[shared] block {block}: 12 envs ({sum(m.num_tets for m in meshes)} tets) equal after 3 frames")
+
+
+def test_shared_xpbd_carried_lambdas(pkg, gpu):
+    meshes = _meshes(pkg, 6, 2000, 12000, seed=17)
+    a, b = _pair(pkg, gpu, meshes, XPBD_I2)
+    assert a.lam_t is not None
+    ref, sh = pkg.SimIntegratorDO(gpu), pkg.SimIntegratorDO(gpu, shared=True)
+    for f in range(3):
+        ref.stepModel(a, refit=True)
+        sh.stepModel(b, refit=True)
+    _equal(a, b, ("lam_t", "lam_e"))
+    _clean(b)
+
+
+def test_shared_env_list(pkg, gpu):
+    meshes = _meshes(pkg, 8, 1000, 15000, seed=23)
+    a, b = _pair(pkg, gpu, meshes, NH_DV)
+    envs = torch.tensor([5, 0, 2], dtype=torch.int32, device=gpu)
+    x0 = b.x.clone()
+    for f in range(2):
+        pkg.lib.ffsrl_frame_envs(a.batch, a.params, envs.data_ptr(), 3, 1024, 1, a.stream())
+        b.enable_shared()
+        pkg.lib.ffsrl_frame_shared(b.batch, b.params, envs.data_ptr(), 3, 1024, 1, b.stream())
+    _equal(a, b)
+    _clean(b)
+    # the envs not listed did not move
+    for e in (1, 3, 4, 6, 7):
+        sl = b.env_slice(e)
+        assert torch.equal(b.x[sl], x0[sl])
+    # then all envs, and a single env
+    pkg.SimIntegratorDO(gpu).stepModel(a, refit=True)
+    pkg.SimIntegratorDO(gpu, shared=True).stepModel(b, refit=True)
+    one = envs[2:]
+    pkg.lib.ffsrl_frame_envs(a.batch, a.params, one.data_ptr(), 1, 1024, 1, a.stream())
+    pkg.lib.ffsrl_frame_shared(b.batch, b.params, one.data_ptr(), 1, 1024, 1, b.stream())
+    _equal(a, b)
+    _clean(b)
+
+
+def test_shared_collision(pkg, gpu):
+    sc = pkg.scenes
+    meshes = [sc.solid("ellipsoid", 5000, seed=21 + i) for i in range(3)]
+    R = 0.12
+    bases, tips = [], []
+    for m in meshes:
+        q = sc.top_point(m).astype(np.float64)
+        c0 = q + np.array([0.0, 0.7 * R, 0.0])
+        bases.append(np.stack([c0 - [0, R, 0], q + [-0.15, 0.0, 0.0], q + [0, 3.0, 0], q + [0, 0.5, 0]]))
+        tips.append(np.stack([c0 + [0, 0.5 + R, 0], q + [-0.15, 0.6, 0.0], q + [0, 3.5, 0], q + [0, 0.5, 0]]))
+    f32 = lambda v: np.asarray(v, np.float32)
+    kw = dict(NH_DV, neoHookean=False)
+    models = [pkg.SimModelDO(meshes, [0, 1, 2], device=gpu, tips=f32(tips), capsuleBases=f32(bases),
+                             capsuleRadii=f32([[R, R, 0.05, 0.0]] * 3), collision=True, globalKsContact=1.0, **kw)
+              for _ in range(2)]
+    a, b = models
+    ref, sh = pkg.SimIntegratorDO(gpu), pkg.SimIntegratorDO(gpu, shared=True)
+    d = torch.tensor([[0.0, -0.004, 0.0]] * 3, dtype=torch.float32, device=gpu)
+    for f in range(4):
+        for m in (a, b):
+            pkg.lib.ffsrl_move_tools(m.batch, d.data_ptr(), None, None, m.stream())
+        ref.stepModel(a, refit=True)
+        sh.stepModel(b, refit=True)
+    _equal(a, b, ("laparoscopeInCollision",))
+    assert int(b.laparoscopeInCollision.sum()) > 0
+    _clean(b)
