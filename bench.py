@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--resolution", type=int, default=None, help="render resolution override")
     ap.add_argument("--envs", type=int, default=None)
     ap.add_argument("--block", type=int, default=1024)
+    ap.add_argument("--shared", choices=["auto", "on", "off"], default="auto",
+                    help="shared-env frame: autotuned (auto), forced on, or off")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --block and largest-first dispatch instead of the autotuned launch shape")
     ap.add_argument("--plain-frames", type=int, default=2)
@@ -222,7 +224,8 @@ def serving_run(pkg, wl, args, dev):
     capacity = np.bincount(cls, minlength=len(sw.classes))
     srv = S.SceneServer(sw.classes, capacity, args.serve_budget_gb * 1e9, sw.model_kwargs, sw.tips, device=dev,
                         resolution=sw.resolution, camera_pos=sw.camera_pos, look_at=sw.look_at,
-                        block=args.block, streams=args.serve_streams, max_inflight=args.serve_inflight)
+                        block=args.block, streams=args.serve_streams, max_inflight=args.serve_inflight,
+                        shared=args.shared)
     for a in range(sw.actors):
         srv.register(a, int(cls[a]))
     log(f"serving {sw.name}: {len(reqs)} requests over {args.serve_seconds:.0f} s, "
@@ -311,6 +314,8 @@ def main():
         t_tune = time.perf_counter()
         tuned = integ.autotune(model)
         log(f"autotuned {tuned} in {time.perf_counter() - t_tune:.1f} s")
+    if args.shared != "auto":
+        integ.shared = args.shared == "on"
     if args.calibrate_copy:
         a = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -366,7 +371,7 @@ def main():
                    "substeps": w.model_kwargs["simSubsteps"], "iterations": w.model_kwargs["simConstraintsSteps"],
                    "render": f"{w.resolution}x{w.resolution} depth{' + RGB (per-triangle colours)' if w.rgb else ''} "
                              f"(2x2 supersampled)", "block": integ.block,
-                   "dispatch": "lpt" if integ.dispatch_sorted else "table",
+                   "dispatch": "lpt" if integ.dispatch_sorted else "table", "shared": integ.shared,
                    "chunk": model.topo[0].chunk, "autotuned": tuned is not None,
                    "form": "batched (1 frame kernel + 1 render kernel per step)",
                    **({"collision": True} if w.model_kwargs.get("collision") else {})},
@@ -379,7 +384,8 @@ def main():
     achieved = alg / (frame_ms.mean() * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(w.name),
-                       "kernel": "k_frame (physics + refit)", "algorithmic_bytes_per_launch": alg}
+                       "kernel": ("k_frame_shared" if integ.shared else "k_frame") + " (physics + refit)",
+                       "algorithmic_bytes_per_launch": alg}
 
     if not args.no_graph:
         log("graph replay")

@@ -7,7 +7,10 @@ Knobs (all leave the results unchanged; the frame's Jacobi sums are integers):
   dispatch  environment dispatch order: "lpt" (largest first) or "table";
   chunk     run length of the colour sort inside the Morton constraint order
             (the lanes of one LDS atomic rarely share a vertex within a run,
-            and a wavefront's position gathers stay local).
+            and a wavefront's position gathers stay local);
+  shared    the shared-env frame (every CU on the batch's envs, each env's
+            frame cut into chunks), tried when there are at most half as
+            many envs as CUs.
 
 A trial runs whole frames on a snapshot of the state, which is restored
 afterwards, so tuning does not advance the simulation.  The cache is a JSON
@@ -31,7 +34,14 @@ from .model import LOCAL_CHUNK, SimModelDO
 BLOCKS = (256, 512, 1024)
 DISPATCH = ("lpt", "table")
 CHUNKS = (64, 128, 256)
-DEFAULT = {"block": 1024, "dispatch": "lpt", "chunk": LOCAL_CHUNK}
+DEFAULT = {"block": 1024, "dispatch": "lpt", "chunk": LOCAL_CHUNK, "shared": False}
+
+
+def shared_candidates(model: SimModelDO) -> tuple:
+    """(False,) or (False, True): the shared-env frame is a candidate when the
+    batch has at most half as many envs as the device has CUs."""
+    cus = torch.cuda.get_device_properties(model.device).multi_processor_count
+    return (False, True) if 2 * model.numEnvs <= cus else (False,)
 
 
 def _bucket(x: float) -> int:
@@ -72,17 +82,22 @@ class Autotuner:
     def _time(model: SimModelDO, cfg: dict, frames: int, stream) -> float:
         order = model.dispatch_order().data_ptr() if cfg["dispatch"] == "lpt" else None
         s = stream.cuda_stream
-        lib.ffsrl_frame(model.batch, model.params, order, cfg["block"], 1, s)          # warm
+        if cfg.get("shared"):
+            model.enable_shared()
+            run = lambda: lib.ffsrl_frame_shared(model.batch, model.params, order, model.numEnvs, cfg["block"], 1, s)
+        else:
+            run = lambda: lib.ffsrl_frame(model.batch, model.params, order, cfg["block"], 1, s)
+        run()                                                                           # warm
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(frames):
-            lib.ffsrl_frame(model.batch, model.params, order, cfg["block"], 1, s)
+            run()
         e1.record(stream)
         e1.synchronize()
         return e0.elapsed_time(e1) / frames
 
     def tune(self, model: SimModelDO, frames: int = 2, force: bool = False) -> dict:
-        """Best {block, dispatch, chunk} for this batch (cached), applied to
+        """Best {block, dispatch, chunk, shared} for this batch (cached), applied to
         the model's constraint order; the state is left as it was."""
         key = key_of(model)
         if not force and key in self.cache:
@@ -98,8 +113,8 @@ class Autotuner:
             model.set_constraint_order("local", chunk)          # carried lambdas follow their constraints
             lam = [(t, t.clone()) for t in self._lambdas(model)]
             torch.cuda.synchronize(model.device)
-            for block, dispatch in itertools.product(BLOCKS, DISPATCH):
-                cfg = {"block": block, "dispatch": dispatch, "chunk": chunk}
+            for block, dispatch, shared in itertools.product(BLOCKS, DISPATCH, shared_candidates(model)):
+                cfg = {"block": block, "dispatch": dispatch, "chunk": chunk, "shared": shared}
                 ms = self._time(model, cfg, frames, stream)
                 for t, saved in state + lam:
                     t.copy_(saved)

@@ -31,9 +31,10 @@ class SimIntegratorDO:
         self.shared = shared
 
     def configure(self, cfg: dict) -> None:
-        """Launch shape from the autotuner: {block, dispatch ("lpt"/"table"), ...}."""
+        """Launch shape from the autotuner: {block, dispatch ("lpt"/"table"), shared, ...}."""
         self.block = int(cfg["block"])
         self.dispatch_sorted = cfg["dispatch"] == "lpt"
+        self.shared = bool(cfg.get("shared", False))
 
     def autotune(self, simModel: SimModelDO, cache_path=None, frames: int = 2) -> dict:
         """Tune (or look up) the launch shape for this batch and use it."""

@@ -13,7 +13,8 @@ mesh's topology).  The loop
   has no request on the GPU (at most ``max_batch``), and launches it without
   waiting for earlier ticks: ``ffsrl_apply_requests`` (resets, tool moves,
   grasp), ``ffsrl_frame_envs`` (one frame of the stepped scenes, refit
-  included), ``ffsrl_render_envs`` (depth of every scene of the tick) and
+  included; ``ffsrl_frame_shared``, every CU on the tick's scenes, when the
+  tick steps few scenes), ``ffsrl_render_envs`` (depth of every scene of the tick) and
   ``ffsrl_observe_envs`` (32 scalars).  Up to ``max_inflight`` ticks are on
   the GPU at once, on ``streams`` HIP streams, so requests that arrive while
   others run are admitted into the next tick and their work overlaps on the
@@ -103,7 +104,7 @@ class SceneServer:
                  model_kwargs: dict, tips: np.ndarray, device="cuda:0", resolution: int = 64,
                  camera_pos=(0.0, 1.6, 5.5), look_at=(0.0, 0.45, 0.0), max_batch: int = 1024,
                  max_inflight: int = 4, streams: int = 2, block: int = 1024, grasp_radius: float = 0.05,
-                 workspace: Optional[tuple] = None):
+                 workspace: Optional[tuple] = None, shared: str = "auto"):
         self.device = torch.device(device)
         self.classes = list(classes)
         self.capacity = [int(c) for c in capacity]
@@ -122,6 +123,13 @@ class SceneServer:
         self.tips0 = m.initialTips.reshape(-1).contiguous()
         self.base0 = m.initialBases.reshape(-1).contiguous()
         self.block = int(block)
+        # shared-env frame for ticks that step at most half as many scenes as
+        # the device has CUs ("auto"), always ("on") or never ("off")
+        self.shared = shared
+        cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+        self.shared_max = {"auto": cus // 2, "on": self.n_slots, "off": 0}[shared]
+        if self.shared_max > 0:
+            m.enable_shared()
         self.radius = float(grasp_radius)
         self.max_batch = int(max_batch)
         self.max_inflight = int(max_inflight)
@@ -286,7 +294,9 @@ class SceneServer:
             r.lo = None if self._lo is None else self._lo.data_ptr()
             r.hi = None if self._hi is None else self._hi.data_ptr()
             lib.ffsrl_apply_requests(model.batch, r, self.radius, s)
-            if m:
+            if m and m <= self.shared_max:
+                lib.ffsrl_frame_shared(model.batch, model.params, p_step, m, self.block, 1, s)
+            elif m:
                 lib.ffsrl_frame_envs(model.batch, model.params, p_step, m, self.block, 1, s)
             cam = self.renderer.camera
             lib.ffsrl_render_envs(model.batch, cam, p_env, n, self.depth.data_ptr(), None, None, None, None, None,

@@ -71,16 +71,17 @@ def _run(server, scripts, classes):
     return got
 
 
-def test_served_alone_equals_in_mix(pkg, gpu):
+@pytest.mark.parametrize("shared", ["off", "on"])
+def test_served_alone_equals_in_mix(pkg, gpu, shared):
     actors = list(range(6))
     classes = {a: a % 2 for a in actors}
     scripts = _script(pkg, actors)
-    mix = _server(pkg, gpu)
+    mix = _server(pkg, gpu, shared=shared)
     got_mix = _run(mix, scripts, classes)
     assert mix.stats["ticks"] < sum(len(r) for r in scripts.values())       # requests were batched
     assert mix.stats["tick_requests"] == sum(len(r) for r in scripts.values())
     for a in actors:
-        alone = _server(pkg, gpu, max_batch=1)
+        alone = _server(pkg, gpu, max_batch=1, shared=shared)
         got = _run(alone, {a: scripts[a]}, classes)
         assert [s for s, _, _ in got[a]] == list(range(len(scripts[a])))
         assert [s for s, _, _ in got_mix[a]] == list(range(len(scripts[a])))
@@ -93,6 +94,24 @@ def test_served_alone_equals_in_mix(pkg, gpu):
         assert obs_last[25] > 0                                              # clamp closed: vertices grasped
         print(f"\n[serve] actor {a}: {len(got[a])} requests identical alone and in the mix; "
               f"grasped {int(obs_last[25])}, max displacement {float(obs_last[30]):.4f}")
+
+
+def test_shared_ticks_equal_batched_ticks(pkg, gpu):
+    """The same request stream served with the shared-env frame in every tick
+    and with one workgroup per scene: identical observations."""
+    actors = list(range(6))
+    classes = {a: a % 2 for a in actors}
+    scripts = _script(pkg, actors, seed=11)
+    runs = {}
+    for shared in ("off", "on"):
+        srv = _server(pkg, gpu, shared=shared)
+        runs[shared] = _run(srv, scripts, classes)
+        if shared == "on":
+            assert int(srv.model.shared_status().abs().sum()) == 0
+    for a in actors:
+        assert len(runs["on"][a]) == len(scripts[a])
+        for (s1, d1, o1), (s2, d2, o2) in zip(runs["off"][a], runs["on"][a]):
+            assert s1 == s2 and torch.equal(d1, d2) and torch.equal(o1, o2), (a, s1)
 
 
 def test_reset_restores_the_initial_observation(pkg, gpu):
