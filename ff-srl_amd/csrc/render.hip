@@ -40,20 +40,31 @@ __device__ __forceinline__ int isect_aabb(f3 o, f3 d, f3 lo, f3 hi, float* tmin_
 __device__ __forceinline__ bool close_rel(float a, float b) { return fabsf(a - b) <= 1e-6f * (fabsf(a) + fabsf(b)); }
 
 //
-// On the fast path the six products are finite (their sum is: a NaN or an
-// infinity in any of them makes it non-finite), so hardware min/max give the
-// compare-select results of the reference's wp.min/wp.max up to the sign of
-// a zero, which no comparison below can see.
-__device__ __forceinline__ bool visit_node(f3 o, f3 d, f3 inv, f3 lo, f3 hi, float tMin) {
+// The fast path takes min/max with IEEE 754-2019 minimum/maximum
+// (v_minimum3 / v_maximum3), which propagate NaN: a NaN among the six
+// products makes tmin and tmax NaN, and |NaN| > 1e-30 is false, so such a
+// node goes to the exact path.  Otherwise min/max select exactly what the
+// reference's compare-select wp.min/wp.max select, up to the sign of a zero,
+// which no comparison below can see.  A product that overflows where the
+// quotient does not could change a selection only among values beyond
+// 1e37: such tmin / tmax go to the exact path, and so do rays with a
+// direction component in (0, 1e-30) (ray_ok; with |1/d| <= 1e30 only
+// coordinate differences above 1e8 can overflow).  Other infinities come
+// from a direction component that is exactly zero, where the quotient is the
+// same infinity.
+__device__ __forceinline__ float vmin_(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+__device__ __forceinline__ float vmax_(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+
+__device__ __forceinline__ bool visit_node(f3 o, f3 d, f3 inv, f3 lo, f3 hi, float tMin, bool ray_ok) {
   const float t1 = (lo.x - o.x) * inv.x, t2 = (hi.x - o.x) * inv.x;
   const float t3 = (lo.y - o.y) * inv.y, t4 = (hi.y - o.y) * inv.y;
   const float t5 = (lo.z - o.z) * inv.z, t6 = (hi.z - o.z) * inv.z;
-  const float tmin = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
-  const float tmax = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
-  const bool finite = __builtin_isfinite((((t1 + t2) + (t3 + t4)) + (t5 + t6)));
+  const float tmin = vmax_(vmax_(vmin_(t1, t2), vmin_(t3, t4)), vmin_(t5, t6));
+  const float tmax = vmin_(vmin_(vmax_(t1, t2), vmax_(t3, t4)), vmax_(t5, t6));
   // every condition is evaluated (bitwise &, no short circuit), so the whole
   // fast path is one basic block with a single branch
-  const bool robust = finite & (fabsf(tmax) > 1e-30f) & !close_rel(tmax, tmin) & !close_rel(tmin, tMin);
+  const bool robust = ray_ok & (fabsf(tmax) > 1e-30f) & (fabsf(tmax) < 1e37f) & (fabsf(tmin) < 1e37f) &
+                      !close_rel(tmax, tmin) & !close_rel(tmin, tMin);
   if (robust) return (tmax >= 0.0f) & (tmax >= tmin) & (tmin < tMin);
   float tb;
   return isect_aabb(o, d, lo, hi, &tb) && tb < tMin;
@@ -115,6 +126,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
   const f3 o = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
   const f3 d = xyz(ld4(cam.dirs, r));
   const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  auto dir_ok = [](float c) { return (c == 0.0f) | (fabsf(c) >= 1e-30f); };
+  const bool ray_ok = dir_ok(d.x) & dir_ok(d.y) & dir_ok(d.z);
 
   // env-uniform bases; node / triangle / vertex ids are non-negative int32,
   // so loads take a 32-bit unsigned offset from a scalar base
@@ -130,7 +143,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
     const int node = stk[(count - 1) * 64];
     count -= 1;
     const float4 lo = ldu4(blo, node), hi = ldu4(bhi, node);
-    if (visit_node(o, d, inv, xyz(lo), xyz(hi), tMin)) {
+    if (visit_node(o, d, inv, xyz(lo), xyz(hi), tMin, ray_ok)) {
       const int n = __float_as_int(hi.w);
       if (n > 0) {
         const int first = __float_as_int(lo.w);

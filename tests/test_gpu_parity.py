@@ -99,6 +99,60 @@ def test_refit_and_depth_bit_exact(pkg, orc, gpu):
     assert np.array_equal(depth[0].cpu().numpy(), d_ref)
 
 
+def _edge_rays(n, rng):
+    """Ray directions that exercise every branch of the slab test: exact axis
+    directions, directions with one zero component (infinite reciprocals,
+    and NaN products where a box face lies in the origin's plane),
+    components below and at the 1e-30 limit of the fast path, and random
+    directions towards the unit cube."""
+    d = []
+    for ax in np.eye(3):
+        d += [-ax, ax]
+    for a in np.linspace(-0.9, 0.9, 40):
+        d += [(0.0, a, -1.0), (a, 0.0, -1.0), (a, -1.0, 0.0)]
+    for tiny in (1e-38, 1e-35, -1e-33, 5e-31, 1e-30, -1e-30, 3e-30, 1e-25):
+        for a in (-0.4, 0.1, 0.3):
+            d += [(tiny, a, -1.0), (a, tiny, -1.0), (a, -1.0, tiny)]
+    d = np.array(d, np.float64)
+    rest = rng.normal(size=(n - len(d), 3))
+    rest[:, 2] = -np.abs(rest[:, 2]) - 1.0
+    d = np.concatenate([d, rest])
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    out = np.zeros((n, 4), np.float32)
+    out[:, :3] = d.astype(np.float32)
+    return out
+
+
+def test_depth_edge_rays_bit_exact(pkg, orc, gpu):
+    """Depth, hits and t for rays with zero, tiny and axis-aligned direction
+    components, from origins that lie in planes of the mesh's grid, equal the
+    oracle's (the division form of the reference) bit for bit."""
+    mesh, _ = _c1_model(pkg)
+    model = pkg.SimModelDO([mesh], [0], device=gpu, **C1)
+    pkg.SimBVH(gpu, model)
+    x = model.vertex().cpu().numpy()
+    xs = np.unique(x[:, 0])
+    ys = np.unique(x[:, 1])
+    bvh = _bvh_dict(model.topo[0])
+    mn, mx = orc.refit(bvh, mesh.tris, x)
+    rng = np.random.default_rng(5)
+    ren = pkg.RaycastRendererDO(gpu, model, cameraPos=(0.0, 0.5, 2.5), resolution=16)
+    for origin in ((xs[0], ys[4], 2.5), (xs[3], ys[0], 2.5), (xs[5], ys[5], 2.5)):
+        ren.camera.origin[:] = tuple(float(c) for c in np.asarray(origin, np.float32))
+        dirs = _edge_rays(32 * 32, rng)
+        ren.dirs.copy_(torch.from_numpy(dirs))
+        hit = torch.full((4 * 16 * 16,), -7, dtype=torch.int32, device=gpu)
+        thit = torch.zeros(4 * 16 * 16, dtype=torch.float32, device=gpu)
+        depth = ren.render_depth(hit=hit, thit=thit)
+        torch.cuda.synchronize()
+        d_ref, h_ref, t_ref, _ = orc.render_depth(16, 16, np.asarray(ren.camera.origin[:], np.float32), dirs,
+                                                  bvh, mn, mx, mesh.tris, x)
+        assert (h_ref >= 0).sum() > 100
+        assert np.array_equal(hit.cpu().numpy(), h_ref), origin
+        assert np.array_equal(thit.cpu().numpy(), t_ref), origin
+        assert np.array_equal(depth[0].cpu().numpy(), d_ref), origin
+
+
 def test_refit_fused_equals_standalone(pkg, gpu):
     mesh, v0 = _c1_model(pkg, 8)
     model = pkg.SimModelDO([mesh], [0], device=gpu, initialVelocity=v0, **C1)
