@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--serve-actors", type=int, default=4096)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-streams", type=int, default=2)
-    ap.add_argument("--serve-inflight", type=int, default=4)
+    ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy before timing (HBM counter calibration)")
     ap.add_argument("--collision", action="store_true",

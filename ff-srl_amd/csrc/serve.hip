@@ -1,7 +1,8 @@
 // Request kernels of the serving loop: apply a tick's reset / step requests to
 // the listed environments, and reduce each listed environment to its
-// observation scalars.  Both run one workgroup per listed environment, so a
-// tick of mixed requests is two launches (plus the frame and the render).
+// observation scalars.  Both run one 1024-thread workgroup per listed
+// environment, so a tick of mixed requests is two launches (plus the frame
+// and the render).
 //
 // Reset follows resetFixedModel (modelDO.py:1589-1622): positions restored,
 // velocities and grasp weights zeroed, instrument points restored; the env's
@@ -17,7 +18,8 @@
 
 namespace ffsrl {
 
-constexpr int kServeBlock = 256;
+constexpr int kServeBlock = 1024;
+constexpr int kServeWaves = kServeBlock / 64;
 
 __global__ __launch_bounds__(kServeBlock) void k_apply_requests(ffsrl_batch b, ffsrl_requests r, float radius) {
   const int i = blockIdx.x;
@@ -64,74 +66,105 @@ __global__ __launch_bounds__(kServeBlock) void k_apply_requests(ffsrl_batch b, f
   }
 }
 
-// Block reduction in a fixed tree (deterministic for a given block size).
-template <typename Op>
-__device__ __forceinline__ float block_reduce(float v, float* sh, Op op) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int s = kServeBlock / 2; s > 0; s >>= 1) {
-    if (tid < s) sh[tid] = op(sh[tid], sh[tid + s]);
-    __syncthreads();
-  }
-  const float out = sh[0];
-  __syncthreads();
-  return out;
+// The per-vertex quantities an observation reduces, in three groups:
+//   sums    0-2 x, 3-5 v, 6-8 grasped x, 9 grasped count, 10 on-ground
+//           count, 11 displacement from the initial position;
+//   minima  12-14 x;
+//   maxima  15-17 x, 18 |v|, 19 displacement.
+constexpr int kObsSums = 12, kObsMins = 3, kObsQ = 20;
+
+__device__ __forceinline__ float obs_op(int k, float a, float c) {
+  return k < kObsSums ? a + c : k < kObsSums + kObsMins ? fmin_(a, c) : fmax_(a, c);
 }
 
+// One vertex into the thread's partial quantities.
+__device__ __forceinline__ void observe_vertex(float q[kObsQ], float4 x, float4 vel, float4 r0, bool grasped,
+                                               float ground) {
+  const float xs[3] = {x.x, x.y, x.z}, vs[3] = {vel.x, vel.y, vel.z};
+  const float d = length(xyz(x) - xyz(r0));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    q[k] += xs[k];
+    q[3 + k] += vs[k];
+    if (grasped) q[6 + k] += xs[k];
+    q[12 + k] = fmin_(q[12 + k], xs[k]);
+    q[15 + k] = fmax_(q[15 + k], xs[k]);
+  }
+  if (grasped) q[9] += 1.0f;
+  if (x.y <= ground) q[10] += 1.0f;
+  q[11] += d;
+  q[18] = fmax_(q[18], length(xyz(vel)));
+  q[19] = fmax_(q[19], d);
+}
+
+// Each thread takes the vertices v = tid + k * kServeBlock, four loads in
+// flight per step; every quantity is then reduced across the wavefront with
+// xor shuffles and across the wavefronts through LDS, in a fixed order, so an
+// env's observation does not depend on what else the launch holds.
 __global__ __launch_bounds__(kServeBlock) void k_observe(ffsrl_batch b, const int32_t* envs, const float* x0,
                                                          float ground, float* out) {
-  __shared__ float sh[kServeBlock];
+  __shared__ float sh[kObsQ][kServeWaves];
   const int env = envs[blockIdx.x];
   const ffsrl_env_desc e = b.envs[env];
   const int64_t vb = e.vbase;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   constexpr float kBig = 3.0e38f;
-  float s[3] = {0, 0, 0}, mn[3] = {kBig, kBig, kBig}, mx[3] = {-kBig, -kBig, -kBig}, sv[3] = {0, 0, 0};
-  float gs[3] = {0, 0, 0}, vmax = 0.0f, ng = 0.0f, nground = 0.0f, dmax = 0.0f, dsum = 0.0f;
-  for (int v = tid; v < e.nv; v += kServeBlock) {
-    const float4 x = ld4(b.x, vb + v), vel = ld4(b.v, vb + v), r0 = ld4(x0, vb + v);
-    const float xs[3] = {x.x, x.y, x.z}, vs[3] = {vel.x, vel.y, vel.z};
-    const bool grasped = b.drag[vb + v] != 0.0f;
+  float q[kObsQ];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      s[k] += xs[k];
-      mn[k] = fmin_(mn[k], xs[k]);
-      mx[k] = fmax_(mx[k], xs[k]);
-      sv[k] += vs[k];
-      if (grasped) gs[k] += xs[k];
+  for (int k = 0; k < kObsQ; ++k) q[k] = k < kObsSums ? 0.0f : k < kObsSums + kObsMins ? kBig : -kBig;
+  q[18] = 0.0f;  // speeds and displacements are >= 0
+  q[19] = 0.0f;
+  constexpr int U = 4;
+  for (int v0 = tid; v0 < e.nv; v0 += U * kServeBlock) {
+    float4 x[U], vel[U], r0[U];
+    float dr[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int v = v0 + u * kServeBlock;
+      if (v < e.nv) {
+        x[u] = ld4(b.x, vb + v);
+        vel[u] = ld4(b.v, vb + v);
+        r0[u] = ld4(x0, vb + v);
+        dr[u] = b.drag[vb + v];
+      }
     }
-    vmax = fmax_(vmax, length(xyz(vel)));
-    if (grasped) ng += 1.0f;
-    if (x.y <= ground) nground += 1.0f;
-    const float d = length(xyz(x) - xyz(r0));
-    dmax = fmax_(dmax, d);
-    dsum += d;
-  }
-  auto add = [](float a, float c) { return a + c; };
-  auto fmin2 = [](float a, float c) { return fmin_(a, c); };
-  auto fmax2 = [](float a, float c) { return fmax_(a, c); };
-  float o[32];
-  const float inv_n = e.nv > 0 ? 1.0f / (float)e.nv : 0.0f;
-  for (int k = 0; k < 12; ++k) o[k] = b.tips[env * 12 + k];
-  for (int k = 0; k < 3; ++k) {
-    o[12 + k] = block_reduce(s[k], sh, add) * inv_n;
-    o[15 + k] = block_reduce(mn[k], sh, fmin2);
-    o[18 + k] = block_reduce(mx[k], sh, fmax2);
-    o[21 + k] = block_reduce(sv[k], sh, add) * inv_n;
-  }
-  o[24] = block_reduce(vmax, sh, fmax2);
-  o[25] = block_reduce(ng, sh, add);
-  for (int k = 0; k < 3; ++k) {
-    const float g = block_reduce(gs[k], sh, add);
-    o[26 + k] = o[25] > 0.0f ? g / o[25] : 0.0f;
-  }
-  o[29] = block_reduce(nground, sh, add);
-  o[30] = block_reduce(dmax, sh, fmax2);
-  o[31] = block_reduce(dsum, sh, add) * inv_n;
-  if (tid == 0) {
 #pragma unroll
-    for (int k = 0; k < 32; ++k) out[(int64_t)env * 32 + k] = o[k];
+    for (int u = 0; u < U; ++u)
+      if (v0 + u * kServeBlock < e.nv) observe_vertex(q, x[u], vel[u], r0[u], dr[u] != 0.0f, ground);
+  }
+#pragma unroll
+  for (int k = 0; k < kObsQ; ++k) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) q[k] = obs_op(k, q[k], __shfl_xor(q[k], m));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kObsQ; ++k) sh[k][wave] = q[k];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    float r[kObsQ];
+#pragma unroll
+    for (int k = 0; k < kObsQ; ++k) r[k] = sh[k][0];
+    for (int w = 1; w < kServeWaves; ++w) {
+#pragma unroll
+      for (int k = 0; k < kObsQ; ++k) r[k] = obs_op(k, r[k], sh[k][w]);
+    }
+    const float inv_n = e.nv > 0 ? 1.0f / (float)e.nv : 0.0f;
+    float* o = out + (int64_t)env * 32;
+    for (int k = 0; k < 12; ++k) o[k] = b.tips[env * 12 + k];  // the four instrument points
+    for (int k = 0; k < 3; ++k) {
+      o[12 + k] = r[k] * inv_n;                           // centroid
+      o[15 + k] = r[12 + k];                              // bounding box min
+      o[18 + k] = r[15 + k];                              // bounding box max
+      o[21 + k] = r[3 + k] * inv_n;                       // mean velocity
+      o[26 + k] = r[9] > 0.0f ? r[6 + k] / r[9] : 0.0f;  // centroid of the grasped vertices
+    }
+    o[24] = r[18];          // max speed
+    o[25] = r[9];           // grasped vertices
+    o[29] = r[10];          // vertices on the ground
+    o[30] = r[19];          // max displacement from the initial positions
+    o[31] = r[11] * inv_n;  // mean displacement
   }
 }
 

@@ -103,7 +103,7 @@ class SceneServer:
     def __init__(self, classes: Sequence[TetMesh], capacity: Sequence[int], budget_bytes: float,
                  model_kwargs: dict, tips: np.ndarray, device="cuda:0", resolution: int = 64,
                  camera_pos=(0.0, 1.6, 5.5), look_at=(0.0, 0.45, 0.0), max_batch: int = 1024,
-                 max_inflight: int = 4, streams: int = 2, block: int = 1024, grasp_radius: float = 0.05,
+                 max_inflight: int = 2, streams: int = 2, block: int = 1024, grasp_radius: float = 0.05,
                  workspace: Optional[tuple] = None, shared: str = "auto"):
         self.device = torch.device(device)
         self.classes = list(classes)

@@ -96,6 +96,53 @@ def test_served_alone_equals_in_mix(pkg, gpu, shared):
               f"grasped {int(obs_last[25])}, max displacement {float(obs_last[30]):.4f}")
 
 
+def _observation_f64(x, v, drag, x0, tips, ground):
+    """The 32 observation scalars of one scene in float64 (the layout of
+    ffsrl_observe_envs)."""
+    x, v, x0 = (np.asarray(a, np.float64)[:, :3] for a in (x, v, x0))
+    g = drag != 0
+    d = np.linalg.norm(x - x0, axis=1)
+    o = np.zeros(32)
+    o[0:12] = tips
+    o[12:15] = x.mean(0)
+    o[15:18] = x.min(0)
+    o[18:21] = x.max(0)
+    o[21:24] = v.mean(0)
+    o[24] = np.linalg.norm(v, axis=1).max()
+    o[25] = g.sum()
+    o[26:29] = x[g].mean(0) if g.any() else 0.0
+    o[29] = (x[:, 1] <= ground).sum()
+    o[30] = d.max()
+    o[31] = d.mean()
+    return o
+
+
+def test_observation_scalars_vs_float64(pkg, gpu):
+    """Every slot's observation equals a float64 evaluation of the same state:
+    counts, minima and maxima exactly (the maxima of norms to float rounding),
+    sums and means within 2e-6 of the scene's scale."""
+    actors = list(range(6))
+    classes = {a: a % 2 for a in actors}
+    srv = _server(pkg, gpu)
+    _run(srv, _script(pkg, actors, seed=5), classes)
+    m = srv.model
+    obs = srv.obs.cpu().numpy()
+    x, v, x0 = m.x.cpu().numpy(), m.v.cpu().numpy(), srv.x0.cpu().numpy()
+    drag, tips = m.drag.cpu().numpy(), m.laparoscopeTip.cpu().numpy().reshape(-1, 12)
+    for a in actors:
+        slot = srv.actor_slot[a]
+        sl = m.env_slice(slot)
+        ref = _observation_f64(x[sl], v[sl], drag[sl], x0[sl], tips[slot], m.groundLevel)
+        got = obs[slot].astype(np.float64)
+        scale = max(1.0, float(np.abs(x[sl, :3]).max()))
+        exact = [*range(12), 15, 16, 17, 18, 19, 20, 25, 29]
+        assert np.array_equal(got[exact], ref[exact]), (a, got[exact] - ref[exact])
+        assert np.allclose(got[[24, 30]], ref[[24, 30]], rtol=1e-6, atol=0.0), (a, got[[24, 30]], ref[[24, 30]])
+        approx = [12, 13, 14, 21, 22, 23, 26, 27, 28, 31]
+        assert np.allclose(got[approx], ref[approx], rtol=0.0, atol=2e-6 * scale), (a, got[approx] - ref[approx])
+    assert obs[[srv.actor_slot[a] for a in actors], 25].max() > 0          # some scene holds a grasp
+
+
 def test_shared_ticks_equal_batched_ticks(pkg, gpu):
     """The same request stream served with the shared-env frame in every tick
     and with one workgroup per scene: identical observations."""
