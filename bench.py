@@ -247,8 +247,8 @@ def serving_run(pkg, wl, args, dev):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is by environment and is "
-                                   "described in DESIGN.md, not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; several GPUs would each step their own "
+                                   "share of the environments, which is not launched here", "n_gpus": args.gpus}))
         sys.exit(2)
     wl = importlib.import_module("ff-srl_amd.workloads")
     if args.workload == "serve":
