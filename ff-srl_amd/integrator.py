@@ -3,7 +3,8 @@
 ``stepModel`` mirrors ``SimIntegratorDO.stepModel`` (``FF_SRL/FF_SRL/
 integrator.py:1640-1744``): S substeps of predict, I iterations of the
 constraint projection with Jacobi averaging, the grasp pull and (with
-``collision``) the instrument collision, the ground clamp and the integrate.  Two launch forms run the same arithmetic:
+``collision``) the instrument collision, the ground clamp and the integrate.
+Three launch forms run the same arithmetic:
 
 * batched (default): ``ffsrl_frame``, one launch for all environments and
   the whole substep loop (and optionally the BVH refit);
