@@ -3,7 +3,9 @@
 The topology of each mesh is built on the host by the library's binned-SAH
 builder when the model is created (``SimModelDO`` holds it because the
 batched frame kernel refits in the same launch); this class exposes the
-reference's refit call, the node arrays and the ``.npz`` checkpoint format.
+reference's refit call, the node arrays and the ``.npz`` checkpoint format
+(``save=True`` writes it, ``load=True`` replaces the built topology with a
+saved one, as ``loadBVH`` does, ``bvh.py:176-197``).
 """
 from __future__ import annotations
 
@@ -18,7 +20,15 @@ class SimBVH:
     def __init__(self, device, simModel: SimModelDO, binsNumber: int = 8, save: bool = False, load: bool = False,
                  path: str = "bvh.npz"):
         if load:
-            raise NotImplementedError("the topology is built with the model; load .npz through SimBVH.load_topology")
+            # the reference shares one topology among its envs: the saved one
+            # goes to every mesh with the saved mesh's triangle count
+            topo = self.topology_from_npz(self.load_topology(path))
+            nf = len(topo["tri_map"])
+            meshes = [k for k, m in enumerate(simModel.meshes) if m.num_tris == nf]
+            if not meshes:
+                raise ValueError(f"{path}: no mesh of the model has {nf} triangles")
+            for k in meshes:
+                simModel.set_bvh(k, topo)
         self.device = device
         self.simModel = simModel
         self.binsNumber = binsNumber
@@ -65,6 +75,18 @@ class SimBVH:
         env = int(np.nonzero(self.simModel.env_mesh == mesh)[0][0])
         lo, hi = self.env_bounds(env)
         np.savez(path, t.tri_map, lo.cpu().numpy(), hi.cpu().numpy(), t.left, t.first, t.count, t.level)
+
+    @staticmethod
+    def topology_from_npz(arrays: dict) -> dict:
+        """The topology of a reference ``.npz`` BVH in the form ``MeshTopology``
+        takes: leaves get left child -1 (the reference leaves the constructor's
+        -1 there too, and marks an unsplit root with 0, ``bvh.py:108,480``)."""
+        count = np.asarray(arrays["triCount"], np.int32)
+        left = np.where(count > 0, -1, np.asarray(arrays["leftNode"], np.int32)).astype(np.int32)
+        level = np.asarray(arrays["level"], np.int32)
+        return {"left": left, "first": np.asarray(arrays["firstTriId"], np.int32), "count": count, "level": level,
+                "tri_map": np.asarray(arrays["triId"], np.int32), "num_nodes": int(count.size),
+                "max_level": int(level.max()) if level.size else 0}
 
     @staticmethod
     def load_topology(path: str):

@@ -495,6 +495,8 @@ class SimModelDO:
         b.sched = _ptr(self.sched)
         b.dp = _ptr(self.dp)
         b.cnt = _ptr(self.cnt)
+        b.split_acc = _ptr(getattr(self, "split_acc", None))
+        b.split_ctr = _ptr(getattr(self, "split_ctr", None))
         b.host_envs = self.env_desc_host.ctypes.data
         b.host_level_off = self.level_off_host.ctypes.data
         return b
@@ -586,6 +588,26 @@ class SimModelDO:
     def getLaparoscopeWorkspaceViolation(self):
         """``modelDO.py:1508-1512``."""
         return self.laparoscopeWorkspaceViolation[:self.numEnvs]
+
+    def set_bvh(self, mesh: int, bvh: dict) -> None:
+        """Replace mesh ``mesh``'s BVH topology with ``bvh`` (keys ``left``,
+        ``first``, ``count``, ``level``, ``tri_map``, ``num_nodes``,
+        ``max_level``; ``SimBVH(load=True)`` reads them from a reference
+        ``.npz``).  Every array that depends on the topology is rebuilt; the
+        simulation state is kept; the bounds are zero until the next refit."""
+        old = self.topo[mesh]
+        nf = self.meshes[mesh].num_tris
+        tri_map = np.asarray(bvh["tri_map"])
+        if tri_map.shape != (nf,) or not np.array_equal(np.sort(tri_map), np.arange(nf)):
+            raise ValueError(f"BVH triangle map does not permute the {nf} triangles of mesh {mesh}")
+        self.topo[mesh] = MeshTopology(self.meshes[mesh], 8, old.constraint_order, old.chunk, bvh=bvh)
+        self._build_topology()
+        self._build_envs()
+        dev = self.device
+        self.bvh_lo = torch.zeros((max(self.numNodes, 1), 4), dtype=torch.float32, device=dev)
+        self.bvh_hi = torch.zeros((max(self.numNodes, 1), 4), dtype=torch.float32, device=dev)
+        self._batch = self._make_batch()
+        self._order = None
 
     def enable_plain(self):
         """Allocate the dP / count scratch of the plain (per-kernel) form."""

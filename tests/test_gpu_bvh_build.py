@@ -88,3 +88,35 @@ def test_batched_build_timing(pkg, gpu):
         assert np.array_equal(d["left"][:h.num_nodes], h.left) and np.array_equal(d["tri_map"], h.tri_map)
     print(f"\n[bvh build] 256 meshes, {sum(m.num_tris for m in meshes)} triangles: device {t_dev * 1e3:.1f} ms "
           f"(incl. transfers), host SAH build (one thread, estimated from 64) {t_host * 1e3:.0f} ms")
+
+
+def test_simbvh_save_then_load(pkg, gpu, tmp_path):
+    """SimBVH(save=True) writes the reference .npz; SimBVH(load=True) on a
+    model whose own tree differs (built with 2 bins) installs the saved
+    topology on every env of that mesh: same node arrays, and after a frame
+    the same bounds and depth image as the model that wrote it."""
+    sc = pkg.scenes
+    mesh = sc.solid("ellipsoid", 6000, seed=31)
+    path = str(tmp_path / "bvh.npz")
+    a = pkg.SimModelDO([mesh], [0, 0, 0], device=gpu)
+    pkg.SimBVH(gpu, a, save=True, path=path)
+    b = pkg.SimModelDO([mesh], [0, 0, 0], device=gpu, bins=2)
+    assert not np.array_equal(b.topo[0].left, a.topo[0].left)      # a different tree to start from
+    pkg.SimBVH(gpu, b, load=True, path=path)
+    for k in ("left", "first", "count", "level", "tri_map", "level_nodes", "level_info", "level_off"):
+        assert np.array_equal(getattr(b.topo[0], k), getattr(a.topo[0], k)), k
+    integ = pkg.SimIntegratorDO(gpu)
+    for m in (a, b):
+        for _ in range(3):
+            integ.stepModel(m, refit=True)
+    torch.cuda.synchronize()
+    assert torch.equal(a.x, b.x) and torch.equal(a.bvh_lo, b.bvh_lo) and torch.equal(a.bvh_hi, b.bvh_hi)
+    depth = []
+    for m in (a, b):
+        ren = pkg.RaycastRendererDO(gpu, m, cameraPos=(0.0, 1.6, 5.5), resolution=32)
+        ren.lookAt((0.0, 0.3, 0.0))
+        depth.append(ren.render_depth().clone())
+    assert torch.equal(depth[0], depth[1]) and float((depth[0] > 0).float().mean()) > 0.05
+    with pytest.raises(ValueError):
+        c = pkg.SimModelDO([sc.solid("box", 2000, seed=3)], [0], device=gpu)
+        pkg.SimBVH(gpu, c, load=True, path=path)
