@@ -585,6 +585,32 @@ class SimModelDO:
         a[:, 4] = -e
         self.applyActions(a)
 
+    def forceLaparoscopeClampVertex(self, vertexId: int, envs, on: float = 1.0, animate: bool = True):
+        """``modelDO.py:1533-1544`` (``forceClampsDragKernel``, ``:282-289``):
+        grasp weight ``on`` for local vertex ``vertexId`` of every env with
+        envs[i] != 0.  ``animate`` is accepted and unused, as in the reference."""
+        ids = torch.nonzero(torch.as_tensor(envs, device=self.device).reshape(-1)).flatten().cpu().numpy()
+        for i in ids:
+            nv = int(self.env_vbase[i + 1] - self.env_vbase[i])
+            if not 0 <= int(vertexId) < nv:
+                raise IndexError(f"vertex {vertexId} outside env {int(i)} ({nv} vertices)")
+            self.drag[int(self.env_vbase[i]) + int(vertexId)] = float(on)
+
+    def resetStochasticModel(self, laparoscopeTranslation, laparoscopeRotation):
+        """``modelDO.py:1624-1645``: every env's positions and instrument back
+        to their initial state (clamp state cleared), then the instrument
+        re-aimed at the given rotation centre (``setRotationCenter``).  As in
+        the reference, velocities, predicted positions and grasp weights are
+        left as they are."""
+        self.x.copy_(self.initialVertex)
+        self.laparoscopeTip.copy_(self.initialTips)
+        self.laparoscopeBase.copy_(self.initialBases)
+        if self._instrument is not None:
+            self.laparoscopeXForm.copy_(self.laparoscopeInitialXForm)
+            self.laparoscopeDragCutHeat.zero_()
+            self.laparoscopeDragCutHeatUpdate.zero_()
+            self.setRotationCenter(laparoscopeTranslation, laparoscopeRotation)
+
     def getLaparoscopeWorkspaceViolation(self):
         """``modelDO.py:1508-1512``."""
         return self.laparoscopeWorkspaceViolation[:self.numEnvs]

@@ -111,6 +111,26 @@ class RaycastRendererDO:
         self.cameraRot[0] = math.atan2(d[0], d[2]) - math.pi
         self._update_camera()
 
+    # Camera controls of the reference (render.py:1163-1184).  Position and
+    # angles accumulate in float32, as the reference's wp.vec3 fields do.
+    def _move(self, delta) -> None:
+        self.cameraPos = (self.cameraPos.astype(np.float32) + np.asarray(delta, np.float32)).astype(np.float64)
+        self.camera.origin[:] = tuple(float(c) for c in self.cameraPos.astype(np.float32))
+
+    def moveCameraForward(self, distance: float = 0.1) -> None:
+        self._move((0.0, 0.0, -distance))
+
+    def moveCameraLeft(self, distance: float = 0.1) -> None:
+        self._move((-distance, 0.0, 0.0))
+
+    def moveCameraUp(self, distance: float = 0.1) -> None:
+        self._move((0.0, distance, 0.0))
+
+    def rotateCameraHV(self, distanceX: float = 5.0, distanceY: float = 5.0, scale: float = 0.1) -> None:
+        inc = np.array([distanceX * scale * math.pi / 180.0, 0.0, distanceY * scale * math.pi / 180.0], np.float32)
+        self.cameraRot = (self.cameraRot.astype(np.float32) + inc).astype(np.float64)
+        self._update_camera()
+
     def set_mode(self, render_mode: int) -> None:
         self.render_mode = render_mode
         self.camera.mode = int(render_mode)

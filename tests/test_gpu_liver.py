@@ -120,3 +120,40 @@ def test_liver_retraction(pkg, orc, gpu):
     torch.cuda.synchronize()
     assert model.getLaparoscopeWorkspaceViolation().cpu().numpy().tolist() == oi.violation.tolist() == [1] * n
     check("workspace move")
+
+
+def test_stochastic_reset_and_forced_clamp_vertex(pkg, gpu):
+    """resetStochasticModel (modelDO.py:1624-1645): positions and instrument
+    back to their initial state, clamp state cleared, then re-aimed at the
+    given rotation centre; velocities are left as they are (as in the
+    reference).  forceLaparoscopeClampVertex (modelDO.py:1533-1544): grasp
+    weight of one local vertex in the selected envs only."""
+    n = 3
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(gpu)
+    model = pkg.SimModelDO.from_usd(FIXTURE, n, device=gpu, **REF)
+    integ = pkg.SimIntegratorDO(gpu)
+    model.applyCartesianActions(dev(np.tile([0.3, -0.4, 0.2], (n, 1))))
+    close = np.zeros((n, 5), np.float32)
+    close[:, 4] = -0.5
+    model.applyActions(dev(close))
+    for _ in range(3):
+        integ.stepModel(model)
+    free = int(np.nonzero(model.scene.meshes[0].inv_mass > 0)[0][0])
+    model.forceLaparoscopeClampVertex(free, torch.tensor([1, 0, 1]), 1.0)
+    torch.cuda.synchronize()
+    d = model.drag.cpu().numpy()
+    vb = model.env_vbase
+    assert d[vb[0] + free] == 1.0 and d[vb[2] + free] == 1.0
+    with pytest.raises(IndexError):
+        model.forceLaparoscopeClampVertex(10 ** 6, torch.tensor([1, 0, 0]))
+    v_before = model.v.clone()
+    assert not torch.equal(model.x, model.initialVertex)
+    model.resetStochasticModel(dev(np.tile(RCM, (n, 1))), dev(np.tile(RCM_ROT, (n, 1))))
+    fresh = pkg.SimModelDO.from_usd(FIXTURE, n, device=gpu, **REF)
+    fresh.setRotationCenter(dev(np.tile(RCM, (n, 1))), dev(np.tile(RCM_ROT, (n, 1))))
+    torch.cuda.synchronize()
+    assert torch.equal(model.x, model.initialVertex) and torch.equal(model.v, v_before)
+    assert torch.equal(model.laparoscopeXForm, fresh.laparoscopeXForm)
+    assert torch.equal(model.laparoscopeTip, fresh.laparoscopeTip)
+    assert torch.equal(model.laparoscopeBase, fresh.laparoscopeBase)
+    assert int(model.laparoscopeDragCutHeat.abs().sum()) == 0
