@@ -76,6 +76,28 @@ class SimBVH:
         lo, hi = self.env_bounds(env)
         np.savez(path, t.tri_map, lo.cpu().numpy(), hi.cpu().numpy(), t.left, t.first, t.count, t.level)
 
+    def printBVH(self, mesh: int = 0) -> None:
+        """Text picture of mesh ``mesh``'s tree as ``printBVH`` draws it
+        (``bvh.py:199-228``): one row per level, each node's first triangle
+        at its column (children at the parent's column -+ 2^(h-2)), blanks
+        elsewhere."""
+        t = self.simModel.topo[mesh]
+        height = t.max_level + 1
+        width = 2 ** height
+        grid = [[-1] * width for _ in range(height)]
+        stack = [(0, width // 2, 0, height)]
+        while stack:
+            node, col, row, h = stack.pop()
+            if h < 1:
+                continue
+            grid[row][col] = int(t.first[node])
+            if t.left[node] > -1:
+                step = 2 ** (h - 2) if h >= 2 else 0
+                stack.append((int(t.left[node]) + 1, col + step, row + 1, h - 1))
+                stack.append((int(t.left[node]), col - step, row + 1, h - 1))
+        for r in grid:
+            print(" ".join(" " if v == -1 else str(v) for v in r))
+
     @staticmethod
     def topology_from_npz(arrays: dict) -> dict:
         """The topology of a reference ``.npz`` BVH in the form ``MeshTopology``

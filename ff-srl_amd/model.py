@@ -596,6 +596,12 @@ class SimModelDO:
                 raise IndexError(f"vertex {vertexId} outside env {int(i)} ({nv} vertices)")
             self.drag[int(self.env_vbase[i]) + int(vertexId)] = float(on)
 
+    def transformSimMeshData(self) -> None:
+        """``modelDO.py:1441-1444`` copies simulation vertices into the render
+        point arrays.  Here the render triangles index the simulation
+        vertices directly, so the renderer always sees the current positions
+        and there is nothing to copy."""
+
     def resetStochasticModel(self, laparoscopeTranslation, laparoscopeRotation):
         """``modelDO.py:1624-1645``: every env's positions and instrument back
         to their initial state (clamp state cleared), then the instrument

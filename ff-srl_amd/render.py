@@ -176,6 +176,19 @@ class RaycastRendererDO:
             return self.render_depth().unsqueeze(-1).expand(-1, -1, -1, 3)
         return self.render_rgb()
 
+    def render(self, simBVH=None, simModel=None, useGraph: bool = False) -> torch.Tensor:
+        """The reference's per-environment form (``render.py:1024-1097``): one
+        launch per environment; the same images as ``renderNew``."""
+        n = self.simModel.numEnvs
+        for i in range(n):
+            if self.render_mode == RenderMode.depth_map:
+                self.render_depth(i, i + 1)
+            else:
+                self.render_rgb(i, i + 1)
+        if self.render_mode == RenderMode.depth_map:
+            return self.depth.unsqueeze(-1).expand(-1, -1, -1, 3)
+        return self.pixels
+
 
 # The reference's class name, for drop-in use.
 WarpRaycastRendererDO = RaycastRendererDO

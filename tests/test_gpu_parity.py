@@ -478,10 +478,14 @@ def test_rgb_modes_and_normals_bit_exact(pkg, orc, gpu, mode):
         print(f"\n[rgb mode {mode}] env {e}: lit pixels {lit:.2f}, max |gpu-oracle| {np.abs(got - ref).max():.2e}")
         assert lit > 0.05
         assert np.array_equal(got, ref)
-    # RGB-D from one traversal: the same colour image and the depth-mode image
+    # the per-env form (one launch per env) gives the same image
     rgb = rgb.clone()
+    per_env = ren.render().clone()
+    assert torch.equal(per_env, rgb)
+    # RGB-D from one traversal: the same colour image and the depth-mode image
     pix, dep = ren.render_rgbd()
     pix, dep = pix.clone(), dep.clone()
-    depth = ren.render_depth()
+    depth = ren.render_depth().clone()
+    assert torch.equal(ren.render()[..., 0], depth)      # depth mode, per env
     torch.cuda.synchronize()
     assert torch.equal(pix, rgb) and torch.equal(dep, depth) and float((depth > 0).float().mean()) > 0.05
