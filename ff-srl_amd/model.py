@@ -241,6 +241,21 @@ class SimModelDO:
                  capsuleBases: Optional[np.ndarray] = None, capsuleRadii: Optional[np.ndarray] = None,
                  laparoscopes=None, workspaceLow=(-1e5, -1e5, -1e5), workspaceHigh=(1e5, 1e5, 1e5),
                  bvh_on_device: bool = False):
+        """``meshes`` and ``env_mesh``: the simulated meshes and each env's
+        mesh.  The reference's form ``SimModelDO(stage, numEnvs, device, ...)``
+        (``modelDO.py:774``) is accepted too: a USD crate scene path (or its
+        converted ``.npz``), or an opened stage (its root layer's file), and
+        the number of envs; every env then holds the scene's meshes and
+        laparoscope (``self.scene`` keeps the parsed scene)."""
+        self.scene = None
+        if isinstance(meshes, (str, os.PathLike)) or hasattr(meshes, "GetRootLayer"):
+            from .usd_scene import merged_mesh, scene
+            path = meshes.GetRootLayer().realPath if hasattr(meshes, "GetRootLayer") else os.fspath(meshes)
+            self.scene = scene(path)
+            n = int(env_mesh)
+            if laparoscopes is None and self.scene.laparoscope is not None:
+                laparoscopes = [self.scene.laparoscope] * n
+            meshes, env_mesh = [merged_mesh(self.scene)], [0] * n
         self.device = torch.device(device)
         self.meshes = list(meshes)
         env_mesh = list(range(len(self.meshes))) if env_mesh is None else list(env_mesh)
@@ -302,12 +317,7 @@ class SimModelDO:
         (``modelDO.py:774-1266``) for a USD crate scene or its converted
         ``.npz``: every env holds the scene's simulated meshes (merged into one
         mesh, lock-box pins applied) and its laparoscope."""
-        from .usd_scene import merged_mesh, scene
-        sc = scene(path)
-        laps = None if sc.laparoscope is None else [sc.laparoscope] * int(numEnvs)
-        model = cls([merged_mesh(sc)], [0] * int(numEnvs), device=device, laparoscopes=laps, **kw)
-        model.scene = sc
-        return model
+        return cls(path, int(numEnvs), device=device, **kw)
 
     # ------------------------------------------------------------------ setup
     def _make_params(self) -> Params:

@@ -16,6 +16,7 @@ import os
 import struct
 
 import numpy as np
+import torch
 import pytest
 
 pkg = importlib.import_module("ff-srl_amd")
@@ -137,3 +138,26 @@ def test_merged_mesh_offsets_ids():
     assert m.num_vertices == 2 * n and m.num_tets == 2 * sc.meshes[0].num_tets
     assert np.array_equal(m.tets[sc.meshes[0].num_tets:], sc.meshes[0].tets + n)
     assert np.array_equal(m.tris[sc.meshes[0].num_tris:], sc.meshes[0].tris + n)
+
+
+def test_model_from_scene_path_like_the_reference():
+    """SimModelDO(stage-or-path, numEnvs, device, ...) as the reference calls it
+    (modelDO.py:774): every env holds the scene's mesh and laparoscope."""
+    m = pkg.SimModelDO(FIXTURE, 3, "cpu", simSubsteps=8)
+    sc = us.load_scene(FIXTURE)
+    assert m.numEnvs == 3 and m.scene is not None and m.simSubsteps == 8
+    assert m.meshes[0].num_vertices == sc.meshes[0].num_vertices and list(m.env_mesh) == [0, 0, 0]
+    assert m.laparoscopeXForm.shape[0] == 7 * 3
+
+    class Layer:
+        realPath = FIXTURE
+
+    class Stage:                      # what pxr's Usd.Stage.Open returns, as far as the model reads it
+        def GetRootLayer(self):
+            return Layer()
+
+    m2 = pkg.SimModelDO(Stage(), 2, "cpu")
+    assert m2.numEnvs == 2 and np.array_equal(m2.x.numpy()[:m.meshes[0].num_vertices],
+                                               m.x.numpy()[:m.meshes[0].num_vertices])
+    m3 = pkg.SimModelDO.from_usd(FIXTURE, 2, device="cpu")
+    assert torch.equal(m3.x, m2.x)
