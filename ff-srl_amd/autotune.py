@@ -41,7 +41,9 @@ def shared_candidates(model: SimModelDO) -> tuple:
     """(False,) or (False, True): the shared-env frame is a candidate when the
     batch has at most half as many envs as the device has CUs."""
     cus = torch.cuda.get_device_properties(model.device).multi_processor_count
-    return (False, True) if 2 * model.numEnvs <= cus else (False,)
+    if 2 * model.numEnvs > cus or lib.ffsrl_xcd_count() <= 0:   # no XCD map: the shared frame is unavailable
+        return (False,)
+    return (False, True)
 
 
 def _bucket(x: float) -> int:

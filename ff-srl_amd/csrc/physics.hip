@@ -508,7 +508,7 @@ static int launch_frame(const ffsrl_batch& b, const ffsrl_params& prm, const int
 // ---------------------------------------------------------------------------
 // Shared-env frame (ffsrl_frame_shared): the workgroups of one XCD share the
 // frames of that XCD's environments (list slot el belongs to XCD el % nxcd,
-// by the hardware id each workgroup reads).
+// XCDs numbered densely from the hardware ids each workgroup reads).
 //
 // The frame of an env is a fixed sequence of passes (predict; then per
 // substep and iteration: projection, apply [, collision, collision apply]),
@@ -759,12 +759,17 @@ __device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, int env
   }
 }
 
+// Hardware XCD id (0..15) -> dense index 0..nxcd-1 (-1: no XCD of that id).
+struct XcdMap {
+  int8_t index[16];
+};
+
 template <int BLOCK, bool CARRY, bool COLLIDE>
 __global__ __launch_bounds__(BLOCK) void k_frame_shared(ffsrl_batch b, ffsrl_params prm, const int32_t* list, int n,
-                                                        int refit, int nxcd) {
+                                                        int refit, int nxcd, XcdMap map) {
   extern __shared__ unsigned long long lds[];
-  const int x = xcc_id();
-  const int mine = (x < nxcd && x < n) ? (n - 1 - x) / nxcd + 1 : 0;  // slots x, x + nxcd, ...
+  const int x = map.index[xcc_id() & 15];
+  const int mine = (x >= 0 && x < nxcd && x < n) ? (n - 1 - x) / nxcd + 1 : 0;  // slots x, x + nxcd, ...
   const int rank = (int)blockIdx.x / nxcd;                           // spreads the XCD's workgroups
   for (int k = 0; k < mine; ++k) {
     const int el = x + nxcd * ((rank + k) % mine);
@@ -795,7 +800,7 @@ __global__ void k_xcc_probe(int32_t* out) {
 
 template <int BLOCK>
 static int launch_shared(const ffsrl_batch& b, const ffsrl_params& prm, const int32_t* list, int n, int refit,
-                         bool carry, int lds, int nxcd, hipStream_t st) {
+                         bool carry, int lds, int nxcd, const XcdMap& map, hipStream_t st) {
   auto kern = prm.collide ? (carry ? k_frame_shared<BLOCK, true, true> : k_frame_shared<BLOCK, false, true>)
                           : (carry ? k_frame_shared<BLOCK, true, false> : k_frame_shared<BLOCK, false, false>);
   FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -805,7 +810,7 @@ static int launch_shared(const ffsrl_batch& b, const ffsrl_params& prm, const in
   FFSRL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   FFSRL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), BLOCK, lds));
   const int grid = std::max(nxcd, cus * std::max(1, per_cu));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), lds, st, b, prm, list, n, refit, nxcd);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), lds, st, b, prm, list, n, refit, nxcd, map);
   FFSRL_HIP(hipGetLastError());
   return 0;
 }
@@ -1110,10 +1115,11 @@ int ffsrl_frame_envs(const ffsrl_batch* b, const ffsrl_params* prm, const int32_
   return frame(b, prm, envs, n, block, refit, nullptr, stream);
 }
 
-// XCD count of each device, probed once: the hardware XCD ids seen by one
-// workgroup per CU must be 0 .. count-1.
+// XCD count of each device, probed once with one workgroup per CU; the
+// hardware ids seen are numbered densely (XcdMap) for the shared frame.
 static std::mutex g_xcd_mu;
 static int g_xcd[64];
+static XcdMap g_xcd_map[64];
 
 int ffsrl_xcd_count(void) {
   int dev = 0, cus = 0;
@@ -1131,16 +1137,17 @@ int ffsrl_xcd_count(void) {
   hipFree(d);
   FFSRL_HIP(e);
   bool seen[16] = {};
-  int mx = -1;
   for (int v : h) {
     if (v < 0 || v >= 16) return fail(FFSRL_E_LIMIT, "XCD id %d outside 0..15", v);
     seen[v] = true;
-    mx = std::max(mx, v);
   }
-  for (int i = 0; i <= mx; ++i)
-    if (!seen[i]) return fail(FFSRL_E_LIMIT, "XCD ids are not 0..%d (no workgroup on XCD %d)", mx, i);
-  g_xcd[dev] = mx + 1;
-  return mx + 1;
+  // dense numbering of the ids seen (a partitioned device may report one id, e.g. 3)
+  XcdMap map;
+  int n = 0;
+  for (int i = 0; i < 16; ++i) map.index[i] = seen[i] ? (int8_t)n++ : (int8_t)-1;
+  g_xcd_map[dev] = map;
+  g_xcd[dev] = n;
+  return n;
 }
 
 int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
@@ -1151,13 +1158,20 @@ int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int3
   if (n == 0) return 0;
   const int nxcd = ffsrl_xcd_count();
   if (nxcd <= 0) return nxcd < 0 ? nxcd : fail(FFSRL_E_LIMIT, "no XCD found");
+  int dev = 0;
+  FFSRL_HIP(hipGetDevice(&dev));
+  XcdMap map;
+  {
+    std::lock_guard<std::mutex> lock(g_xcd_mu);
+    map = g_xcd_map[dev];
+  }
   const bool carry = carries_lambda(*prm);
   const int lds = lds_bytes_for(b->max_nv < 1 ? 1 : b->max_nv);
   hipStream_t st = as_stream(stream);
   switch (block) {
-    case 256: return launch_shared<256>(*b, *prm, envs, n, refit, carry, lds, nxcd, st);
-    case 512: return launch_shared<512>(*b, *prm, envs, n, refit, carry, lds, nxcd, st);
-    case 1024: return launch_shared<1024>(*b, *prm, envs, n, refit, carry, lds, nxcd, st);
+    case 256: return launch_shared<256>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, st);
+    case 512: return launch_shared<512>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, st);
+    case 1024: return launch_shared<1024>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, st);
     default: return fail(FFSRL_E_ARG, "block must be 256, 512 or 1024 (got %d)", block);
   }
 }

@@ -128,6 +128,8 @@ class SceneServer:
         self.shared = shared
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         self.shared_max = {"auto": cus // 2, "on": self.n_slots, "off": 0}[shared]
+        if shared == "auto" and lib.ffsrl_xcd_count() <= 0:   # no XCD map: one workgroup per scene
+            self.shared_max = 0
         if self.shared_max > 0:
             m.enable_shared()
         self.radius = float(grasp_radius)
