@@ -523,10 +523,12 @@ static int launch_frame(const ffsrl_batch& b, const ffsrl_params& prm, const int
 // vertices.  Integer sums do not depend on the order of the additions, so the
 // result equals the one-workgroup frame bit for bit.
 //
-// Progress: a workgroup waits only after it has counted every chunk it holds,
-// and only running workgroups hold chunks, so every pass completes whatever
-// number of workgroups is resident.  Waits are bounded anyway
-// (kShareSpinLimit); a frame that could not complete sets ctr[2].
+// Progress: a workgroup waits only for the passes before its current chunk,
+// and only after it has counted every chunk it has finished; only running
+// workgroups hold chunks, so the lowest chunk not yet counted always belongs
+// to a workgroup that can run it, and every pass completes whatever number
+// of workgroups is resident.  Waits are bounded anyway (kShareSpinLimit); a
+// frame that could not complete sets ctr[2].
 //
 // Coherence: all workgroups of an env run on one XCD and share its L2.
 // Stores and atomics have reached the L2 when `s_waitcnt vmcnt(0)` returns
