@@ -135,3 +135,43 @@ def test_shared_collision(pkg, gpu):
     _equal(a, b, ("laparoscopeInCollision",))
     assert int(b.laparoscopeInCollision.sum()) > 0
     _clean(b)
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_frame_graph_replays_equal_eager(pkg, gpu, shared):
+    """One frame (batched or shared) captured in a HIP graph and replayed four
+    times, with eager work and synchronisations between the replays, gives
+    after every replay the state of the same number of eager frames (the
+    shared frame resets its own chunk counters; nothing in it is a memset)."""
+    import ctypes as C
+    meshes = _meshes(pkg, 8, 1000, 12000, seed=41)
+    a, b = _pair(pkg, gpu, meshes, NH_DV)
+    integ = pkg.SimIntegratorDO(gpu, shared=shared)
+    lib = pkg.lib
+    if shared:
+        b.enable_shared()                      # allocation and the XCD probe stay outside the capture
+    order = b.dispatch_order()
+    torch.cuda.synchronize()
+    g = C.c_void_p()
+    cs = torch.cuda.Stream()
+    with torch.cuda.stream(cs):
+        lib.ffsrl_graph_begin(cs.cuda_stream)
+        if shared:
+            lib.ffsrl_frame_shared(b.batch, b.params, order.data_ptr(), b.numEnvs, 1024, 1, cs.cuda_stream)
+        else:
+            lib.ffsrl_frame(b.batch, b.params, order.data_ptr(), 1024, 1, cs.cuda_stream)
+        lib.ffsrl_graph_end(cs.cuda_stream, C.byref(g))
+    torch.cuda.synchronize()
+    assert torch.equal(a.x, b.x)                   # capturing ran nothing
+    try:
+        for k in range(4):
+            integ.stepModel(a, refit=True)
+            with torch.cuda.stream(cs):
+                lib.ffsrl_graph_launch(g, cs.cuda_stream)
+            torch.cuda.synchronize()
+            _ = float(b.x[:, :3].sum())                # eager work between the replays
+            _equal(a, b)
+            if shared:
+                _clean(b)
+    finally:
+        lib.ffsrl_graph_destroy(g)
