@@ -115,6 +115,13 @@ def test_argument_errors_are_reported(pkg):
         pkg.lib.ffsrl_apply_requests(b, r, 0.1, None)
     with pytest.raises(pkg.FFSRLError, match="NULL argument"):
         pkg.lib.ffsrl_observe_envs(b, None, 2, None, 0.0, None, None)
+    # the shared-env frame checks its scratch and counts before any device call
+    with pytest.raises(pkg.FFSRLError, match="split_acc and split_ctr"):
+        pkg.lib.ffsrl_frame_shared(b, prm, dummy.ctypes.data, 2, 1024, 0, None)
+    with pytest.raises(pkg.FFSRLError, match="5 environments listed"):
+        pkg.lib.ffsrl_frame_shared(b, prm, dummy.ctypes.data, 5, 1024, 0, None)
+    b.split_acc = b.split_ctr = dummy.ctypes.data
+    assert pkg.lib.ffsrl_frame_shared(b, prm, None, 0, 1024, 0, None) == 0      # nothing listed: no launch
 
 
 def test_frame_rejects_envs_beyond_lds(pkg):
