@@ -45,7 +45,9 @@ int check_hip(hipError_t e, const char* what) {
 // LDS budget of one workgroup on gfx950.
 constexpr int kLdsBytes = 160 * 1024;
 constexpr int lds_bytes_for(int nv) { return nv * 16; }
-constexpr int kMaxFrameVertices = (kLdsBytes - 256) / 16;  // 256 B of static LDS for the scheduler
+// 512 B are left for the kernels' static LDS (scheduler and pass words, the
+// refit's 65 group offsets: 272-288 B today)
+constexpr int kMaxFrameVertices = (kLdsBytes - 512) / 16;
 
 // Jacobi sums in LDS are fixed point: each delta component is rounded to the
 // nearest multiple of the env's quantum q = 2^dp_exp (about 2^-27 of the mesh
@@ -110,7 +112,7 @@ struct EdgeIn {
 };
 
 // The frame reads the 16-bit packed ids (half the index bytes of int4 / int2;
-// an env of the batched frame has at most 10,240 vertices).
+// an env of the batched frame has at most 10,208 vertices).
 __device__ __forceinline__ TetIn load_tet(const ffsrl_batch& b, int64_t ti) {
   TetIn t;
   const uint64_t w = b.tets16[ti];
@@ -358,7 +360,7 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
 
   for (int i = tid; i < 2 * nv; i += BLOCK) c.acc[i] = 0ull;
   // predict of substep 0; each thread notes which of its vertices (i = tid +
-  // k * BLOCK, k < 64 since nv <= 10,224) are grasped, so the apply phases
+  // k * BLOCK, k < 64 since nv <= 10,208) are grasped, so the apply phases
   // read the grasp weight of those only
   uint64_t grasped = 0;
   for (int i = tid, k = 0; i < nv; i += BLOCK, ++k) {

@@ -35,7 +35,7 @@ def _pack16(ids: np.ndarray) -> np.ndarray:
     """Rows of 4 (tets) or 2 (edges) mesh-local ids packed as 16-bit fields,
     first id in the low bits: int64 [n] for 4 ids, int32 [n] for 2.  Ids of
     meshes with more than 65,536 vertices are stored modulo 2^16 (such meshes
-    exceed the batched frame's 10,224-vertex limit and use the plain form)."""
+    exceed the batched frame's 10,208-vertex limit and use the plain form)."""
     ids = np.asarray(ids).astype(np.uint64) & np.uint64(0xffff)
     if ids.ndim != 2 or ids.shape[1] not in (2, 4):
         raise ValueError("expected (n, 2) or (n, 4) ids")

@@ -175,3 +175,35 @@ def test_frame_graph_replays_equal_eager(pkg, gpu, shared):
                 _clean(b)
     finally:
         lib.ffsrl_graph_destroy(g)
+
+
+def test_env_at_the_vertex_limit(pkg, gpu):
+    """An env with exactly ffsrl_frame_max_vertices() vertices runs in the
+    batched and the shared frame (its LDS and the kernels' static LDS fit in
+    160 KB): a 21^3 cube grid padded with pinned, isolated vertices inside its
+    own box (same diagonal, so the same fixed-point quantum) evolves exactly
+    like the cube alone."""
+    sc = pkg.scenes
+    cube = sc.cube_grid(21)
+    limit = pkg.lib.ffsrl_frame_max_vertices()
+    pad = limit - cube.num_vertices
+    assert pad > 0
+    verts = np.concatenate([cube.verts, np.full((pad, 3), 0.5, np.float32)])
+    inv_mass = np.concatenate([cube.inv_mass, np.zeros(pad, np.float32)])
+    padded = sc._finish("padded-cube", verts, cube.tets, inv_mass)
+    assert padded.num_vertices == limit and np.array_equal(padded.tris, cube.tris)
+    kw = dict(NH_DV, neoHookean=False)
+    for shared in (False, True):
+        ref = pkg.SimModelDO([cube], [0], device=gpu, **kw)
+        big = pkg.SimModelDO([padded], [0], device=gpu, **kw)
+        integ = pkg.SimIntegratorDO(gpu, shared=shared)
+        for _ in range(3):
+            integ.stepModel(ref, refit=True)
+            integ.stepModel(big, refit=True)
+        torch.cuda.synchronize()
+        n = cube.num_vertices
+        assert torch.equal(big.x[:n], ref.x) and torch.equal(big.v[:n], ref.v), shared
+        assert torch.equal(big.bvh_lo, ref.bvh_lo) and torch.equal(big.bvh_hi, ref.bvh_hi), shared
+        assert not torch.equal(ref.x, ref.initialVertex)          # the cube moved
+        if shared:
+            _clean(big)
