@@ -4,7 +4,8 @@
 //
 // Arithmetic policy Ar<FAST>:
 //   FAST = false: IEEE division and square root, the reference's operation
-//                 order (the plain form);
+//                 order (Neo-Hookean: up to the order of additions; the
+//                 plain form);
 //   FAST = true:  hardware reciprocal / square root / reciprocal square root
 //                 (<= 1 ulp), multiplications by reciprocals, squared lengths
 //                 taken as dot products (the batched frame).
@@ -143,97 +144,80 @@ __host__ __device__ inline NHConst nh_const(float volC, float devC, float dT) {
   return c;
 }
 
-// m = row-major 3x3; F = Ds * M with Ds = columns (p1, p2, p3); k summed 0,1,2.
+// M = row-major 3x3; F = Ds * M with Ds = columns (p1, p2, p3).
 // neoHookeanConstraints (integrator.py:578-698): hydrostatic det F - 1 - gamma
 // with gradient rows of transpose(mat33(F1xF2, F2xF0, F0xF1) * M^T), then
 // deviatoric |F|_F with gradient rows of transpose(F * M^T / |F|); both
-// deltas from the same positions; lambda is not carried.
+// deltas from the same positions; lambda is not carried.  The gradient of
+// each vertex is formed when it is used (weights, then the summed delta)
+// instead of as two 3x3 matrices, which keeps the batched frame within its
+// 128 registers; the arithmetic is the reference's up to the order of the
+// additions.
 template <bool FAST>
 __device__ __forceinline__ int neohookean(const float4 P[4], float rest, const float M[9], const NHConst& k,
                                           f3 d[4]) {
   using A = Ar<FAST>;
-  f3 a = xyz(P[0]);
-  f3 p1 = xyz(P[1]) - a, p2 = xyz(P[2]) - a, p3 = xyz(P[3]) - a;
-  const float mA = P[0].w, mB = P[1].w, mC = P[2].w, mD = P[3].w;
-  // Ds columns p1,p2,p3 -> Ds[i][j] = p_{j+1}[i]
-  const float Ds[9] = {p1.x, p2.x, p3.x, p1.y, p2.y, p3.y, p1.z, p2.z, p3.z};
-  float F[9];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      float s = 0.0f;
-      s = s + Ds[3 * i + 0] * M[0 * 3 + j];
-      s = s + Ds[3 * i + 1] * M[1 * 3 + j];
-      s = s + Ds[3 * i + 2] * M[2 * 3 + j];
-      F[3 * i + j] = s;
-    }
-  f3 F0 = mk(F[0], F[1], F[2]), F1 = mk(F[3], F[4], F[5]), F2 = mk(F[6], F[7], F[8]);
-  f3 c1 = cross(F1, F2), c2 = cross(F2, F0), c3 = cross(F0, F1);
-  // dF = columns (c1, c2, c3); g = dF * M^T ; rows of g^T = columns of g.
-  const float dF[9] = {c1.x, c2.x, c3.x, c1.y, c2.y, c3.y, c1.z, c2.z, c3.z};
-  float g[9];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      float s = 0.0f;
-      s = s + dF[3 * i + 0] * M[j * 3 + 0];
-      s = s + dF[3 * i + 1] * M[j * 3 + 1];
-      s = s + dF[3 * i + 2] * M[j * 3 + 2];
-      g[3 * i + j] = s;
-    }
-  f3 g1 = mk(g[0], g[3], g[6]), g2 = mk(g[1], g[4], g[7]), g3 = mk(g[2], g[5], g[8]);
-  f3 g0 = ((-g1) - g2) - g3;
-  float volume = dot(F0, cross(F1, F2));
-  float C = volume - 1.0f - k.gamma;
+  const f3 a = xyz(P[0]);
+  const f3 p1 = xyz(P[1]) - a, p2 = xyz(P[2]) - a, p3 = xyz(P[3]) - a;
+  const float m[4] = {P[0].w, P[1].w, P[2].w, P[3].w};
+  // rows of F = Ds * M (Ds columns p1, p2, p3)
+  f3 F[3];
+  F[0] = mk(p1.x * M[0] + p2.x * M[3] + p3.x * M[6], p1.x * M[1] + p2.x * M[4] + p3.x * M[7],
+            p1.x * M[2] + p2.x * M[5] + p3.x * M[8]);
+  F[1] = mk(p1.y * M[0] + p2.y * M[3] + p3.y * M[6], p1.y * M[1] + p2.y * M[4] + p3.y * M[7],
+            p1.y * M[2] + p2.y * M[5] + p3.y * M[8]);
+  F[2] = mk(p1.z * M[0] + p2.z * M[3] + p3.z * M[6], p1.z * M[1] + p2.z * M[4] + p3.z * M[7],
+            p1.z * M[2] + p2.z * M[5] + p3.z * M[8]);
+  const f3 c1 = cross(F[1], F[2]), c2 = cross(F[2], F[0]), c3 = cross(F[0], F[1]);
+  // gradient column of vertex j+1: hydrostatic g_j = c1 M[3j] + c2 M[3j+1] + c3 M[3j+2],
+  // deviatoric h_j = (F0 . M_j, F1 . M_j, F2 . M_j) / |F|; vertex 0 takes minus their sum
+  auto gcol = [&](int j) { return (c1 * M[3 * j] + c2 * M[3 * j + 1]) + c3 * M[3 * j + 2]; };
+  auto hcol = [&](int j) {
+    const f3 r = mk(M[3 * j], M[3 * j + 1], M[3 * j + 2]);
+    return mk(dot(F[0], r), dot(F[1], r), dot(F[2], r));
+  };
+  f3 s = mk(0.0f, 0.0f, 0.0f);
   float w = 0.0f;
-  w = w + A::sq_len(g0) * mA;
-  w = w + A::sq_len(g1) * mB;
-  w = w + A::sq_len(g2) * mC;
-  w = w + A::sq_len(g3) * mD;
-  if (w < kFeps) return 0;
-  const float inv_rest = FAST ? __builtin_amdgcn_rcpf(rest) : 0.0f;
-  float alpha = FAST ? k.alpha_h * inv_rest : k.alpha_h / rest;
-  float dl = A::div(-C, w + alpha);
-  f3 dA = (g0 * dl) * mA, dB = (g1 * dl) * mB, dC = (g2 * dl) * mC, dD = (g3 * dl) * mD;
-  float rsi;
-  if (FAST) {
-    const float r2 = (dot(F0, F0) + dot(F1, F1)) + dot(F2, F2);
-    rsi = __builtin_amdgcn_rsqf(r2);
-    C = r2 * rsi;
-  } else {
-    float l0 = length(F0), l1 = length(F1), l2 = length(F2);
-    float rs = sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
-    rsi = 1.0f / rs;
-    C = rs;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const f3 g = gcol(j);
+    w = w + A::sq_len(g) * m[j + 1];
+    s = s + g;
   }
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      float s = 0.0f;
-      s = s + F[3 * i + 0] * M[j * 3 + 0];
-      s = s + F[3 * i + 1] * M[j * 3 + 1];
-      s = s + F[3 * i + 2] * M[j * 3 + 2];
-      g[3 * i + j] = s * rsi;
-    }
-  g1 = mk(g[0], g[3], g[6]);
-  g2 = mk(g[1], g[4], g[7]);
-  g3 = mk(g[2], g[5], g[8]);
-  g0 = ((-g1) - g2) - g3;
-  w = 0.0f;
-  w = w + A::sq_len(g0) * mA;
-  w = w + A::sq_len(g1) * mB;
-  w = w + A::sq_len(g2) * mC;
-  w = w + A::sq_len(g3) * mD;
+  w = w + A::sq_len(s) * m[0];
   if (w < kFeps) return 0;
-  alpha = FAST ? k.alpha_d * inv_rest : k.alpha_d / rest;
-  dl = A::div(-C, w + alpha);
-  d[0] = dA + (g0 * dl) * mA;
-  d[1] = dB + (g1 * dl) * mB;
-  d[2] = dC + (g2 * dl) * mC;
-  d[3] = dD + (g3 * dl) * mD;
+  const float C_h = dot(F[0], c1) - 1.0f - k.gamma;
+  const float inv_rest = FAST ? __builtin_amdgcn_rcpf(rest) : 0.0f;
+  const float dl_h = A::div(-C_h, w + (FAST ? k.alpha_h * inv_rest : k.alpha_h / rest));
+  float rsi, C_d;
+  if (FAST) {
+    const float r2 = (dot(F[0], F[0]) + dot(F[1], F[1])) + dot(F[2], F[2]);
+    rsi = __builtin_amdgcn_rsqf(r2);
+    C_d = r2 * rsi;
+  } else {
+    const float l0 = length(F[0]), l1 = length(F[1]), l2 = length(F[2]);
+    C_d = sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
+    rsi = 1.0f / C_d;
+  }
+  s = mk(0.0f, 0.0f, 0.0f);
+  w = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const f3 h = hcol(j) * rsi;
+    w = w + A::sq_len(h) * m[j + 1];
+    s = s + h;
+  }
+  w = w + A::sq_len(s) * m[0];
+  if (w < kFeps) return 0;
+  const float dl_d = A::div(-C_d, w + (FAST ? k.alpha_d * inv_rest : k.alpha_d / rest)) * rsi;
+  s = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const f3 t = gcol(j) * dl_h + hcol(j) * dl_d;
+    d[j + 1] = t * m[j + 1];
+    s = s + t;
+  }
+  d[0] = -s * m[0];
   return 1;
 }
 
