@@ -6,6 +6,7 @@ identifier; import it with ``importlib.import_module("ff-srl_amd")``.
 """
 from ._lib import FFSRLError, lib
 from .bvh import SimBVH
+from .env import SimEnvironment
 from .instrument import Laparoscope
 from .integrator import SimIntegratorDO
 from .model import SimModelDO
@@ -16,5 +17,5 @@ from . import serve
 
 __all__ = [
     "FFSRLError", "lib", "Laparoscope", "SimBVH", "SimIntegratorDO", "SimModelDO", "RaycastRendererDO", "RenderMode",
-    "WarpRaycastRendererDO", "autotune", "scenes", "serve",
+    "SimEnvironment", "WarpRaycastRendererDO", "autotune", "scenes", "serve",
 ]
