@@ -37,7 +37,10 @@ class SimEnvironment:
     ``ffsrl_frame`` launch; with ``useGraph`` it is captured into a HIP graph
     on the first step and replayed afterwards (actions are applied before the
     replay, outside the graph, since they take the caller's tensors).  Results
-    are the same with and without the graph.
+    are the same with and without the graph.  A graph holds the array
+    pointers, parameters and launch shape it was captured with; when any of
+    them has changed since (a new constraint order, an autotuned launch shape,
+    the shared frame switched on), the next step captures the frame again.
     """
 
     def __init__(self, simModel: SimModelDO, renderer: Optional[RaycastRendererDO] = None,
@@ -52,6 +55,8 @@ class SimEnvironment:
         self.useGraph = bool(useGraph)
         self.capturedGraph = None
         self._graph_stream = None
+        self._graph_key = None
+        self.captures = 0  # frame graphs captured so far
         self.image = None
         self.frames = 0
 
@@ -78,18 +83,24 @@ class SimEnvironment:
             self.simIntegrator.stepModel(m, refit=True)
             return
         cur = torch.cuda.current_stream(self.device)
+        integ = self.simIntegrator
+        if integ.shared:
+            m.enable_shared()  # allocates and probes once: never inside the capture
+        key = (bytes(m.batch), bytes(m.params), integ.block, integ.dispatch_sorted, integ.shared,
+               m.dispatch_order().data_ptr())
+        if self.capturedGraph is not None and key != self._graph_key:
+            self.close()
         if self.capturedGraph is None:
-            if self.simIntegrator.shared:
-                m.enable_shared()  # allocates and probes: not inside the capture
-            m.dispatch_order()
+            self._graph_key = key
             self._graph_stream = torch.cuda.Stream(self.device)
             self._graph_stream.wait_stream(cur)
             g = C.c_void_p()
             with torch.cuda.stream(self._graph_stream):
                 lib.ffsrl_graph_begin(self._graph_stream.cuda_stream)
-                self.simIntegrator.stepModel(m, refit=True)
+                integ.stepModel(m, refit=True)
                 lib.ffsrl_graph_end(self._graph_stream.cuda_stream, C.byref(g))
             self.capturedGraph = g
+            self.captures += 1
         self._graph_stream.wait_stream(cur)
         lib.ffsrl_graph_launch(self.capturedGraph, self._graph_stream.cuda_stream)
         cur.wait_stream(self._graph_stream)
@@ -157,6 +168,7 @@ class SimEnvironment:
             torch.cuda.synchronize(self.device)
             lib.ffsrl_graph_destroy(self.capturedGraph)
             self.capturedGraph = None
+            self._graph_key = None
 
     def __del__(self):
         try:

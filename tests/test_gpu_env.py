@@ -8,7 +8,8 @@ calls the environment stands for (resetCollisionInfo, applyActions,
 applyCartesianActionsInWorkspace, stepModel with the fused refit,
 renderNew).  Positions, instrument points, clamp states and images agree bit
 for bit after every frame (the graph is replayed 7 times, with host reads
-between replays).  reset(env_ids) restores the listed environments only.
+between replays; after a change of dispatch order the frame is captured
+again).  reset(env_ids) restores the listed environments only.
 """
 import os
 
@@ -69,25 +70,35 @@ def test_environment_eager_graph_and_facade_agree(pkg, gpu):
         _place(e.simModel, dev)
     _place(model, dev)
 
-    seen = False
-    for f, (joint, cart) in enumerate(_actions()):
-        got = []
-        for e in envs:
-            obs = e.step(actions=None if joint is None else dev(joint), actionsCartesian=dev(cart), render=True)
-            assert obs["image"].shape == (N, RES, RES, 3) and obs["tool"].shape == (N, 4, 3)
-            got.append(_state(e.simModel, obs["image"]))
-        model.resetCollisionInfo()
-        if joint is not None:
-            model.applyActions(dev(joint))
-        model.applyCartesianActionsInWorkspace(dev(cart))
-        integ.stepModel(model, refit=True)
-        got.append(_state(model, ren.renderNew(bvh, model)))
-        for g in got[:2]:
-            for a, b in zip(g, got[2]):
-                assert np.array_equal(a, b), f"frame {f}"
-        seen |= bool(got[2][4].any())
-    assert seen  # the tissue is in view
-    assert envs[1].capturedGraph is not None
+    def run(actions, what):
+        seen = False
+        for f, (joint, cart) in enumerate(actions):
+            got = []
+            for e in envs:
+                obs = e.step(actions=None if joint is None else dev(joint), actionsCartesian=dev(cart), render=True)
+                assert obs["image"].shape == (N, RES, RES, 3) and obs["tool"].shape == (N, 4, 3)
+                got.append(_state(e.simModel, obs["image"]))
+            model.resetCollisionInfo()
+            if joint is not None:
+                model.applyActions(dev(joint))
+            model.applyCartesianActionsInWorkspace(dev(cart))
+            integ.stepModel(model, refit=True)
+            got.append(_state(model, ren.renderNew(bvh, model)))
+            for g in got[:2]:
+                for a, b in zip(g, got[2]):
+                    assert np.array_equal(a, b), f"{what} frame {f}"
+            seen |= bool(got[2][4].any())
+        return seen
+
+    assert run(_actions(), "first")  # the tissue is in view
+    assert envs[1].capturedGraph is not None and envs[1].captures == 1
+    # a new launch shape (table dispatch order; results unchanged) makes the
+    # graph environment capture its frame again; a new constraint order is
+    # written into the same arrays, so the graph reads it either way
+    envs[1].simIntegrator.dispatch_sorted = False
+    envs[1].simModel.set_constraint_order("local", chunk=64)
+    run(_actions()[:3], "re-captured")
+    assert envs[1].captures == 2
 
     # reset of env 1 only
     e = envs[0]

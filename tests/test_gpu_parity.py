@@ -414,7 +414,10 @@ def test_reset_restores_initial_state(pkg, gpu):
 @pytest.mark.parametrize("iters", [1, 2])
 def test_autotune_keeps_state_and_results(pkg, gpu, tmp_path, iters):
     """Tuning restores the state it timed on, the tuned launch shape gives
-    bit-identical frames, and a second tune is served from the cache."""
+    the same frames bit for bit as the batched frame at the tuned block size
+    (the 1024-thread and the 256/512-thread loops contract multiply-adds
+    differently, so the block size is the one knob that changes rounding),
+    and a second tune is served from the cache."""
     sc = pkg.scenes
     meshes = [sc.solid("ellipsoid", 3000, seed=31), sc.solid("box", 1200, seed=32), sc.solid("ellipsoid", 800, seed=33)]
     kw = dict(NH, distance="xpbd", volume="xpbd", distanceCompliance=1e-6, volumeComplianceXPBD=0.0,
@@ -434,6 +437,7 @@ def test_autotune_keeps_state_and_results(pkg, gpu, tmp_path, iters):
     assert cfg["block"] in (256, 512, 1024) and cfg["dispatch"] in ("lpt", "table") and cfg["chunk"] in (64, 128, 256)
     for a, b in zip(before, (tuned.x, tuned.v, tuned.bvh_lo)):
         assert torch.equal(a, b)
+    integ_ref.block = cfg["block"]
     for _ in range(4):
         integ_ref.stepModel(ref, refit=True)
         integ.stepModel(tuned, refit=True)
