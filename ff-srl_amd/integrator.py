@@ -11,7 +11,7 @@ Three launch forms run the same arithmetic:
 * shared (``shared=True``): ``ffsrl_frame_shared``, the same launch with all
   CUs sharing the environments (each env's frame cut into chunks that the
   workgroups of one XCD take in turn), for batches with fewer environments
-  than CUs; bit-identical to the batched form at the same block size;
+  than CUs; bit-identical to the batched form;
 * plain: the reference's launch model with one launch per kernel per
   environment (``integrator.py:1779-1929`` is the reference's per-env
   precedent), used as the comparison baseline.

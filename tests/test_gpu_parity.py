@@ -354,6 +354,46 @@ def test_constraint_order_does_not_change_results(pkg, gpu, iters):
             assert torch.equal(a, b), order
 
 
+@pytest.mark.parametrize("collision", [False, True])
+def test_block_size_does_not_change_results(pkg, gpu, collision):
+    """The 1024-thread loop (indices one stride ahead) and the 256/512-thread
+    loop (positions one stride ahead) fuse the same multiply-adds (the
+    physics is built with expression-level contraction), so every block size
+    gives bit-identical positions, velocities, carried lambdas and bounds;
+    with collision, the contact counts too."""
+    sc = pkg.scenes
+    meshes = [sc.solid("ellipsoid", 3500, seed=41), sc.solid("box", 1500, seed=42)]
+    kw = dict(NH, distance="xpbd", volume="xpbd", distanceCompliance=1e-6, volumeComplianceXPBD=0.0,
+              simConstraintsSteps=2)
+    top = sc.top_point(meshes[0])
+    tips = np.zeros((3, 4, 3), np.float32)
+    tips[:, 2] = top + np.array([0.0, 0.05, 0.0])
+    if collision:
+        # capsule 0 pressed into the top of env 0's and env 2's mesh
+        bases = np.zeros((3, 4, 3), np.float32)
+        radii = np.zeros((3, 4), np.float32)
+        tips[:, 0] = top - np.array([0.0, 0.02, 0.0])
+        bases[:, 0] = top + np.array([0.0, 0.3, 0.0])
+        radii[:, 0] = 0.05
+        kw.update(collision=True, globalKsContact=1.0, capsuleBases=bases, capsuleRadii=radii)
+    out = {}
+    for block in (1024, 512, 256):
+        m = pkg.SimModelDO(meshes, [0, 1, 0], device=gpu, tips=tips, **kw)
+        m.set_grasp(0, sc.nearest_surface_vertices(meshes[0], top, 8))
+        integ = pkg.SimIntegratorDO(gpu, block=block)
+        for _ in range(6):
+            integ.stepModel(m, refit=True)
+        out[block] = [m.x.cpu(), m.v.cpu(), m.lam_t.cpu(), m.lam_e.cpu(), m.bvh_lo.cpu(), m.bvh_hi.cpu(),
+                      m.laparoscopeInCollision.cpu()]
+        assert float((out[block][0] - m.initialVertex.cpu()).abs().max()) > 1e-4
+    if collision:
+        assert int(out[1024][6].sum()) > 0
+    for block in (512, 256):
+        for k, (a, b) in enumerate(zip(out[1024], out[block])):
+            assert bool(torch.isfinite(a.float()).all())
+            assert torch.equal(a, b), (block, k)
+
+
 def test_xpbd_two_iterations_carry_lambda(pkg, orc, gpu):
     mesh, v0 = _c1_model(pkg, 7)
     kw = dict(C1, simConstraintsSteps=2)
@@ -414,10 +454,7 @@ def test_reset_restores_initial_state(pkg, gpu):
 @pytest.mark.parametrize("iters", [1, 2])
 def test_autotune_keeps_state_and_results(pkg, gpu, tmp_path, iters):
     """Tuning restores the state it timed on, the tuned launch shape gives
-    the same frames bit for bit as the batched frame at the tuned block size
-    (the 1024-thread and the 256/512-thread loops contract multiply-adds
-    differently, so the block size is the one knob that changes rounding),
-    and a second tune is served from the cache."""
+    bit-identical frames, and a second tune is served from the cache."""
     sc = pkg.scenes
     meshes = [sc.solid("ellipsoid", 3000, seed=31), sc.solid("box", 1200, seed=32), sc.solid("ellipsoid", 800, seed=33)]
     kw = dict(NH, distance="xpbd", volume="xpbd", distanceCompliance=1e-6, volumeComplianceXPBD=0.0,
@@ -437,7 +474,6 @@ def test_autotune_keeps_state_and_results(pkg, gpu, tmp_path, iters):
     assert cfg["block"] in (256, 512, 1024) and cfg["dispatch"] in ("lpt", "table") and cfg["chunk"] in (64, 128, 256)
     for a, b in zip(before, (tuned.x, tuned.v, tuned.bvh_lo)):
         assert torch.equal(a, b)
-    integ_ref.block = cfg["block"]
     for _ in range(4):
         integ_ref.stepModel(ref, refit=True)
         integ.stepModel(tuned, refit=True)
