@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_frame launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (x2.0 and x1.0 calibration factors from a 256 MiB copy), by
 # workload; profiles/k_frame_k_render_mixed1024.md holds the runs.
-MEASURED_TRAFFIC = {"mixed-1024env-logU[1000,40000]tets": 9.88e9}
+MEASURED_TRAFFIC = {"mixed-1024env-logU[1000,40000]tets": 9.91e9}
 
 
 def log(msg):
