@@ -150,10 +150,12 @@ def cpu_baseline(pkg, orc, w, model, ren, seconds, seed, threads):
     dirs = ren.dirs.cpu().numpy()
     origin = np.asarray(ren.camera.origin[:], np.float32)
     light = np.asarray(ren.camera.light_dir[:], np.float32)
-    bvhs = {}
+    bvhs, tex = {}, {}
     for k in set(w.env_mesh[e] for e in order[:4096]):
         t = model.topo[k]
         bvhs[k] = {"left": t.left, "first": t.first, "count": t.count, "level": t.level, "tri_map": t.tri_map}
+        if w.rgb:
+            tex[k] = model.texture_inputs(k)
 
     def one(e):
         k = w.env_mesh[e]
@@ -171,7 +173,7 @@ def cpu_baseline(pkg, orc, w, model, ren, seconds, seed, threads):
         if w.rgb:
             nrm = orc.vertex_normals(m.tris, sim.x)
             orc.render_rgb(ren.height, ren.width, 1, origin, dirs, bvh, mn, mx, m.tris, sim.x, nrm,
-                           tcolor=m.tri_colors, light=light, intensity=ren.camera.light_intensity)
+                           vcolor=m.colors, light=light, intensity=ren.camera.light_intensity, **tex[k])
         return m.num_tets
 
     def run(workers, budget, envs):
@@ -282,7 +284,7 @@ def main():
 
     cam_rgb = pkg._lib.Camera()
     C.pointer(cam_rgb)[0] = ren.camera
-    cam_rgb.mode = pkg.RenderMode.texture              # per-triangle colours, Lambert-lit
+    cam_rgb.mode = pkg.RenderMode.texture              # texture lookup (each mesh's seeded texture), Lambert-lit
 
     def render(sh, eb=0, ee=n):
         if w.rgb:
@@ -361,6 +363,7 @@ def main():
     render_ms = np.array([ev[i][2].elapsed_time(ev[i][3]) for i in range(args.steps)])
     value = n * args.steps / elapsed
     finite = bool(torch.isfinite(model.x).all().item())
+    status = model.frame_status().cpu().numpy()
 
     out = {
         "metric": "env-steps/s @1024 mixed-size envs + p50/p99 step latency, batched vs plain",
@@ -369,7 +372,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": w.data,
         "config": {"workload": w.name, "envs": n, **{k: int(v) for k, v in w.totals().items()},
                    "substeps": w.model_kwargs["simSubsteps"], "iterations": w.model_kwargs["simConstraintsSteps"],
-                   "render": f"{w.resolution}x{w.resolution} depth{' + RGB (per-triangle colours)' if w.rgb else ''} "
+                   "render": f"{w.resolution}x{w.resolution} depth{' + RGB (texture mode)' if w.rgb else ''} "
                              f"(2x2 supersampled)", "block": integ.block,
                    "dispatch": "lpt" if integ.dispatch_sorted else "table", "shared": integ.shared,
                    "chunk": model.topo[0].chunk, "autotuned": tuned is not None,
@@ -379,6 +382,8 @@ def main():
         "frame_kernel_ms": float(frame_ms.mean()), "render_kernel_ms": float(render_ms.mean()),
         "rays_per_s": float(4 * w.resolution ** 2 * n / (render_ms.mean() * 1e-3)),
         "finite": finite, "setup_s": setup_s,
+        # envs whose frames reported a saturated fixed-point term / an incomplete shared frame
+        "status_envs": {"saturated": int(((status & 1) != 0).sum()), "incomplete": int(((status & 2) != 0).sum())},
     }
     alg = algorithmic_bytes(w, model)
     achieved = alg / (frame_ms.mean() * 1e-3) / 1e9

@@ -15,13 +15,16 @@ LIB_PATH = os.environ.get("FFSRL_LIB") or os.path.join(HERE, "_build", "libffsrl
 
 OFF, PBD, XPBD = 0, 1, 2
 E_ARG, E_LIMIT, E_STATE = -1, -2, -3
+ABI_VERSION = 10
+STATUS_SATURATED, STATUS_INCOMPLETE = 1, 2   # FFSRL_STATUS_* bits of ffsrl_batch.status
+MAX_TERMS = 128                              # FFSRL_MAX_TERMS
 
 
 class EnvDesc(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "vbase", "nv", "ebase", "ne", "tbase", "nt", "fbase", "nf", "nodebase", "nn",
         "boundbase", "lvlbase", "nlvl", "lam_ebase", "lam_tbase", "dp_exp", "vtopo_base", "bvh_depth",
-        "reserved1", "reserved2")]
+        "max_terms", "reserved2")]
 
 
 ENV_DESC_FIELDS = [f[0] for f in EnvDesc._fields_]
@@ -37,11 +40,13 @@ class Batch(C.Structure):
         ("edges", _vp), ("edge_rest", _vp), ("tets", _vp), ("tets16", _vp), ("edges16", _vp), ("tet_rest_volume", _vp),
         ("tet_inv_rest", _vp), ("tris", _vp), ("node_left", _vp), ("node_first", _vp),
         ("node_count", _vp), ("level_nodes", _vp), ("level_info", _vp), ("level_off", _vp), ("vf_off", _vp),
-        ("vf_idx", _vp),
+        ("vf_idx", _vp), ("tri_uv", _vp), ("tri_tex", _vp), ("texels", _vp), ("tex_info", _vp),
+        ("n_texels", C.c_int64),
         ("x", _vp), ("p", _vp), ("v", _vp), ("drag", _vp), ("lam_e", _vp), ("lam_t", _vp),
         ("bvh_lo", _vp), ("bvh_hi", _vp), ("tips", _vp), ("caps_base", _vp), ("caps_radius", _vp),
         ("in_collision", _vp), ("sched", _vp),
-        ("dp", _vp), ("cnt", _vp), ("split_acc", _vp), ("split_ctr", _vp), ("host_envs", _vp), ("host_level_off", _vp),
+        ("dp", _vp), ("cnt", _vp), ("split_acc", _vp), ("split_ctr", _vp), ("status", _vp), ("host_envs", _vp),
+        ("host_level_off", _vp),
     ]
 
 
@@ -125,6 +130,7 @@ SIGNATURES = {
     "ffsrl_graph_end": [_vp, _P(_vp)],
     "ffsrl_graph_launch": [_vp, _vp],
     "ffsrl_graph_destroy": [_vp],
+    "ffsrl_graph_abort": [_vp],
 }
 # Exported functions that do not return an error code.
 PLAIN_SIGNATURES = {

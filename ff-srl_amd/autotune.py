@@ -51,8 +51,12 @@ def _bucket(x: float) -> int:
 
 
 def key_of(model: SimModelDO) -> str:
+    """Cache key: env-count and tets buckets, the constraint set, the
+    collision pass (a different kernel) and whether the shared-env frame can
+    run on this device (an XCD map)."""
     tets = np.array([model.meshes[k].num_tets for k in model.env_mesh], dtype=np.float64)
-    flags = f"d{model.dist_mode}v{model.vol_mode}n{int(model.nh_on)}s{model.simSubsteps}i{model.simConstraints}"
+    flags = (f"d{model.dist_mode}v{model.vol_mode}n{int(model.nh_on)}s{model.simSubsteps}i{model.simConstraints}"
+             f"c{int(model.collision)}x{int(lib.ffsrl_xcd_count() > 0)}")
     return f"envs{_bucket(model.numEnvs)}-tets{_bucket(tets.mean())}-max{_bucket(tets.max())}-{flags}"
 
 
@@ -74,7 +78,8 @@ class Autotuner:
 
     @staticmethod
     def _state(model: SimModelDO):
-        return [model.x, model.p, model.v, model.bvh_lo, model.bvh_hi]
+        # the contact counters and status words too: trial frames add to them
+        return [model.x, model.p, model.v, model.bvh_lo, model.bvh_hi, model.laparoscopeInCollision, model.status]
 
     @staticmethod
     def _lambdas(model: SimModelDO):

@@ -54,8 +54,12 @@ constexpr int kMaxFrameVertices = (kLdsBytes - 512) / 16;
 // diagonal D) and added with an integer LDS atomic.  Integer atomics run at
 // the LDS's full rate on gfx950 (float LDS atomics are ~30x slower), and
 // integer sums do not depend on the order of the additions, so the frame is
-// deterministic.  Each term is clamped to +-2^24 q (D/16 .. D/8), so sums of
-// up to 128 terms stay inside int32; the rounding error per term is <= q/2.
+// deterministic.  Each term is clamped to +-(2^24 - 1) q (about D/16 .. D/8),
+// so the sums of up to FFSRL_MAX_TERMS = 128 terms (a vertex's incident
+// constraints in one pass, checked at launch) stay inside int32; the
+// rounding error per term is <= q/2.  A term beyond the range, or not
+// finite, is clamped and flags the env (FFSRL_STATUS_SATURATED): the frame
+// then no longer equals the plain form, which sums floats.
 //
 // A vertex owns 16 B of LDS, two 64-bit words updated with ds_add_u64:
 //   w0 = X + Y * 2^32,  w1 = Z + count * 2^32   (two's complement, mod 2^64)
@@ -64,15 +68,35 @@ constexpr int kMaxFrameVertices = (kLdsBytes - 512) / 16;
 // order; since |X| < 2^31, X is the low half of w0 read as int32 and
 // Y = high half - (X < 0 ? -1 : 0) (likewise Z and the count).  One 64-bit
 // atomic thus carries two sums at one address.
+constexpr float kFixMax = 16777215.0f;  // 2^24 - 1 quanta
+
 struct FrameCtx {
   unsigned long long* acc;  // 2 words per vertex
   float inv_q;              // 2^-dp_exp
   float q;                  // 2^dp_exp
 };
 
-__device__ __forceinline__ int32_t to_fix(float x, float inv_q) {
-  float v = rintf(x * inv_q);  // scaling by a power of two is exact
-  v = fminf(fmaxf(v, -16777216.0f), 16777216.0f);
+// A term that left the range or was not finite adds kSatMark to its count
+// (counts stay below 2^16: at most FFSRL_MAX_TERMS terms of weight <= 2), so
+// the apply phase sees it in the sum's count and sets the workgroup's
+// saturation flag (one LDS word, cleared and read once per env frame); no
+// register holds a flag across the projection loops.
+constexpr uint32_t kSatMark = 1u << 16;
+
+__device__ __forceinline__ int& sat_flag() {
+  __shared__ int flag;
+  return flag;
+}
+
+// The count of a decoded sum without saturation marks; raises the flag.
+__device__ __forceinline__ int32_t sum_count(int32_t n) {
+  if (__builtin_expect((uint32_t)n >= kSatMark, 0)) sat_flag() = 1;
+  return n & (int32_t)(kSatMark - 1u);
+}
+
+__device__ __forceinline__ int32_t to_fix(float v) {
+  v = rintf(v);
+  v = fminf(fmaxf(v, -kFixMax), kFixMax);
   return (int32_t)v;
 }
 // (lo + hi * 2^32) as a 64-bit two's complement addend, lo sign-extended.
@@ -88,9 +112,13 @@ __device__ __forceinline__ void unpack2(unsigned long long w, int32_t* lo, int32
 // Adds sign * d (fixed point) and n to the count of vertex i.
 template <int SIGN>
 __device__ __forceinline__ void lds_acc(const FrameCtx& c, int i, f3 d, uint32_t n) {
-  const int32_t x = to_fix(d.x, c.inv_q), y = to_fix(d.y, c.inv_q), z = to_fix(d.z, c.inv_q);
+  const float sx = d.x * c.inv_q, sy = d.y * c.inv_q, sz = d.z * c.inv_q;  // scaling by 2^k is exact
+  // IEEE 754-2019 maximum propagates NaN, which fails the comparison
+  const float m = __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(sx), fabsf(sy)), fabsf(sz));
+  const uint32_t mark = m <= kFixMax ? 0u : kSatMark;
+  const int32_t x = to_fix(sx), y = to_fix(sy), z = to_fix(sz);
   atomicAdd(&c.acc[2 * i], pack2(SIGN * x, SIGN * y));
-  atomicAdd(&c.acc[2 * i + 1], pack2(SIGN * z, (int32_t)n));
+  atomicAdd(&c.acc[2 * i + 1], pack2(SIGN * z, (int32_t)(n + mark)));
 }
 
 __device__ __forceinline__ float4 with_w(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
@@ -342,6 +370,7 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
   const int nv = e.nv;
   FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
   const int tid = threadIdx.x;
+  if (tid == 0) sat_flag() = 0;  // read after the frame's barriers
   const int64_t vb = e.vbase;
   float* X = b.x;
   float* P = b.p;
@@ -415,6 +444,7 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
         int32_t fx, fy, fz, n;
         unpack2(w0, &fx, &fy);
         unpack2(w1, &fz, &n);
+        n = sum_count(n);
         float4 p4 = ld4(P, vb + i);
         f3 p = xyz(p4);
         if (n > 0) {
@@ -441,6 +471,7 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
           int32_t fx, fy, fz, n;
           unpack2(w0, &fx, &fy);
           unpack2(w1, &fz, &n);
+          (void)sum_count(n);  // collision terms carry no count, only saturation marks
           const float4 p4 = ld4(P, vb + i);
           const f3 p = xyz(p4) + mk((float)fx, (float)fy, (float)fz) * c.q;
           if (last_it)
@@ -453,6 +484,8 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
     }
   }
   if (collide && tid == 0) b.in_collision[env] += contacts;  // counts accumulate until the caller clears them
+  __syncthreads();
+  if (tid == 0 && sat_flag()) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_SATURATED);
   if (refit) refit_env<BLOCK>(b, e);
 }
 
@@ -529,8 +562,15 @@ static int launch_frame(const ffsrl_batch& b, const ffsrl_params& prm, const int
 // and only after it has counted every chunk it has finished; only running
 // workgroups hold chunks, so the lowest chunk not yet counted always belongs
 // to a workgroup that can run it, and every pass completes whatever number
-// of workgroups is resident.  Waits are bounded anyway (kShareSpinLimit); a
-// frame that could not complete sets ctr[2].
+// of workgroups is resident.  Waits are bounded anyway (kShareSpinLimit): a
+// workgroup that gives up sets the env's give-up flag ctr[2], which makes
+// the env's other waits give up too; the last workgroup out then reports
+// the frame (FFSRL_STATUS_INCOMPLETE), clears its partial global sums and
+// resets its counters.
+//
+// Coherence is an assumption about the hardware, not a guarantee of the
+// HIP memory model; it is what a probe of the MI355X showed, and the
+// bit-equality tests against the one-workgroup frame are its standing check.
 //
 // Coherence: all workgroups of an env run on one XCD and share its L2.
 // Stores and atomics have reached the L2 when `s_waitcnt vmcnt(0)` returns
@@ -603,12 +643,31 @@ __device__ __forceinline__ ShareChunk share_chunk(const SharePlan& p, int c, int
   return k;
 }
 
+// Where the shared frame keeps an env's global sums (2 words per vertex)
+// and chunk counters (taken, done, give-up flag, exits): indexed by env and
+// vertex base (the batch's split_acc / split_ctr), or by list slot el, with
+// slot_nv vertices per slot (a workspace of the library's own).
+struct ShareWs {
+  unsigned long long* acc;
+  uint32_t* ctr;
+  int by_slot;
+  int slot_nv;
+  __device__ unsigned long long* acc_of(int el, const ffsrl_env_desc& e) const {
+    return acc + 2 * (by_slot ? (int64_t)el * slot_nv : (int64_t)e.vbase);
+  }
+  __device__ uint32_t* ctr_of(int el, int env) const { return ctr + 4 * (int64_t)(by_slot ? el : env); }
+};
+
 // Polls ctr[1] until it reaches target; false when the wait is given up
 // (limit reached, or another workgroup of the env gave up: ctr[2] set).
+// The give-up flag is read with the first poll and every 256th, both loads
+// in flight together; a pass found complete is never given up.
 __device__ __forceinline__ bool share_wait(uint32_t* ctr, uint32_t target) {
   for (int k = 0; k < kShareSpinLimit; ++k) {
-    if (atomicAdd(&ctr[1], 0u) >= target) return true;
-    if ((k & 255) == 255 && atomicAdd(&ctr[2], 0u) != 0u) return false;
+    const uint32_t done = atomicAdd(&ctr[1], 0u);
+    const uint32_t gave_up = (k & 255) == 0 ? atomicAdd(&ctr[2], 0u) : 0u;
+    if (done >= target) return true;
+    if (gave_up) return false;
     __builtin_amdgcn_s_sleep(2);
   }
   return false;
@@ -616,13 +675,13 @@ __device__ __forceinline__ bool share_wait(uint32_t* ctr, uint32_t target) {
 
 // The env's share of one workgroup: chunks until the env's counter runs out.
 template <int BLOCK, bool CARRY, bool COLLIDE>
-__device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, int env, int refit,
-                          unsigned long long* lds) {
+__device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, const ShareWs& ws, int el, int env,
+                          int refit, unsigned long long* lds) {
   __shared__ int s_chunk, s_ok, s_last, s_contacts;
   const ffsrl_env_desc e = b.envs[env];
   const SharePlan pl = share_plan(e, prm, COLLIDE);
-  uint32_t* ctr = b.split_ctr + 4 * (int64_t)env;
-  unsigned long long* gacc = reinterpret_cast<unsigned long long*>(b.split_acc) + 2 * (int64_t)e.vbase;
+  uint32_t* ctr = ws.ctr_of(el, env);
+  unsigned long long* gacc = ws.acc_of(el, e);
   const int nv = e.nv, tid = threadIdx.x;
   const int64_t vb = e.vbase;
   FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
@@ -659,7 +718,10 @@ __device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, int env
   };
 
   for (int i = tid; i < 2 * nv; i += BLOCK) lds[i] = 0ull;
-  if (tid == 0) s_contacts = 0;
+  if (tid == 0) {
+    s_contacts = 0;
+    sat_flag() = 0;
+  }
   int pend_base = -1, pend = 0;
   for (;;) {
     __syncthreads();  // LDS and the shared words of the previous chunk are done with
@@ -725,6 +787,7 @@ __device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, int env
         int32_t fx, fy, fz, n;
         unpack2(w0, &fx, &fy);
         unpack2(w1, &fz, &n);
+        n = sum_count(n);
         const float4 p4 = ld4c(P, vb + i);
         f3 p = xyz(p4);
         if (n > 0) {
@@ -747,6 +810,7 @@ __device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, int env
         int32_t fx, fy, fz, n;
         unpack2(w0, &fx, &fy);
         unpack2(w1, &fz, &n);
+        (void)sum_count(n);  // collision terms carry no count, only saturation marks
         const float4 p4 = ld4c(P, vb + i);
         const f3 p = xyz(p4) + mk((float)fx, (float)fy, (float)fz) * c.q;
         if (last_it)
@@ -761,6 +825,8 @@ __device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, int env
     __syncthreads();
     if (refit && s_last) refit_env<BLOCK, true>(b, e);  // every chunk of the frame is done
   }
+  __syncthreads();
+  if (tid == 0 && sat_flag()) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_SATURATED);
 }
 
 // Hardware XCD id (0..15) -> dense index 0..nxcd-1 (-1: no XCD of that id).
@@ -770,32 +836,44 @@ struct XcdMap {
 
 template <int BLOCK, bool CARRY, bool COLLIDE>
 __global__ __launch_bounds__(BLOCK) void k_frame_shared(ffsrl_batch b, ffsrl_params prm, const int32_t* list, int n,
-                                                        int refit, int nxcd, XcdMap map) {
+                                                        int refit, int nxcd, XcdMap map, ShareWs ws) {
   extern __shared__ unsigned long long lds[];
+  __shared__ int s_lastwg, s_incomplete;
   const int x = map.index[xcc_id() & 15];
   const int mine = (x >= 0 && x < nxcd && x < n) ? (n - 1 - x) / nxcd + 1 : 0;  // slots x, x + nxcd, ...
   const int rank = (int)blockIdx.x / nxcd;                           // spreads the XCD's workgroups
   for (int k = 0; k < mine; ++k) {
     const int el = x + nxcd * ((rank + k) % mine);
-    share_env<BLOCK, CARRY, COLLIDE>(b, prm, list ? list[el] : el, refit, lds);
+    share_env<BLOCK, CARRY, COLLIDE>(b, prm, ws, el, list ? list[el] : el, refit, lds);
   }
-  // The last workgroup out checks that every env's frame completed and
-  // clears the chunk counters for the next launch.
+  // The last workgroup out checks that every env's frame completed, clears
+  // the global sums of any frame that did not (a workgroup that gave up
+  // leaves partial sums), reports it, and resets the counters for the next
+  // launch.  Every other workgroup has exited, so nothing else touches them.
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t* exits = b.split_ctr + 4 * (int64_t)(list ? list[0] : 0) + 3;
-    if (atomicAdd(exits, 1u) == gridDim.x - 1) {
-      for (int el = 0; el < n; ++el) {
-        const int env = list ? list[el] : el;
-        uint32_t* ctr = b.split_ctr + 4 * (int64_t)env;
-        const SharePlan p = share_plan(b.envs[env], prm, COLLIDE);
-        if (atomicAdd(&ctr[1], 0u) != (uint32_t)p.total) atomicOr(&ctr[2], 2u);
-        atomicExch(&ctr[0], 0u);
-        atomicExch(&ctr[1], 0u);
-      }
-      atomicExch(exits, 0u);
+  uint32_t* exits = ws.ctr_of(0, list ? list[0] : 0) + 3;
+  if (threadIdx.x == 0) s_lastwg = atomicAdd(exits, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_lastwg) return;
+  for (int el = 0; el < n; ++el) {
+    const int env = list ? list[el] : el;
+    const ffsrl_env_desc e = b.envs[env];
+    uint32_t* ctr = ws.ctr_of(el, env);
+    if (threadIdx.x == 0) s_incomplete = atomicAdd(&ctr[1], 0u) != (uint32_t)share_plan(e, prm, COLLIDE).total;
+    __syncthreads();
+    if (s_incomplete) {
+      unsigned long long* gacc = ws.acc_of(el, e);
+      for (int i = threadIdx.x; i < 2 * e.nv; i += BLOCK) atomicExch(&gacc[i], 0ull);
     }
+    if (threadIdx.x == 0) {
+      if (s_incomplete) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_INCOMPLETE);
+      atomicExch(&ctr[0], 0u);
+      atomicExch(&ctr[1], 0u);
+      atomicExch(&ctr[2], 0u);
+    }
+    __syncthreads();
   }
+  if (threadIdx.x == 0) atomicExch(exits, 0u);
 }
 
 __global__ void k_xcc_probe(int32_t* out) {
@@ -804,7 +882,7 @@ __global__ void k_xcc_probe(int32_t* out) {
 
 template <int BLOCK>
 static int launch_shared(const ffsrl_batch& b, const ffsrl_params& prm, const int32_t* list, int n, int refit,
-                         bool carry, int lds, int nxcd, const XcdMap& map, hipStream_t st) {
+                         bool carry, int lds, int nxcd, const XcdMap& map, const ShareWs& ws, hipStream_t st) {
   auto kern = prm.collide ? (carry ? k_frame_shared<BLOCK, true, true> : k_frame_shared<BLOCK, false, true>)
                           : (carry ? k_frame_shared<BLOCK, true, false> : k_frame_shared<BLOCK, false, false>);
   FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -814,7 +892,7 @@ static int launch_shared(const ffsrl_batch& b, const ffsrl_params& prm, const in
   FFSRL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   FFSRL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), BLOCK, lds));
   const int grid = std::max(nxcd, cus * std::max(1, per_cu));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), lds, st, b, prm, list, n, refit, nxcd, map);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), lds, st, b, prm, list, n, refit, nxcd, map, ws);
   FFSRL_HIP(hipGetLastError());
   return 0;
 }
@@ -1088,6 +1166,14 @@ static int frame_checks(const ffsrl_batch* b, const ffsrl_params* prm, int32_t n
   if (prm->collide && (!b->tris || !b->caps_base || !b->caps_radius || !b->in_collision))
     return fail(FFSRL_E_ARG, "collision needs triangles, capsule bases / radii and the in_collision counters");
   if (n < 0 || n > b->n_env) return fail(FFSRL_E_ARG, "%d environments listed, batch has %d", n, b->n_env);
+  if (!b->status) return fail(FFSRL_E_ARG, "the batched frames need the per-env status array");
+  if (b->host_envs) {
+    // a vertex's terms of one pass must fit the 32-bit fixed-point sums
+    for (int i = 0; i < b->n_env; ++i)
+      if (b->host_envs[i].max_terms > FFSRL_MAX_TERMS)
+        return fail(FFSRL_E_LIMIT, "env %d: a vertex receives %d Jacobi terms in one pass (batched frame limit %d);"
+                    " use the plain form", i, b->host_envs[i].max_terms, FFSRL_MAX_TERMS);
+  }
   return 0;
 }
 
@@ -1138,7 +1224,7 @@ int ffsrl_xcd_count(void) {
   hipLaunchKernelGGL(k_xcc_probe, dim3(cus), dim3(64), 0, 0, d);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpy(h.data(), d, sizeof(int32_t) * cus, hipMemcpyDeviceToHost);
-  hipFree(d);
+  (void)hipFree(d);
   FFSRL_HIP(e);
   bool seen[16] = {};
   for (int v : h) {
@@ -1172,10 +1258,11 @@ int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int3
   const bool carry = carries_lambda(*prm);
   const int lds = lds_bytes_for(b->max_nv < 1 ? 1 : b->max_nv);
   hipStream_t st = as_stream(stream);
+  const ShareWs ws{reinterpret_cast<unsigned long long*>(b->split_acc), b->split_ctr, 0, 0};
   switch (block) {
-    case 256: return launch_shared<256>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, st);
-    case 512: return launch_shared<512>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, st);
-    case 1024: return launch_shared<1024>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, st);
+    case 256: return launch_shared<256>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, ws, st);
+    case 512: return launch_shared<512>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, ws, st);
+    case 1024: return launch_shared<1024>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, ws, st);
     default: return fail(FFSRL_E_ARG, "block must be 256, 512 or 1024 (got %d)", block);
   }
 }
@@ -1383,6 +1470,14 @@ int ffsrl_graph_end(void* stream, void** graph_exec) {
 int ffsrl_graph_launch(void* graph_exec, void* stream) {
   if (!graph_exec) return fail(FFSRL_E_ARG, "graph_exec NULL");
   FFSRL_HIP(hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graph_exec), as_stream(stream)));
+  return 0;
+}
+
+int ffsrl_graph_abort(void* stream) {
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(as_stream(stream), &g);
+  if (g) (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
   return 0;
 }
 

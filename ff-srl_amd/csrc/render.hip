@@ -90,6 +90,40 @@ __device__ __forceinline__ T ldu4(const T* base, int i) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint32_t)i * (uint32_t)sizeof(T));
 }
 
+// float -> int as the reference's int() on the device: truncation toward
+// zero, NaN to 0, saturating at the int range.
+__device__ __forceinline__ int trunc_i(float x) {
+  if (!(x == x)) return 0;
+  if (x >= 2147483647.0f) return 2147483647;
+  if (x <= -2147483648.0f) return -2147483647 - 1;
+  return (int)x;
+}
+
+// One texel of the concatenated table; an index outside it reads as zero
+// (the reference's lookup reads one column past a texture's last, where the
+// weight is zero).
+__device__ __forceinline__ f3 texel(const ffsrl_batch& b, int64_t i) {
+  if (i < 0 || i >= b.n_texels) return mk(0.0f, 0.0f, 0.0f);
+  return xyz(ld4(b.texels, i));
+}
+
+// texture_interpolation (render.py:124-148): bilinear lookup in square
+// texture tx at texture coordinates (s, t), with the reference's indexing
+// y + x * width and no clamp of x0 + 1 / y0 + 1.
+__device__ __forceinline__ f3 texture_lookup(const ffsrl_batch& b, int tx, float s, float t) {
+  const int64_t first = b.tex_info[2 * tx];
+  const int size = (int)b.tex_info[2 * tx + 1];
+  const float fx = s * (float)(size - 1), fy = (1.0f - t) * (float)(size - 1);
+  const int x0 = trunc_i(fx), y0 = trunc_i(fy);
+  const int x1 = x0 + 1, y1 = y0 + 1;
+  const float ax = fx - (float)x0, ay = fy - (float)y0;
+  const f3 c00 = texel(b, first + y0 + (int64_t)x0 * size), c10 = texel(b, first + y0 + (int64_t)x1 * size);
+  const f3 c01 = texel(b, first + y1 + (int64_t)x0 * size), c11 = texel(b, first + y1 + (int64_t)x1 * size);
+  const f3 lower = c00 * (1.0f - ax) + c10 * ax;
+  const f3 upper = c01 * (1.0f - ax) + c11 * ax;
+  return lower * (1.0f - ay) + upper * ay;
+}
+
 struct RenderOut {
   float* depth;
   float* rgb;
@@ -188,12 +222,18 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
       if (cam.mode == 2) {
         color = normal * 0.5f + mk(0.5f, 0.5f, 0.5f);
       } else {
+        const int tex = (cam.mode == 1 && b.tri_tex) ? b.tri_tex[fb + best] : -1;
         if (cam.mode == 0) {
           color = mk(1.0f, 1.0f, 1.0f);
-        } else if (cam.mode == 1) {
+        } else if (cam.mode == 5) {
           const float* tc = out.tcolor + 3 * (int64_t)(fb + best);
           color = mk(tc[0], tc[1], tc[2]);
-        } else {
+        } else if (tex >= 0) {  // texture mode: tex = u * tex_a + v * tex_b + w * tex_c (render.py:356-377)
+          const float* uv = b.tri_uv + 6 * (int64_t)(fb + best);
+          const float s = (u * uv[0] + v * uv[2]) + w * uv[4];
+          const float t = (u * uv[1] + v * uv[3]) + w * uv[5];
+          color = texture_lookup(b, tex, s, t);
+        } else {  // vertex colour, and the texture mode's fallback (render.py:378-380)
           const float* ca = out.vcolor + 3 * (vb + tri.x);
           const float* cb = out.vcolor + 3 * (vb + tri.y);
           const float* cc = out.vcolor + 3 * (vb + tri.z);
@@ -268,13 +308,15 @@ static int render(const ffsrl_batch* b, const ffsrl_camera* cam, int32_t eb, con
     return fail(FFSRL_E_ARG, "render size %dx%d: width and height must be positive multiples of 4", cam->height,
                 cam->width);
   const int m = cam->mode;
-  if (m != 0 && m != 1 && m != 2 && m != 3 && m != 4) return fail(FFSRL_E_ARG, "render mode %d", m);
+  if (m < 0 || m > 5) return fail(FFSRL_E_ARG, "render mode %d", m);
   if (m == 4 && !out.depth && !out.hit) return fail(FFSRL_E_ARG, "depth mode needs a depth (or hit) output");
   // a colour mode also writes the depth image of the same traversal when a
   // depth output is given (RGB-D in one pass)
   if (m != 4 && (!out.rgb || !out.normals)) return fail(FFSRL_E_ARG, "colour modes need rgb output and normals");
-  if (m == 3 && !out.vcolor) return fail(FFSRL_E_ARG, "vertex colour mode needs vertex colours");
-  if (m == 1 && !out.tcolor) return fail(FFSRL_E_ARG, "triangle colour mode needs triangle colours");
+  if ((m == 3 || m == 1) && !out.vcolor) return fail(FFSRL_E_ARG, "vertex colour and texture modes need vertex colours");
+  if (m == 1 && b->tri_tex && (!b->tri_uv || !b->texels || !b->tex_info))
+    return fail(FFSRL_E_ARG, "texture mode: triangle textures given without coordinates, texels or texture table");
+  if (m == 5 && !out.tcolor) return fail(FFSRL_E_ARG, "triangle colour mode needs triangle colours");
   if (n == 0) return 0;
   const int tiles = (2 * cam->width / 8) * (2 * cam->height / 8);
   // Stack entries per ray: the deepest BVH of the envs (levels = max level + 1

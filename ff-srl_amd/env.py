@@ -91,14 +91,20 @@ class SimEnvironment:
         if self.capturedGraph is not None and key != self._graph_key:
             self.close()
         if self.capturedGraph is None:
-            self._graph_key = key
             self._graph_stream = torch.cuda.Stream(self.device)
             self._graph_stream.wait_stream(cur)
             g = C.c_void_p()
+            s = self._graph_stream.cuda_stream
             with torch.cuda.stream(self._graph_stream):
-                lib.ffsrl_graph_begin(self._graph_stream.cuda_stream)
-                integ.stepModel(m, refit=True)
-                lib.ffsrl_graph_end(self._graph_stream.cuda_stream, C.byref(g))
+                lib.ffsrl_graph_begin(s)
+                try:
+                    integ.stepModel(m, refit=True)
+                except BaseException:
+                    # leave capture mode and drop the partial graph before re-raising
+                    lib.ffsrl_graph_abort(s)
+                    raise
+                lib.ffsrl_graph_end(s, C.byref(g))
+            self._graph_key = key
             self.capturedGraph = g
             self.captures += 1
         self._graph_stream.wait_stream(cur)

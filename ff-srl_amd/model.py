@@ -25,7 +25,8 @@ from typing import Optional, Sequence
 import numpy as np
 import torch
 
-from ._lib import Batch, BvhJob, EnvDesc, ENV_DESC_FIELDS, Instrument, OFF, PBD, XPBD, Params, lib
+from ._lib import (Batch, BvhJob, EnvDesc, ENV_DESC_FIELDS, FFSRLError, Instrument, OFF, PBD, XPBD, Params,
+                   STATUS_INCOMPLETE, STATUS_SATURATED, lib)
 from .scenes import TetMesh
 
 MAX_STACK = 32  # traversal stack entries per ray (render.py:424, 924)
@@ -125,6 +126,57 @@ def build_bvhs_device(meshes: Sequence[TetMesh], device="cuda:0", bins: int = 8)
     return res
 
 
+def validate_bvh(bvh: dict, nf: int) -> None:
+    """Raises ValueError unless ``bvh`` is a tree the kernels can traverse
+    and refit safely: the root is node 0 at level 0; an inner node (count
+    0) has its children at left and left + 1, both in range and one level
+    deeper; a leaf (count > 0, left -1) holds triangles [first, first +
+    count) within [0, nf); every node is reached from the root exactly once;
+    the leaves' ranges tile [0, nf) exactly once; tri_map permutes the nf
+    triangles.  The render and refit kernels index nodes and triangles with
+    these values unchecked."""
+    n = int(bvh["num_nodes"])
+    left, first, count, level = (np.asarray(bvh[k], np.int64) for k in ("left", "first", "count", "level"))
+    if n < 1 or min(left.size, first.size, count.size, level.size) < n:
+        raise ValueError(f"BVH: {n} nodes declared, arrays hold {min(left.size, first.size, count.size, level.size)}")
+    if n > max(1, 2 * nf - 1):
+        raise ValueError(f"BVH: {n} nodes for {nf} triangles (at most {2 * nf - 1})")
+    tri_map = np.asarray(bvh["tri_map"])
+    if tri_map.shape != (nf,) or not np.array_equal(np.sort(tri_map), np.arange(nf)):
+        raise ValueError(f"BVH: the triangle map does not permute the {nf} triangles")
+    if level[0] != 0:
+        raise ValueError("BVH: the root is not at level 0")
+    seen = np.zeros(n, bool)
+    cover = np.zeros(nf, np.int64)
+    stack = [0]
+    while stack:
+        k = stack.pop()
+        if seen[k]:
+            raise ValueError(f"BVH: node {k} is reached twice")
+        seen[k] = True
+        if count[k] > 0:
+            if left[k] != -1:
+                raise ValueError(f"BVH: leaf {k} has a left child")
+            if first[k] < 0 or first[k] + count[k] > nf:
+                raise ValueError(f"BVH: leaf {k} holds triangles [{first[k]}, {first[k] + count[k]}) of {nf}")
+            cover[first[k]:first[k] + count[k]] += 1
+        elif count[k] == 0:
+            a = left[k]
+            if not (0 < a and a + 1 < n):
+                raise ValueError(f"BVH: inner node {k} has children {a}, {a + 1} outside (0, {n})")
+            if level[a] != level[k] + 1 or level[a + 1] != level[k] + 1:
+                raise ValueError(f"BVH: the children of node {k} are not one level deeper")
+            stack += [int(a), int(a + 1)]
+        else:
+            raise ValueError(f"BVH: node {k} has a negative triangle count")
+    if not seen.all():
+        raise ValueError(f"BVH: nodes {np.nonzero(~seen)[0][:5].tolist()} are not reached from the root")
+    if (cover != 1).any():
+        raise ValueError("BVH: the leaves do not hold every triangle exactly once")
+    if int(bvh["max_level"]) != int(level[:n].max()):
+        raise ValueError("BVH: max_level differs from the deepest node level")
+
+
 class MeshTopology:
     """Host-side topology of one mesh plus its BVH (built once)."""
 
@@ -168,6 +220,9 @@ class MeshTopology:
         self.level_info = np.stack([np.where(leaf[nodes], self.first[nodes], self.left[nodes]),
                                     self.count[nodes]], axis=1).astype(np.int32)
         self.tri_colors_leaf = mesh.tri_colors[tri_map] if mesh.tri_colors is not None else None
+        # texture coordinates of each leaf-order triangle's corners (s, t) x 3
+        self.tri_uv_leaf = (np.ascontiguousarray(mesh.uv[mesh.tris[tri_map]].reshape(nf, 6)).astype(np.float32)
+                            if mesh.uv is not None and mesh.texture is not None else None)
         # vertex -> faces adjacency for the normals: faces of each vertex in
         # increasing original face id, stored as leaf positions
         leaf_of = np.empty(nf, np.int64)
@@ -381,7 +436,57 @@ class SimModelDO:
         tc = [t.tri_colors_leaf if t.tri_colors_leaf is not None else np.ones((t.mesh.num_tris, 3), np.float32)
               for t in self.topo]
         self.t_tri_colors = torch.from_numpy(cat(tc).astype(np.float32)).to(dev)
+        self._build_textures()
         self.n_tet_total = int(self.t_tets.shape[0])
+
+    def _build_textures(self):
+        """Texture table of the render meshes (colour mode 1): the texels of
+        every mesh that has a texture (float4, each texture's consecutive),
+        (first texel, side) per texture, and per leaf-order triangle its
+        corners' texture coordinates and texture id (-1: vertex colours)."""
+        dev = self.device
+        uv, tex_id, texels, info, first = [], [], [], [], 0
+        for t in self.topo:
+            m, nf = t.mesh, t.mesh.num_tris
+            if t.tri_uv_leaf is None:
+                uv.append(np.zeros((nf, 6), np.float32))
+                tex_id.append(np.full(nf, -1, np.int32))
+                continue
+            side = int(m.texture.shape[0])
+            tx = np.zeros((side * side, 4), np.float32)
+            tx[:, :3] = m.texture.reshape(-1, 3)
+            uv.append(t.tri_uv_leaf)
+            tex_id.append(np.full(nf, len(info), np.int32))
+            texels.append(tx)
+            info.append((first, side))
+            first += side * side
+        self.has_textures = bool(info)
+        self._texels_host = None
+        self.t_tri_uv = torch.from_numpy(np.concatenate(uv)).to(dev) if self.has_textures else None
+        self.t_tri_tex = torch.from_numpy(np.concatenate(tex_id)).to(dev) if self.has_textures else None
+        self.t_texels = torch.from_numpy(np.concatenate(texels)).to(dev) if self.has_textures else None
+        self.tex_info_host = np.asarray(info, np.int64).reshape(-1, 2)
+        self.t_tex_info = torch.from_numpy(self.tex_info_host).to(dev) if self.has_textures else None
+        self.n_texels = first
+
+    def texture_inputs(self, mesh: int) -> dict:
+        """The texture arrays of mesh ``mesh`` by original face id, with the
+        model's whole texel table (the oracle's render_rgb inputs)."""
+        t = self.topo[mesh]
+        nf = t.mesh.num_tris
+        out = {"tri_tex": np.full(nf, -1, np.int32), "tri_uv": np.zeros((nf, 3, 2), np.float32),
+               "texels": np.zeros((1, 3), np.float32), "tex_info": np.zeros((1, 2), np.int64)}
+        if not self.has_textures or t.tri_uv_leaf is None:
+            return out
+        base = int(self.mesh_off["f"][mesh])
+        tid = self.t_tri_tex[base:base + nf].cpu().numpy()
+        out["tri_tex"][t.tri_map] = tid
+        out["tri_uv"][t.tri_map] = t.tri_uv_leaf.reshape(nf, 3, 2)
+        if getattr(self, "_texels_host", None) is None:
+            self._texels_host = self.t_texels[:, :3].cpu().numpy()
+        out["texels"] = self._texels_host
+        out["tex_info"] = self.tex_info_host
+        return out
 
     def _build_envs(self):
         n = self.numEnvs
@@ -410,6 +515,7 @@ class SimModelDO:
             diag = max(m.bbox_diagonal(), 1e-6)
             row[F["dp_exp"]] = int(math.floor(math.log2(diag))) - 27
             row[F["vtopo_base"]] = self.mesh_off["vt"][k]
+            row[F["max_terms"]] = self._max_terms(k)
             vb += m.num_vertices
             nb += t.num_nodes
             if carry:
@@ -427,6 +533,25 @@ class SimModelDO:
         self.max_nv = int(max(m.num_vertices for m in self.meshes)) if self.meshes else 0
         work = np.array([self.meshes[k].num_tets * 4 + self.meshes[k].num_edges for k in self.env_mesh])
         self.work = work
+
+    def _max_terms(self, k: int) -> int:
+        """Most Jacobi terms one vertex of mesh k receives in one pass of the
+        batched frame: a tet adds one term to each of its vertices (volume and
+        Neo-Hookean summed), an edge one to each end; the collision pass adds
+        one per incident surface triangle."""
+        cache = self.__dict__.setdefault("_terms", {})
+        if k not in cache:
+            m = self.meshes[k]
+            inc = np.zeros(m.num_vertices, np.int64)
+            if self.vol_mode != OFF or self.nh_on:
+                inc += np.bincount(m.tets.reshape(-1), minlength=m.num_vertices)
+            if self.dist_mode != OFF:
+                inc += np.bincount(m.edges.reshape(-1), minlength=m.num_vertices)
+            terms = int(inc.max()) if inc.size else 0
+            if self.collision and m.num_tris:
+                terms = max(terms, int(np.bincount(m.tris.reshape(-1), minlength=m.num_vertices).max()))
+            cache[k] = terms
+        return cache[k]
 
     def _build_state(self, initial_velocity, tips, bases, radii):
         dev = self.device
@@ -465,6 +590,7 @@ class SimModelDO:
         self.vertexColors = torch.from_numpy(vc.astype(np.float32)).to(dev)
         self.normals = torch.zeros((V, 4), dtype=torch.float32, device=dev)
         self.sched = torch.zeros(2, dtype=torch.int32, device=dev)      # frame work counters
+        self.status = torch.zeros(max(self.numEnvs, 1), dtype=torch.int32, device=dev)   # FFSRL_STATUS_* bits
         self.dp = None
         self.cnt = None
 
@@ -490,6 +616,11 @@ class SimModelDO:
         b.level_off = _ptr(self.t_level_off)
         b.vf_off = _ptr(self.t_vf_off)
         b.vf_idx = _ptr(self.t_vf_idx)
+        b.tri_uv = _ptr(self.t_tri_uv)
+        b.tri_tex = _ptr(self.t_tri_tex)
+        b.texels = _ptr(self.t_texels)
+        b.tex_info = _ptr(self.t_tex_info)
+        b.n_texels = self.n_texels
         b.x = _ptr(self.x)
         b.p = _ptr(self.p)
         b.v = _ptr(self.v)
@@ -507,6 +638,7 @@ class SimModelDO:
         b.cnt = _ptr(self.cnt)
         b.split_acc = _ptr(getattr(self, "split_acc", None))
         b.split_ctr = _ptr(getattr(self, "split_ctr", None))
+        b.status = _ptr(self.status)
         b.host_envs = self.env_desc_host.ctypes.data
         b.host_level_off = self.level_off_host.ctypes.data
         return b
@@ -638,10 +770,7 @@ class SimModelDO:
         ``.npz``).  Every array that depends on the topology is rebuilt; the
         simulation state is kept; the bounds are zero until the next refit."""
         old = self.topo[mesh]
-        nf = self.meshes[mesh].num_tris
-        tri_map = np.asarray(bvh["tri_map"])
-        if tri_map.shape != (nf,) or not np.array_equal(np.sort(tri_map), np.arange(nf)):
-            raise ValueError(f"BVH triangle map does not permute the {nf} triangles of mesh {mesh}")
+        validate_bvh(bvh, self.meshes[mesh].num_tris)
         self.topo[mesh] = MeshTopology(self.meshes[mesh], 8, old.constraint_order, old.chunk, bvh=bvh)
         self._build_topology()
         self._build_envs()
@@ -672,10 +801,30 @@ class SimModelDO:
             if lib.ffsrl_xcd_count() <= 0:
                 raise RuntimeError(f"ffsrl_xcd_count: {lib.ffsrl_last_error().decode()}")
 
-    def shared_status(self) -> torch.Tensor:
-        """Per-env status words of the shared-env frame (0: every frame
-        completed; 1: a pass wait gave up; 2: a frame ended incomplete)."""
-        return self.split_ctr[:, 2]
+    def frame_status(self) -> torch.Tensor:
+        """Per-env status bits of the batched and shared frames, sticky until
+        ``clear_status``: STATUS_SATURATED (a fixed-point Jacobi term left
+        its range or was not finite, so the frame no longer equals the plain
+        form) and STATUS_INCOMPLETE (a shared frame could not complete)."""
+        return self.status[:self.numEnvs]
+
+    shared_status = frame_status
+
+    def clear_status(self) -> None:
+        self.status.zero_()
+
+    def check_status(self) -> None:
+        """Raises FFSRLError naming the envs with a nonzero status (a host
+        synchronisation: call it at the caller's own sync points)."""
+        st = self.frame_status().cpu().numpy()
+        bad = np.nonzero(st)[0]
+        if bad.size:
+            kinds = []
+            if (st & STATUS_SATURATED).any():
+                kinds.append(f"{int(((st & STATUS_SATURATED) != 0).sum())} with a saturated fixed-point term")
+            if (st & STATUS_INCOMPLETE).any():
+                kinds.append(f"{int(((st & STATUS_INCOMPLETE) != 0).sum())} with an incomplete shared frame")
+            raise FFSRLError(f"frame status set for {bad.size} envs ({', '.join(kinds)}), first env {int(bad[0])}")
 
     @property
     def batch(self) -> Batch:

@@ -11,6 +11,7 @@ float64 and rounded to float32; every environment reads the same table.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 
 import numpy as np
@@ -22,10 +23,11 @@ from .model import SimModelDO
 
 class RenderMode:
     grayscale = 0
-    texture = 1          # per-triangle colours stand in for texture lookups
+    texture = 1          # the reference's default: texture lookup, vertex colours where a mesh has no texture
     normal_map = 2
     vertex_color = 3
     depth_map = 4
+    triangle_color = 5   # per-triangle colours (seeded; no reference counterpart)
 
 
 def quat_rpy(roll: float, pitch: float, yaw: float) -> np.ndarray:
@@ -135,13 +137,19 @@ class RaycastRendererDO:
         self.render_mode = render_mode
         self.camera.mode = int(render_mode)
 
+    def depth_camera(self) -> Camera:
+        """A copy of the camera in depth mode (the renderer keeps its mode)."""
+        cam = Camera()
+        C.pointer(cam)[0] = self.camera
+        cam.mode = RenderMode.depth_map
+        return cam
+
     def render_depth(self, env_begin: int = 0, env_end: int = None, hit=None, thit=None) -> torch.Tensor:
         m = self.simModel
         ee = m.numEnvs if env_end is None else env_end
-        if self.camera.mode != RenderMode.depth_map:
-            self.set_mode(RenderMode.depth_map)
+        cam = self.camera if self.camera.mode == RenderMode.depth_map else self.depth_camera()
         out = self.depth[env_begin:ee]
-        lib.ffsrl_render(m.batch, self.camera, env_begin, ee, out.data_ptr(), None, None, None, None,
+        lib.ffsrl_render(m.batch, cam, env_begin, ee, out.data_ptr(), None, None, None, None,
                          None if hit is None else hit.data_ptr(), None if thit is None else thit.data_ptr(),
                          m.stream())
         return out

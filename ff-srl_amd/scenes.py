@@ -50,6 +50,8 @@ class TetMesh:
     tris: np.ndarray             # (F, 3) int32 boundary faces, outward
     colors: np.ndarray = None    # (V, 3) float32 vertex colours (render)
     tri_colors: np.ndarray = None  # (F, 3) float32 per-triangle colours
+    uv: np.ndarray = None        # (V, 2) float32 texture coordinates of the vertices (texture mode)
+    texture: np.ndarray = None   # (side, side, 3) float32 texels, or None (vertex colours are drawn)
 
     @property
     def num_vertices(self) -> int:
@@ -133,7 +135,28 @@ def _finish(name, verts32, tets, inv_mass, rng=None) -> TetMesh:
     return TetMesh(name=name, verts=verts32, inv_mass=inv_mass.astype(np.float32),
                    tets=tets.astype(np.int32), tet_rest_volume=rest_vol, tet_inv_rest=inv_rest,
                    edges=edges, edge_rest=edge_rest, tris=tris.astype(np.int32),
-                   colors=colors, tri_colors=tri_colors)
+                   colors=colors, tri_colors=tri_colors, uv=planar_uv(verts32), texture=seeded_texture(rng))
+
+
+def planar_uv(verts: np.ndarray) -> np.ndarray:
+    """Texture coordinates of a front (x, y) projection: s = (x - xmin) /
+    (xmax - xmin), t = (y - ymin) / (ymax - ymin) of the rest positions."""
+    v = np.asarray(verts, np.float64)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    ext = np.where(hi > lo, hi - lo, 1.0)
+    return ((v[:, :2] - lo[:2]) / ext[:2]).astype(np.float32)
+
+
+def seeded_texture(rng, side: int = 64, cells: int = 8) -> np.ndarray:
+    """A side x side RGB texture: colours drawn on a coarse (cells + 1)^2 grid
+    and interpolated bilinearly between its nodes, row-major (side, side, 3)."""
+    grid = rng.uniform(0.2, 1.0, size=(cells + 1, cells + 1, 3))
+    c = np.linspace(0.0, cells, side)
+    i0 = np.minimum(np.floor(c).astype(np.int64), cells - 1)
+    f = (c - i0)[:, None, None]
+    rows = grid[i0] * (1.0 - f) + grid[i0 + 1] * f                        # (side, cells + 1, 3)
+    g = (c - i0)[None, :, None]
+    return (rows[:, i0] * (1.0 - g) + rows[:, i0 + 1] * g).astype(np.float32)
 
 
 def cube_grid(n: int = 10, size: float = 1.0, orientation: int = 1, seed: int = 0) -> TetMesh:

@@ -434,7 +434,10 @@ def serve(server: SceneServer, requests: Sequence[Request], duration: float, min
     wall = server.now()
     if gc_was:
         gc.enable()
-    return summarize(server.completions, wall, duration)
+    out = summarize(server.completions, wall, duration)
+    st = server.model.frame_status().cpu().numpy()
+    out["status_scenes"] = int((st != 0).sum())      # slots whose frames reported a status bit
+    return out
 
 
 def summarize(completions: Sequence[Completion], wall: float, duration: float) -> dict:

@@ -26,11 +26,22 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 9
+#define FFSRL_ABI_VERSION 10
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
 #define FFSRL_E_STATE (-3)    /* call made in the wrong state */
+
+/* Per-env status bits of the batched and shared frames (ffsrl_batch.status),
+ * sticky until the caller clears them. */
+#define FFSRL_STATUS_SATURATED 1u   /* a Jacobi term beyond the fixed-point range, or not finite:
+                                       clamped, so the frame no longer equals the plain form */
+#define FFSRL_STATUS_INCOMPLETE 2u  /* a shared frame could not complete (a bounded wait gave up) */
+
+/* Most Jacobi terms one vertex may receive in one pass of the batched frame
+ * (ffsrl_env_desc.max_terms): each term is clamped to 2^24 - 1 quanta, so
+ * 128 of them cannot overflow the 32-bit fixed-point sums. */
+#define FFSRL_MAX_TERMS 128
 
 /* Constraint modes (per constraint family). */
 #define FFSRL_OFF 0
@@ -55,7 +66,10 @@ typedef struct ffsrl_env_desc {
                               frame: 2^dp_exp (about 2^-27 x bbox diagonal) */
   int32_t vtopo_base;      /* first vf_off entry of the env's mesh */
   int32_t bvh_depth;       /* levels of the env's BVH (max node level + 1) */
-  int32_t reserved1, reserved2;
+  int32_t max_terms;       /* most Jacobi terms of one vertex in one pass: incident tets plus
+                              incident edges, or incident surface triangles with collision;
+                              the batched frames reject envs above FFSRL_MAX_TERMS */
+  int32_t reserved2;
 } ffsrl_env_desc;
 
 /* Device arrays of a batch of environments. */
@@ -84,6 +98,15 @@ typedef struct ffsrl_batch {
   const int32_t* vf_off;          /* per mesh [nv+1] at vtopo_base: CSR offsets of each vertex's faces */
   const int32_t* vf_idx;          /* per mesh [3F] at 3*fbase: leaf positions of a vertex's faces,
                                      in increasing original face id */
+  /* textures of the render mesh (colour mode 1); NULL when no mesh has one */
+  const float* tri_uv;            /* [F][6] per render triangle (leaf order): texture coordinates
+                                     (s, t) of its corners a, b, c */
+  const int32_t* tri_tex;         /* [F] texture of each render triangle (leaf order), or -1: the
+                                     mean of its vertex colours */
+  const float* texels;            /* float4 [n_texels]: RGB texels of every texture, the texels of
+                                     each square texture consecutive */
+  const int64_t* tex_info;        /* [n_tex][2]: first texel and side of each texture */
+  int64_t n_texels;
   /* state (per environment) */
   float* x;                       /* float4 [V] positions + inverse mass */
   float* p;                       /* float4 [V] predicted positions + inverse mass */
@@ -105,8 +128,10 @@ typedef struct ffsrl_batch {
   /* shared-env frame scratch (ffsrl_frame_shared; NULL otherwise) */
   uint64_t* split_acc;            /* [V][2] fixed-point Jacobi sums; zero before the first
                                      frame, left zero by every frame */
-  uint32_t* split_ctr;            /* [n_env][4] chunk counters (taken, done, status, exits);
-                                     zero before the first frame, left zero except status */
+  uint32_t* split_ctr;            /* [n_env][4] chunk counters (taken, done, give-up flag, exits);
+                                     zero before the first frame, left zero by every frame */
+  int32_t* status;                /* [n_env] FFSRL_STATUS_* bits, or'ed by the batched and shared
+                                     frames (required by them); the caller clears them */
   /* host mirrors needed by the plain form, which launches per env (and per
    * BVH level) from the host: the env table and the level offsets */
   const ffsrl_env_desc* host_envs;   /* [n_env] (host memory) */
@@ -138,7 +163,8 @@ typedef struct ffsrl_camera {
   float origin[3];
   const float* dirs;              /* float4 [2H*2W] (device), row-major ray order */
   int32_t height, width;          /* output pixels per env; width % 4 == 0, height % 4 == 0 */
-  int32_t mode;                   /* 4 depth, 3 vertex colour, 2 normal, 0 grayscale, 1 triangle colour */
+  int32_t mode;                   /* 4 depth, 3 vertex colour, 2 normal, 0 grayscale, 1 texture (vertex
+                                     colours where a triangle has none), 5 per-triangle colour */
   float light_dir[3];
   float light_intensity;
 } ffsrl_camera;
@@ -169,7 +195,10 @@ const char* ffsrl_last_error(void);
  * Replaces SimIntegratorDO.stepModel (integrator.py:1640-1744) followed by
  * SimBVH.refitBVH (bvh.py:442-451).  `order` (device, may be NULL) gives the
  * dispatch order of environments (largest first balances mixed sizes).
- * block = threads per environment (256, 512 or 1024). */
+ * block = threads per environment (256, 512 or 1024).  The Jacobi sums are
+ * fixed point: an env whose terms leave the range (or are not finite) gets
+ * FFSRL_STATUS_SATURATED in status[env].  Envs above FFSRL_MAX_TERMS (read
+ * from host_envs when given) are rejected with FFSRL_E_LIMIT. */
 int ffsrl_frame(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* order, int32_t block,
                 int32_t refit, void* stream);
 
@@ -189,9 +218,10 @@ int ffsrl_frame_envs(const ffsrl_batch* b, const ffsrl_params* prm, const int32_
  * envs NULL: environments 0 .. n-1.
  * A workgroup only waits for chunks that running workgroups hold, so the
  * launch needs no co-residency; waits are bounded, and a frame that could
- * not complete sets split_ctr[4*env+2] (status) to nonzero.  Launches that
- * overlap on one environment race.  Same reference counterpart as
- * ffsrl_frame. */
+ * not complete sets FFSRL_STATUS_INCOMPLETE in status[env] (its global
+ * sums are cleared and its counters reset, so the next frame starts
+ * clean).  Launches that overlap on one environment race.  Same reference
+ * counterpart as ffsrl_frame. */
 int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n,
                        int32_t block, int32_t refit, void* stream);
 
@@ -209,7 +239,14 @@ int ffsrl_frame_max_vertices(void);
 int ffsrl_bvh_refit(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, void* stream);
 
 /* Ray-cast render of environments [env_begin, env_end) with the 2x2 box
- * filter.  depth: [n][H][W] (mode 4) ; rgb: [n][H][W][3] (other modes);
+ * filter.  Colour mode 1 is the reference's texture mode (render.py:356-380):
+ * the hit's texture coordinates u s_a + v s_b + w s_c (batch tri_uv) looked
+ * up bilinearly in its triangle's texture (texture_interpolation,
+ * render.py:124-148, with its y + x * width indexing; reads past the texel
+ * table give zero), or the mean of the triangle's vertex colours (vcolor,
+ * required) where it has no texture; mode 5 uses per-triangle colours
+ * (tcolor, leaf order).
+ * depth: [n][H][W] (mode 4) ; rgb: [n][H][W][3] (other modes);
  * a colour mode given a depth output too writes both from one traversal
  * (RGB-D), each equal to what its own mode writes;
  * hit / thit (optional, may be NULL): per-ray face ids and t, [n][2H][2W].
@@ -399,6 +436,10 @@ int ffsrl_graph_begin(void* stream);
 int ffsrl_graph_end(void* stream, void** graph_exec);
 int ffsrl_graph_launch(void* graph_exec, void* stream);
 int ffsrl_graph_destroy(void* graph_exec);
+/* Ends a capture begun with ffsrl_graph_begin and discards what it recorded
+ * (for a caller whose work failed mid-capture; the stream leaves capture
+ * mode).  No reference counterpart. */
+int ffsrl_graph_abort(void* stream);
 
 #ifdef __cplusplus
 }

@@ -21,10 +21,14 @@ Parity status of each restated function (see the tests that pin them):
   refit                             - pinned: brute-force min/max, bit-exact
   ray cast (AABB, MT, traversal)    - pinned: brute-force nearest hit, analytic
                                       plane scene
-  BVH build (bvh_build below)       - pinned: invariants (partition, nesting,
-                                      SAH stop rule); exact float promotion of
-                                      the reference's NumPy 1.x code is parity
-                                      unpinned
+  BVH build (bvh_build below)       - pinned: a brute-force SAH (explicit bin
+                                      sets, no prefix sweep) node for node,
+                                      hand-built trees, a hand-worked
+                                      two-pointer partition; planted mistakes
+                                      fail (tests/test_bvh_sah_pin.py); the
+                                      float64 bin arithmetic is an assumption
+                                      about the reference's NumPy 1.x / Warp
+                                      promotion (parity unpinned at bin edges)
   instrument collision              - pinned: sphere-on-triangle closed form,
                                       no-contact cases, the barycentric-point
                                       quirk, summed responses
@@ -249,32 +253,88 @@ def vertex_normals(faces, vertex, dtype=np.float32):
 
 
 def render_rgb(H, W, mode, origin, dirs, bvh, aabb_min, aabb_max, faces, vertex, normals, vcolor=None, tcolor=None,
-               light=(3.0, 0.0, 0.0), intensity=0.3, dtype=np.float32):
-    """Restated drawSimBVHKernelManyEnvs colour modes (0 grayscale, 1 per-face
-    colour, 2 normal map, 3 vertex colour) + per-channel downsample.
-    Returns rgb (H, W, 3)."""
+               light=(3.0, 0.0, 0.0), intensity=0.3, dtype=np.float32, tri_uv=None, tri_tex=None, texels=None,
+               tex_info=None):
+    """Restated drawSimBVHKernelManyEnvs colour modes (0 grayscale, 1 texture
+    with the vertex-colour fallback, 2 normal map, 3 vertex colour, 5
+    per-face colour) + per-channel downsample.  Texture inputs, by original
+    face id: tri_uv (F, 3, 2) corner texture coordinates, tri_tex (F,) texture
+    id or -1; texels (N, 3) of every texture; tex_info (n_tex, 2) = (first
+    texel, side).  Returns rgb (H, W, 3)."""
     rgb = np.zeros((H, W, 3), dtype)
     sfx = "_f32" if dtype == np.float32 else "_f64"
     fn = getattr(lib(), "oracle_render_rgb" + sfx)
     i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
     f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=dtype)
+    nf = np.asarray(faces).reshape(-1, 3).shape[0]
+    if tri_tex is None:
+        tri_tex = np.full(nf, -1, np.int32)
+    if tri_uv is None:
+        tri_uv = np.zeros((nf, 3, 2))
+    texels = np.zeros((1, 3)) if texels is None else np.asarray(texels).reshape(-1, 3)
+    tex_info = np.zeros((1, 2), np.int64) if tex_info is None else np.asarray(tex_info, np.int64).reshape(-1, 2)
     arrs = [f(origin), f(np.asarray(dirs)[:, :3]), f(aabb_min), f(aabb_max), i32(bvh["left"]), i32(bvh["first"]),
             i32(bvh["count"]), i32(bvh["tri_map"]), i32(np.asarray(faces).reshape(-1)), f(vertex), f(normals),
-            f(vcolor), f(tcolor), f(np.asarray(light))]
-    fn(C.c_int(H), C.c_int(W), C.c_int(mode), *[_p(a) for a in arrs], (C.c_float if dtype == np.float32 else
-                                                                      C.c_double)(intensity), _p(rgb))
+            f(vcolor), f(tcolor), f(np.asarray(tri_uv).reshape(-1)), i32(tri_tex), f(texels)]
+    info = np.ascontiguousarray(tex_info, np.int64)
+    fn(C.c_int(H), C.c_int(W), C.c_int(mode), *[_p(a) for a in arrs], C.c_long(texels.shape[0]), _p(info),
+       _p(f(np.asarray(light))), (C.c_float if dtype == np.float32 else C.c_double)(intensity), _p(rgb))
     return rgb
 
 
+def texture_lookup(s, t, texels, shift, size, dtype=np.float32):
+    """Restated texture_interpolation (render.py:124-148) at texture
+    coordinates (s, t) of the square texture of side `size` starting at
+    texel `shift` of the table `texels` (N, 3)."""
+    texels = np.ascontiguousarray(np.asarray(texels).reshape(-1, 3), dtype)
+    out = np.zeros(3, dtype)
+    real = C.c_float if dtype == np.float32 else C.c_double
+    sfx = "_f32" if dtype == np.float32 else "_f64"
+    getattr(lib(), "oracle_texture_lookup" + sfx)(real(s), real(t), _p(texels), C.c_long(texels.shape[0]),
+                                                  C.c_long(shift), C.c_int(size), _p(out))
+    return out
+
+
 # ------------------------------------------------------------ BVH build
-def bvh_build(vertex, faces, bins=8):
+MUTANTS = ("partition_le", "right_sweep_off_by_one", "no_bin_clamp")
+
+
+def sah_partition(tri, first, count, cen_axis, pos, le=False):
+    """The in-place two-pointer partition of subdivide (bvh.py:367-376):
+    tri[first:first+count] is reordered so that the triangles whose centroid
+    coordinate is < pos come first; returns their number.  ``le`` (test
+    mutant only) takes <= instead of <."""
+    i, j = first, first + count - 1
+    while i <= j:
+        c = float(cen_axis[tri[i]])
+        if (c <= pos) if le else (c < pos):
+            i += 1
+        else:
+            tri[i], tri[j] = tri[j], tri[i]
+            j -= 1
+    return i - first
+
+
+def bvh_build(vertex, faces, bins=8, mutant=None):
     """NumPy restatement of SimBVH.buildBVH (bvh.py:456-518) with
     findBestSplitPlaneBinned (bvh.py:281-340) and subdivide (bvh.py:354-401).
 
     Centroids are float32 (calculateCentroids, bvh.py:6-21); bin positions,
-    areas and costs are float64.  Returns left / first / count / level arrays
-    (nodes in allocation order), the triangle permutation and the max level.
+    areas and costs are float64.  The float64 choice is an assumption about
+    the reference's promotion: it mixes np.float32 centroids with Python
+    floats (and the Python-scope wp.min of Warp 0.10.1) in
+    findBestSplitPlaneBinned; the min/max centroid bounds are exact either
+    way, and a float32 evaluation could bin differently only a centroid
+    within rounding of a bin edge.  Returns left / first / count / level
+    arrays (nodes in allocation order), the triangle permutation and the max
+    level.
+
+    ``mutant`` plants one of MUTANTS (a plausible misreading of the
+    reference) so that tests can show their checks catch it; it is never set
+    outside tests.
     """
+    if mutant is not None and mutant not in MUTANTS:
+        raise ValueError(f"unknown mutant {mutant!r}")
     vertex = np.asarray(vertex, dtype=np.float32)
     faces = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
     nf = faces.shape[0]
@@ -308,18 +368,28 @@ def bvh_build(vertex, faces, bins=8):
             if hi_c == lo_c:
                 continue
             scale = bins / (hi_c - lo_c)
-            b = np.minimum(bins - 1, ((c - lo_c) * scale).astype(np.int64))
+            b = ((c - lo_c) * scale).astype(np.int64)
+            if mutant == "no_bin_clamp":        # the top centroid lands past the last bin
+                keep = b < bins
+                b, bt_lo, bt_hi = b[keep], tmin[ids][keep], tmax[ids][keep]
+            else:
+                b = np.minimum(bins - 1, b)
+                bt_lo, bt_hi = tmin[ids], tmax[ids]
             cnt = np.bincount(b, minlength=bins)
             blo = np.full((bins, 3), 1e30)
             bhi = np.full((bins, 3), -1e30)
-            np.minimum.at(blo, b, tmin[ids])
-            np.maximum.at(bhi, b, tmax[ids])
+            np.minimum.at(blo, b, bt_lo)
+            np.maximum.at(bhi, b, bt_hi)
             lcnt = np.cumsum(cnt)[:-1]
             rcnt = np.cumsum(cnt[::-1])[:-1][::-1]
             llo = np.minimum.accumulate(blo, axis=0)[:-1]
             lhi = np.maximum.accumulate(bhi, axis=0)[:-1]
             rlo = np.minimum.accumulate(blo[::-1], axis=0)[:-1][::-1]
             rhi = np.maximum.accumulate(bhi[::-1], axis=0)[:-1][::-1]
+            if mutant == "right_sweep_off_by_one":   # right side of plane j starts one bin late
+                rcnt = np.append(rcnt[1:], 0)
+                rlo = np.vstack([rlo[1:], np.full((1, 3), 1e30)])
+                rhi = np.vstack([rhi[1:], np.full((1, 3), -1e30)])
             cost = lcnt * area(llo, lhi) + rcnt * area(rlo, rhi)
             step = (hi_c - lo_c) / bins
             for j in range(bins - 1):
@@ -332,14 +402,8 @@ def bvh_build(vertex, faces, bins=8):
         parent = count[node] * area(bmin[node], bmax[node])
         if best >= parent:
             return
-        i, j = first[node], first[node] + count[node] - 1
-        while i <= j:
-            if float(cen[tri[i], axis]) < pos:
-                i += 1
-            else:
-                tri[i], tri[j] = tri[j], tri[i]
-                j -= 1
-        lc = i - first[node]
+        lc = sah_partition(tri, first[node], count[node], cen[:, axis], pos, le=(mutant == "partition_le"))
+        i = first[node] + lc
         if lc == 0 or lc == count[node]:
             return
         a, b = used[0], used[0] + 1

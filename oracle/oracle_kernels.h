@@ -645,10 +645,54 @@ void FN(oracle_vertex_normals)(long nf, const int* faces, long nv, const REAL* v
   for (long v = 0; v < nv; ++v) FN(st)(normals, v, FN(normalize)(FN(ld)(normals, v)));
 }
 
+/* float -> int as Warp's int() on the device: truncation toward zero;
+ * NaN gives 0 and values beyond the int range saturate. */
+static inline int FN(trunc_i)(REAL x) {
+  if (x != x) return 0;
+  if (x >= R(2147483647.0)) return 2147483647;
+  if (x <= R(-2147483648.0)) return -2147483647 - 1;
+  return (int)x;
+}
+
+/* One texel of the concatenated texture table (RGB triples, n_texels of
+ * them); an index outside the table reads as zero (the reference reads past
+ * a texture's last column, where its weight is zero). */
+static inline V3 FN(texel)(const REAL* texels, long n_texels, long i) {
+  if (i < 0 || i >= n_texels) return FN(v3)(R(0), R(0), R(0));
+  return FN(ld)(texels, i);
+}
+
+/* texture_interpolation (render.py:124-148): bilinear lookup of a square
+ * texture of side `size` starting at texel `shift`, with the reference's
+ * indexing y + x * width (x along the first texture coordinate) and no clamp
+ * of x0 + 1 / y0 + 1. */
+V3 FN(oracle_texture_interp)(REAL s, REAL t, const REAL* texels, long n_texels, long shift, int size) {
+  REAL tx = s * (REAL)(size - 1), ty = (R(1.0) - t) * (REAL)(size - 1);
+  int x0 = FN(trunc_i)(tx), y0 = FN(trunc_i)(ty);
+  int x1 = x0 + 1, y1 = y0 + 1;
+  REAL ax = tx - (REAL)x0, ay = ty - (REAL)y0;
+  V3 c00 = FN(texel)(texels, n_texels, shift + y0 + (long)x0 * size);
+  V3 c10 = FN(texel)(texels, n_texels, shift + y0 + (long)x1 * size);
+  V3 c01 = FN(texel)(texels, n_texels, shift + y1 + (long)x0 * size);
+  V3 c11 = FN(texel)(texels, n_texels, shift + y1 + (long)x1 * size);
+  V3 lower = FN(add)(FN(mul)(c00, R(1.0) - ax), FN(mul)(c10, ax));
+  V3 upper = FN(add)(FN(mul)(c01, R(1.0) - ax), FN(mul)(c11, ax));
+  return FN(add)(FN(mul)(lower, R(1.0) - ay), FN(mul)(upper, ay));
+}
+
+/* Entry point of the texture lookup for tests: out = RGB at (s, t). */
+void FN(oracle_texture_lookup)(REAL s, REAL t, const REAL* texels, long n_texels, long shift, int size, REAL* out) {
+  FN(st)(out, 0, FN(oracle_texture_interp)(s, t, texels, n_texels, shift, size));
+}
+
 /* drawSimBVHKernelManyEnvs (render.py:276-409) colour modes for one
  * environment, plus downsample_kernel (render.py:699-724) per channel.
- *   mode 0 grayscale, 3 vertex colour, 1 per-triangle colour (stands in for
- *   the texture lookup, indexed by face id), each times the Lambert term
+ *   mode 0 grayscale; 1 texture (render.py:356-380): the hit triangle's
+ *   texture coordinates u tex_a + v tex_b + w tex_c (tri_uv: 6 per face id),
+ *   looked up in texture tri_tex[face] (tex_info: first texel, side per
+ *   texture), or, for a triangle without a texture (tri_tex < 0), the mean
+ *   of its vertex colours; 3 vertex colour; 5 per-triangle colour (tcolor,
+ *   this build's seeded colours); each times the Lambert term
  *   max(0, intensity * dot(normal, light)) of the single light whose
  *   direction is the raw light position (render.py:387-393); 2 normal map.
  * Barycentrics from cross-product lengths, normal = u na + v nb + w nc
@@ -657,7 +701,8 @@ void FN(oracle_render_rgb)(int H, int W, int mode, const REAL* origin, const REA
                            const REAL* aabbMin, const REAL* aabbMax, const int* leftNode,
                            const int* firstTri, const int* triCount, const int* triMap,
                            const int* faces, const REAL* vertex, const REAL* normals, const REAL* vcolor,
-                           const REAL* tcolor, const REAL* light, REAL intensity, REAL* rgb) {
+                           const REAL* tcolor, const REAL* tri_uv, const int* tri_tex, const REAL* texels,
+                           long n_texels, const long* tex_info, const REAL* light, REAL intensity, REAL* rgb) {
   int RW = 2 * W, RH = 2 * H, ms = 0;
   V3 o = FN(v3)(origin[0], origin[1], origin[2]);
   V3 L = FN(v3)(light[0], light[1], light[2]);
@@ -680,9 +725,19 @@ void FN(oracle_render_rgb)(int H, int W, int mode, const REAL* origin, const REA
       if (mode == 2) {
         color = FN(add)(FN(mul)(normal, R(0.5)), FN(v3)(R(0.5), R(0.5), R(0.5)));
       } else {
-        if (mode == 0) color = FN(v3)(R(1), R(1), R(1));
-        else if (mode == 1) color = FN(ld)(tcolor, h);
-        else color = FN(divs)(FN(add)(FN(add)(FN(ld)(vcolor, i), FN(ld)(vcolor, j)), FN(ld)(vcolor, k)), R(3.0));
+        if (mode == 0) {
+          color = FN(v3)(R(1), R(1), R(1));
+        } else if (mode == 5) {
+          color = FN(ld)(tcolor, h);
+        } else if (mode == 1 && tri_tex[h] >= 0) {
+          const REAL* uv = tri_uv + 6 * (long)h;
+          REAL s = (u * uv[0] + v * uv[2]) + w * uv[4];
+          REAL tt = (u * uv[1] + v * uv[3]) + w * uv[5];
+          const long* ti = tex_info + 2 * (long)tri_tex[h];
+          color = FN(oracle_texture_interp)(s, tt, texels, n_texels, ti[0], (int)ti[1]);
+        } else {  /* vertex colour, and texture mode's fallback */
+          color = FN(divs)(FN(add)(FN(add)(FN(ld)(vcolor, i), FN(ld)(vcolor, j)), FN(ld)(vcolor, k)), R(3.0));
+        }
         REAL val = intensity * FN(dot)(normal, L);
         if (val < R(0)) val = R(0);
         REAL lambert = R(0.0) + val;

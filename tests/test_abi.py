@@ -33,7 +33,7 @@ def test_every_declared_symbol_is_exported(pkg):
 
 
 def test_abi_version(pkg):
-    assert pkg.lib.ffsrl_abi_version() == 9
+    assert pkg.lib.ffsrl_abi_version() == 10
     assert pkg.lib.ffsrl_frame_max_vertices() >= 10000
 
 
@@ -116,12 +116,52 @@ def test_argument_errors_are_reported(pkg):
     with pytest.raises(pkg.FFSRLError, match="NULL argument"):
         pkg.lib.ffsrl_observe_envs(b, None, 2, None, 0.0, None, None)
     # the shared-env frame checks its scratch and counts before any device call
+    b.status = dummy.ctypes.data
     with pytest.raises(pkg.FFSRLError, match="split_acc and split_ctr"):
         pkg.lib.ffsrl_frame_shared(b, prm, dummy.ctypes.data, 2, 1024, 0, None)
     with pytest.raises(pkg.FFSRLError, match="5 environments listed"):
         pkg.lib.ffsrl_frame_shared(b, prm, dummy.ctypes.data, 5, 1024, 0, None)
     b.split_acc = b.split_ctr = dummy.ctypes.data
+    b.status = None
+    with pytest.raises(pkg.FFSRLError, match="status array"):
+        pkg.lib.ffsrl_frame_shared(b, prm, None, 0, 1024, 0, None)
+    b.status = dummy.ctypes.data
     assert pkg.lib.ffsrl_frame_shared(b, prm, None, 0, 1024, 0, None) == 0      # nothing listed: no launch
+
+
+def test_frame_rejects_too_many_terms_per_vertex(pkg):
+    """The batched frames' fixed-point sums hold FFSRL_MAX_TERMS terms per
+    vertex and pass; an env above it (from the host mirror of the env table)
+    is rejected before any device call."""
+    L = pkg._lib
+    b = L.Batch()
+    dummy = np.zeros(64, np.float32)
+    for f in ("envs", "x", "p", "v", "drag", "tips", "status"):
+        setattr(b, f, dummy.ctypes.data)
+    envs = (L.EnvDesc * 2)()
+    envs[1].max_terms = L.MAX_TERMS + 1
+    b.host_envs = C.addressof(envs)
+    b.n_env = 2
+    prm = L.Params()
+    prm.substeps = prm.iterations = 1
+    prm.dt = 0.01
+    with pytest.raises(pkg.FFSRLError, match="env 1: a vertex receives 129 Jacobi terms"):
+        pkg.lib.ffsrl_frame(b, prm, None, 1024, 0, None)
+    envs[1].max_terms = L.MAX_TERMS
+    assert pkg.lib.ffsrl_frame_envs(b, prm, dummy.ctypes.data, 0, 1024, 0, None) == 0   # nothing listed
+
+
+def test_model_counts_terms_per_vertex(pkg):
+    """max_terms of the env table: incident tets plus edges per vertex, or
+    incident surface triangles with collision, whichever is larger."""
+    m = pkg.scenes.cube_grid(5)
+    model = pkg.SimModelDO([m], [0], device="cpu", distance="xpbd", volume="xpbd")
+    inc = np.bincount(m.tets.reshape(-1), minlength=m.num_vertices) + \
+        np.bincount(m.edges.reshape(-1), minlength=m.num_vertices)
+    F = pkg._lib.ENV_DESC_FIELDS.index("max_terms")
+    assert model.env_desc_host[0, F] == inc.max()
+    only_edges = pkg.SimModelDO([m], [0], device="cpu", distance="xpbd", volume=None)
+    assert only_edges.env_desc_host[0, F] == np.bincount(m.edges.reshape(-1)).max()
 
 
 def test_frame_rejects_envs_beyond_lds(pkg):

@@ -118,3 +118,27 @@ def test_environment_eager_graph_and_facade_agree(pkg, gpu):
         e.reset([N])
     for env in envs:
         env.close()
+
+
+def test_failed_capture_leaves_the_stream_usable(pkg, gpu):
+    """A frame that fails inside the graph capture (here an invalid block
+    size, rejected by the library before any launch) ends the capture and
+    drops the partial graph (ffsrl_graph_abort); the error reaches the
+    caller, no graph is kept, and the next step captures and replays a
+    frame equal to an eager one."""
+    envs = [pkg.SimEnvironment.from_usd(FIXTURE, 2, device=gpu, resolution=RES, useGraph=g, **REF)
+            for g in (True, False)]
+    g, eager = envs
+    g.simIntegrator.block = 333
+    with pytest.raises(pkg.FFSRLError, match="block must be"):
+        g.step()
+    assert g.capturedGraph is None and g._graph_key is None
+    g.simIntegrator.block = 1024
+    for _ in range(2):
+        g.step()
+        eager.step()
+    torch.cuda.synchronize()
+    assert g.captures == 1
+    assert torch.equal(g.simModel.x, eager.simModel.x) and torch.equal(g.simModel.v, eager.simModel.v)
+    for env in envs:
+        env.close()

@@ -484,11 +484,11 @@ def test_autotune_keeps_state_and_results(pkg, gpu, tmp_path, iters):
     assert again.tune(tuned) == cfg
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 5])
 def test_rgb_modes_and_normals_bit_exact(pkg, orc, gpu, mode):
-    """Colour renders (grayscale, per-triangle colour, normal map, vertex
-    colour, each Lambert-lit by the raw light position) and the vertex normals
-    equal the oracle's on the same deformed positions."""
+    """Colour renders (grayscale, texture lookup, normal map, vertex colour,
+    per-triangle colour, each Lambert-lit by the raw light position) and the
+    vertex normals equal the oracle's on the same deformed positions."""
     sc = pkg.scenes
     meshes = [sc.solid("ellipsoid", 3000, seed=41), sc.solid("box", 2000, seed=42)]
     tips = np.stack([np.repeat((sc.top_point(m) + np.array([0.0, 0.05, 0.0]))[None], 4, axis=0) for m in meshes])
@@ -512,7 +512,8 @@ def test_rgb_modes_and_normals_bit_exact(pkg, orc, gpu, mode):
         mn, mx = orc.refit(bvh, m.tris, x)
         ref = orc.render_rgb(64, 64, mode, np.asarray(ren.camera.origin[:], np.float32), ren.dirs.cpu().numpy(),
                              bvh, mn, mx, m.tris, x, nrm, vcolor=m.colors, tcolor=m.tri_colors,
-                             light=np.asarray(ren.camera.light_dir[:], np.float32), intensity=ren.camera.light_intensity)
+                             light=np.asarray(ren.camera.light_dir[:], np.float32), intensity=ren.camera.light_intensity,
+                             **model.texture_inputs(e))
         got = rgb[e].cpu().numpy()
         lit = float((ref.sum(-1) > 0).mean())
         print(f"\n[rgb mode {mode}] env {e}: lit pixels {lit:.2f}, max |gpu-oracle| {np.abs(got - ref).max():.2e}")
@@ -526,6 +527,7 @@ def test_rgb_modes_and_normals_bit_exact(pkg, orc, gpu, mode):
     pix, dep = ren.render_rgbd()
     pix, dep = pix.clone(), dep.clone()
     depth = ren.render_depth().clone()
+    ren.set_mode(pkg.RenderMode.depth_map)
     assert torch.equal(ren.render()[..., 0], depth)      # depth mode, per env
     torch.cuda.synchronize()
     assert torch.equal(pix, rgb) and torch.equal(dep, depth) and float((depth > 0).float().mean()) > 0.05

@@ -45,16 +45,26 @@ STATE = ("x", "v", "p", "bvh_lo", "bvh_hi")
 
 
 def _equal(a, b, extra=()):
+    """Bit-equal states, and equal status words: a saturated fixed-point
+    term (the 8-vertex grasps of the odd envs collapse their tets) is
+    reported alike by both forms."""
     torch.cuda.synchronize()
-    for k in STATE + tuple(extra):
+    for k in STATE + tuple(extra) + ("status",):
         ta, tb = getattr(a, k), getattr(b, k)
         assert torch.equal(ta, tb), (k, float((ta - tb).abs().max()))
 
 
 def _clean(m):
+    """Every shared frame completed and left its sums and counters at zero."""
     torch.cuda.synchronize()
-    assert int(m.shared_status().abs().sum()) == 0, m.shared_status()
+    incomplete = m.frame_status() & pkg_status_incomplete()
+    assert int(incomplete.abs().sum()) == 0, m.frame_status()
     assert int(m.split_ctr.abs().sum()) == 0 and int(m.split_acc.abs().sum()) == 0
+
+
+def pkg_status_incomplete():
+    import importlib
+    return importlib.import_module("ff-srl_amd")._lib.STATUS_INCOMPLETE
 
 
 @pytest.mark.parametrize("block", [1024, 512, 256])
@@ -207,3 +217,38 @@ def test_env_at_the_vertex_limit(pkg, gpu):
         assert not torch.equal(ref.x, ref.initialVertex)          # the cube moved
         if shared:
             _clean(big)
+
+
+def test_abandoned_frame_is_reported_and_cleaned(pkg, gpu):
+    """A shared frame whose bounded waits give up: the host sets two envs'
+    give-up flags before the launch, so their first wait that does not find
+    the pass complete gives up and every workgroup exits normally.  The
+    frame is reported (STATUS_INCOMPLETE) for exactly those envs, their
+    partial global sums are cleared and every counter is reset; from the
+    same starting state the next frame equals the batched frame bit for
+    bit, with no status."""
+    meshes = _meshes(pkg, 6, 4000, 20000, seed=29)
+    a, b = _pair(pkg, gpu, meshes, NH_DV, grasp=False)
+    ref, sh = pkg.SimIntegratorDO(gpu), pkg.SimIntegratorDO(gpu, shared=True)
+    sh.stepModel(b, refit=True)                       # allocates the shared workspace
+    ref.stepModel(a, refit=True)
+    _equal(a, b)
+    _clean(b)
+    saved = {k: getattr(b, k).clone() for k in STATE}
+    b.split_ctr[[1, 4], 2] = 1                        # give-up flags, as a workgroup that gave up leaves them
+    sh.stepModel(b, refit=True)
+    torch.cuda.synchronize()
+    st = b.frame_status().cpu().numpy()
+    print(f"\n[shared] status after the abandoned frame: {st.tolist()}")
+    assert st[1] & pkg._lib.STATUS_INCOMPLETE and st[4] & pkg._lib.STATUS_INCOMPLETE
+    assert (st[[0, 2, 3, 5]] == 0).all()
+    assert int(b.split_ctr.abs().sum()) == 0 and int(b.split_acc.abs().sum()) == 0
+    with pytest.raises(pkg.FFSRLError, match="incomplete shared frame"):
+        b.check_status()
+    b.clear_status()
+    for k, t in saved.items():
+        getattr(b, k).copy_(t)
+    ref.stepModel(a, refit=True)
+    sh.stepModel(b, refit=True)
+    _equal(a, b)
+    _clean(b)
