@@ -99,6 +99,7 @@ SIGNATURES = {
     "ffsrl_frame": [_P(Batch), _P(Params), _vp, _i32, _i32, _vp],
     "ffsrl_frame_envs": [_P(Batch), _P(Params), _vp, _i32, _i32, _i32, _vp],
     "ffsrl_frame_shared": [_P(Batch), _P(Params), _vp, _i32, _i32, _i32, _vp],
+    "ffsrl_frame_shared_noalloc": [_P(Batch), _P(Params), _vp, _i32, _i32, _i32, _vp],
     "ffsrl_render_envs": [_P(Batch), _P(Camera), _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ffsrl_apply_requests": [_P(Batch), _P(Requests), _f32, _vp],
     "ffsrl_observe_envs": [_P(Batch), _vp, _i32, _vp, _f32, _vp, _vp],

@@ -1240,6 +1240,83 @@ int ffsrl_xcd_count(void) {
   return n;
 }
 
+// ---------------------------------------------------------------------------
+// No-allocation shared frame (ffsrl_frame_shared_noalloc): the global sums
+// and chunk counters live in `scratch`, one copy per device for the life of
+// the process, indexed by list slot: slot el owns sums for max_nv vertices
+// at el * max_nv and counter row el.  Every frame leaves the sums and the
+// counters it used at zero (an apply chunk clears the sums it reads; the
+// last workgroup out resets the counters and clears the sums of a frame
+// that did not complete), and the array is cleared once per device at its
+// first use, so every call starts from zeros.
+constexpr int64_t kScratchAccWords = (32ll << 20) / 8;  // 32 MiB of global sums (2 words per vertex)
+constexpr int kScratchSlots = 4096;                    // list slots (counter rows of 4 words)
+constexpr int64_t kScratchWords = kScratchAccWords + kScratchSlots * 4 / 2;
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+
+__global__ void k_fill_u64(unsigned long long* a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0ull;
+}
+
+// Device address of `scratch` on the current device (fetched once per
+// device, with that device current; the array's name in host code is a
+// host-side stand-in, never a device address).
+static std::mutex g_scratch_mu;
+static void* g_scratch[64];
+
+static int scratch_of(int dev, hipStream_t st, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    FFSRL_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(FFSRL_E_STATE, "no device address for the frame scratch");
+    // cleared once, by a fill kernel on the caller's stream (ordered before the frame)
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((kScratchWords + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<unsigned long long*>(ptr), kScratchWords);
+    FFSRL_HIP(hipGetLastError());
+    g_scratch[dev] = ptr;
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
+int ffsrl_frame_shared_noalloc(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n,
+                               int32_t block, int32_t refit, void* stream) {
+  int rc = frame_checks(b, prm, n, refit);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  const int64_t slot_nv = b->max_nv < 1 ? 1 : b->max_nv;
+  if (n > kScratchSlots || (int64_t)n * slot_nv * 2 > kScratchAccWords)
+    return fail(FFSRL_E_LIMIT, "no-allocation shared frame: %d envs x %lld vertices exceed its workspace (%d envs, %lld "
+                "vertex slots)", n, (long long)slot_nv, kScratchSlots, (long long)(kScratchAccWords / 2));
+  int dev = 0;
+  FFSRL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(FFSRL_E_LIMIT, "device %d", dev);
+  int nxcd;
+  XcdMap map;
+  {
+    std::lock_guard<std::mutex> lock(g_xcd_mu);
+    nxcd = g_xcd[dev];
+    map = g_xcd_map[dev];
+  }
+  // the XCD probe allocates, so it is not run here
+  if (nxcd <= 0) return fail(FFSRL_E_STATE, "call ffsrl_xcd_count() once on this device before the no-allocation frame");
+  hipStream_t st = as_stream(stream);
+  void* ws_ptr = nullptr;
+  if ((rc = scratch_of(dev, st, &ws_ptr))) return rc;
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(ws_ptr);
+  const ShareWs ws{acc, reinterpret_cast<uint32_t*>(acc + kScratchAccWords), 1, (int)slot_nv};
+  const bool carry = carries_lambda(*prm);
+  const int lds = lds_bytes_for((int)slot_nv);
+  switch (block) {
+    case 256: return launch_shared<256>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, ws, st);
+    case 512: return launch_shared<512>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, ws, st);
+    case 1024: return launch_shared<1024>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, ws, st);
+    default: return fail(FFSRL_E_ARG, "block must be 256, 512 or 1024 (got %d)", block);
+  }
+}
+
 int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n, int32_t block,
                        int32_t refit, void* stream) {
   int rc = frame_checks(b, prm, n, refit);

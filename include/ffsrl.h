@@ -225,6 +225,22 @@ int ffsrl_frame_envs(const ffsrl_batch* b, const ffsrl_params* prm, const int32_
 int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n,
                        int32_t block, int32_t refit, void* stream);
 
+/* ffsrl_frame_shared for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * global sums and chunk counters live in the library's own device
+ * workspace, 32 MiB of sums (2,097,152 vertex slots of 16 B) plus counters
+ * for 4096 listed envs, one copy per device held until the process ends,
+ * instead of b->split_acc / b->split_ctr (ignored).  Listed env el uses
+ * vertex slots [el * max_nv, (el + 1) * max_nv): n * max_nv must stay
+ * within 2,097,152 and n within 4096, else FFSRL_E_LIMIT.  Nothing is
+ * allocated: call ffsrl_xcd_count() once per device beforehand (it
+ * allocates; FFSRL_E_STATE otherwise).  Every frame leaves the workspace at
+ * zero; calls that overlap on one device race.  Results equal
+ * ffsrl_frame_shared bit for bit.  No reference counterpart (the reference
+ * allocates through Warp). */
+int ffsrl_frame_shared_noalloc(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n,
+                               int32_t block, int32_t refit, void* stream);
+
 /* Number of XCDs of the current device (probed once with one workgroup per
  * CU; the shared frame assigns environments to XCDs).  Allocates on its
  * first call per device, so call it before capturing a shared frame into a

@@ -194,3 +194,23 @@ def test_pack16_layout(pkg):
     e = np.array([[5, 9], [10239, 0]], np.int32)
     q = pkg.model._pack16(e).view(np.uint32)
     assert [(int(q[0]) & 0xffff, int(q[0]) >> 16), (int(q[1]) & 0xffff, int(q[1]) >> 16)] == [(5, 9), (10239, 0)]
+
+
+def test_noalloc_frame_limits(pkg):
+    """ffsrl_frame_shared_noalloc refuses batches beyond its fixed workspace
+    (list slots of max_nv vertices each, 4096 envs) before any device call."""
+    L = pkg._lib
+    b = L.Batch()
+    dummy = np.zeros(64, np.float32)
+    for f in ("envs", "x", "p", "v", "drag", "tips", "status"):
+        setattr(b, f, dummy.ctypes.data)
+    b.n_env, b.max_nv = 5000, 10000
+    prm = L.Params()
+    prm.substeps = prm.iterations = 1
+    prm.dt = 0.01
+    with pytest.raises(pkg.FFSRLError, match="300 envs x 10000 vertices exceed its workspace"):
+        pkg.lib.ffsrl_frame_shared_noalloc(b, prm, dummy.ctypes.data, 300, 1024, 0, None)
+    b.max_nv = 1
+    with pytest.raises(pkg.FFSRLError, match="4097 envs"):
+        pkg.lib.ffsrl_frame_shared_noalloc(b, prm, dummy.ctypes.data, 4097, 1024, 0, None)
+    assert pkg.lib.ffsrl_frame_shared_noalloc(b, prm, None, 0, 1024, 0, None) == 0      # nothing listed
