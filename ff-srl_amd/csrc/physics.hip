@@ -1083,19 +1083,28 @@ __global__ void k_zero(float* a, int64_t base, int n) {
   if (i < n) a[base + i] = 0.0f;
 }
 
+// One thread per env: the components of delta that would take the drag
+// point (point 3) out of [lo, hi] are dropped (transfromCartesianActions-
+// KernelInWorkspace, modelDO.py:97-154), and the four points move by the
+// rest.
 __global__ void k_move_tools(ffsrl_batch b, const float* delta, const float* lo, const float* hi) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b.n_env * 4) return;
-  const int env = i >> 2;
+  const int env = blockIdx.x * blockDim.x + threadIdx.x;
+  if (env >= b.n_env) return;
+  float d[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const float t0 = b.tips[i * 3 + k];
-    float t = t0 + delta[env * 3 + k];
-    if (lo) t = fmax_(t, lo[k]);
-    if (hi) t = fmin_(t, hi[k]);
-    b.tips[i * 3 + k] = t;
-    // capsule bases follow their tips (the instrument is translated rigidly)
-    if (b.caps_base) b.caps_base[i * 3 + k] += t - t0;
+    const float moved = b.tips[env * 12 + 9 + k] + delta[env * 3 + k];
+    d[k] = ((lo && moved < lo[k]) || (hi && moved > hi[k])) ? 0.0f : delta[env * 3 + k];
+  }
+  for (int p = 0; p < 4; ++p) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int i = env * 12 + p * 3 + k;
+      const float t0 = b.tips[i], t = t0 + d[k];
+      b.tips[i] = t;
+      // capsule bases follow their tips (the instrument is translated rigidly)
+      if (b.caps_base) b.caps_base[i] += t - t0;
+    }
   }
 }
 
@@ -1497,7 +1506,7 @@ int ffsrl_zero_lambdas(const ffsrl_batch* b, int32_t eb, int32_t ee, void* strea
 int ffsrl_move_tools(const ffsrl_batch* b, const float* delta, const float* lo, const float* hi, void* stream) {
   if (!b || !b->tips || !delta) return fail(FFSRL_E_ARG, "tips and delta must be non-NULL");
   if (b->n_env == 0) return 0;
-  hipLaunchKernelGGL(k_move_tools, dim3(nblk(b->n_env * 4, 256)), dim3(256), 0, as_stream(stream), *b, delta, lo, hi);
+  hipLaunchKernelGGL(k_move_tools, dim3(nblk(b->n_env, 256)), dim3(256), 0, as_stream(stream), *b, delta, lo, hi);
   FFSRL_HIP(hipGetLastError());
   return 0;
 }

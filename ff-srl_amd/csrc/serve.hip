@@ -7,9 +7,11 @@
 // Reset follows resetFixedModel (modelDO.py:1589-1622): positions restored,
 // velocities and grasp weights zeroed, instrument points restored; the env's
 // BVH is refit in the same workgroup so the reset observation renders the
-// restored mesh.  A step moves the instrument points by the request's delta
-// (clamped to the workspace, as applyCartesianActionsInWorkspace clamps,
-// modelDO.py:1385-1404) and, when the request opens or closes the clamp,
+// restored mesh.  A step moves the instrument points by the request's delta,
+// less the components that would take the drag point (point 3) out of the
+// workspace box, as transfromCartesianActionsKernelInWorkspace drops them
+// (modelDO.py:97-154); the whole instrument moves by the same delta.  When
+// the request opens or closes the clamp,
 // reselects the grasped vertices like updateClampsDragKernel
 // (modelDO.py:247-279): drag = 1 for vertices within `radius` of the drag
 // point tip[env*4+3] when closed, 0 when open.
@@ -44,12 +46,17 @@ __global__ __launch_bounds__(kServeBlock) void k_apply_requests(ffsrl_batch b, f
   }
   // step: move the four instrument points, then (optionally) reselect the grasp
   const float* a = r.action + 4 * (int64_t)i;
+  __shared__ float delta[3];
+  if (tid < 3) {
+    // a component that would take the drag point out of the workspace is dropped
+    const float d = a[tid], moved = b.tips[env * 12 + 9 + tid] + d;
+    delta[tid] = ((r.lo && moved < r.lo[tid]) || (r.hi && moved > r.hi[tid])) ? 0.0f : d;
+  }
+  __syncthreads();
   if (tid < 12) {
     const int k = tid % 3;
     const float t0 = b.tips[env * 12 + tid];
-    float t = t0 + a[k];
-    if (r.lo) t = fmax_(t, r.lo[k]);
-    if (r.hi) t = fmin_(t, r.hi[k]);
+    const float t = t0 + delta[k];
     b.tips[env * 12 + tid] = t;
     // capsule bases follow their tips (the instrument is translated rigidly)
     if (b.caps_base) b.caps_base[env * 12 + tid] += t - t0;

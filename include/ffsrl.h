@@ -290,10 +290,12 @@ int ffsrl_vertex_normals(const ffsrl_batch* b, int32_t env_begin, int32_t env_en
                          void* stream);
 
 /* Per-env tool update: adds delta[env] (3 floats) to the 4 instrument points
- * of each env and clamps them to [lo, hi]; capsule bases, when present, move
- * by the same clamped displacement as their tips.  Stands in for
- * applyCartesianActionsInWorkspace (modelDO.py:1385-1404) with the tool tip
- * driven directly (no remote-centre-of-motion kinematics). */
+ * of each env, less the components that would take the drag point (point 3)
+ * out of [lo, hi] (lo, hi device float[3] each, or NULL), which are dropped
+ * as transfromCartesianActionsKernelInWorkspace drops them
+ * (modelDO.py:97-154); capsule bases, when present, move with their tips.
+ * Stands in for applyCartesianActionsInWorkspace (modelDO.py:1385-1404) with
+ * the tool translated directly (no remote-centre-of-motion kinematics). */
 int ffsrl_move_tools(const ffsrl_batch* b, const float* delta, const float* lo, const float* hi,
                      void* stream);
 
@@ -359,8 +361,9 @@ int ffsrl_instrument_update(const ffsrl_batch* b, const ffsrl_instrument* ins, v
 /* Applies one tick of requests: a reset restores the env like resetFixedModel
  * (modelDO.py:1589-1622: positions from x0, velocities and grasp weights 0,
  * instrument points from tips0) and refits its BVH; a step adds the action's
- * delta to the env's 4 instrument points (clamped to [lo, hi] when given, as
- * applyCartesianActionsInWorkspace clamps, modelDO.py:1385-1404) and, when
+ * delta to the env's 4 instrument points, less the components that would
+ * take the drag point out of [lo, hi] when given (dropped, as
+ * transfromCartesianActionsKernelInWorkspace does, modelDO.py:97-154), and, when
  * the clamp command is 0 or 1, reselects the grasp like updateClampsDragKernel
  * (modelDO.py:247-279): drag = |x - tip[env*4+3]| < radius when closed, 0 when
  * open. */

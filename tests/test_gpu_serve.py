@@ -226,3 +226,32 @@ def test_poisson_driver(pkg, gpu):
     assert out["completed_steps"] + out["completed_resets"] == len(reqs)
     assert 0.0 < out["p50_step_ms"] <= out["p99_step_ms"] < 1000.0
     assert all(c.latency > 0 for c in srv.completions)
+
+
+def test_workspace_drops_the_offending_component(pkg, gpu):
+    """A step whose move would take the drag point (point 3) out of the
+    workspace box loses that component only, and the whole instrument moves
+    by the rest (transfromCartesianActionsKernelInWorkspace,
+    modelDO.py:97-154: the component is dropped, not clamped).  The same
+    rule holds for ffsrl_move_tools."""
+    S = pkg.serve
+    meshes, tips = _classes(pkg)
+    p3 = tips[0, 3].astype(np.float64)
+    lo = tuple(p3 - [1.0, 0.01, 1.0])
+    hi = tuple(p3 + [1.0, 0.01, 1.0])
+    srv = _server(pkg, gpu, workspace=(lo, hi))
+    srv.register(0, 0)
+    move = (0.1, 0.05, -0.2)                           # y would leave the box: dropped
+    reqs = [S.Request(0, S.REQ_RESET, seq=0), S.Request(0, S.REQ_STEP, (*move, S.CLAMP_KEEP), seq=1)]
+    _run(srv, {0: reqs}, {0: 0})
+    slot = srv.actor_slot[0]
+    got = srv.model.laparoscopeTip[slot].cpu().numpy()
+    want = tips[0] + np.float32([0.1, 0.0, -0.2])
+    assert np.array_equal(got, want.astype(np.float32))
+    # the direct tool move of a model follows the same rule
+    m = pkg.SimModelDO([meshes[0]], [0], device=gpu, tips=tips[:1], **PHYS)
+    w = torch.tensor([lo, hi], dtype=torch.float32, device=gpu)
+    d = torch.tensor([move], dtype=torch.float32, device=gpu)
+    pkg.lib.ffsrl_move_tools(m.batch, d.data_ptr(), w[0].data_ptr(), w[1].data_ptr(), m.stream())
+    torch.cuda.synchronize()
+    assert np.array_equal(m.laparoscopeTip[0].cpu().numpy(), want.astype(np.float32))
