@@ -50,16 +50,18 @@ constexpr int lds_bytes_for(int nv) { return nv * 16; }
 constexpr int kMaxFrameVertices = (kLdsBytes - 512) / 16;
 
 // Jacobi sums in LDS are fixed point: each delta component is rounded to the
-// nearest multiple of the env's quantum q = 2^dp_exp (about 2^-27 of the mesh
-// diagonal D) and added with an integer LDS atomic.  Integer atomics run at
+// nearest multiple of the env's quantum q = 2^dp_exp (2^-26 of the mesh
+// diagonal D rounded down to a power of two) and added with an integer LDS
+// atomic.  Integer atomics run at
 // the LDS's full rate on gfx950 (float LDS atomics are ~30x slower), and
 // integer sums do not depend on the order of the additions, so the frame is
-// deterministic.  Each term is clamped to +-(2^24 - 1) q (about D/16 .. D/8),
-// so the sums of up to FFSRL_MAX_TERMS = 128 terms (a vertex's incident
-// constraints in one pass, checked at launch) stay inside int32; the
-// rounding error per term is <= q/2.  A term beyond the range, or not
-// finite, is clamped and flags the env (FFSRL_STATUS_SATURATED): the frame
-// then no longer equals the plain form, which sums floats.
+// deterministic.  Each term is clamped to +-(2^24 - 1) q (D/8 .. D/4), so the
+// sums of up to FFSRL_MAX_TERMS = 128 terms (a vertex's incident constraints
+// in one pass, checked at launch) stay inside int32; the rounding error per
+// term is <= q/2, 1/16 of the float32 spacing of a coordinate of size D.  A
+// term beyond the range, or not finite, is clamped and flags the env
+// (FFSRL_STATUS_SATURATED): the frame then no longer equals the plain form,
+// which sums floats.
 //
 // A vertex owns 16 B of LDS, two 64-bit words updated with ds_add_u64:
 //   w0 = X + Y * 2^32,  w1 = Z + count * 2^32   (two's complement, mod 2^64)
@@ -166,14 +168,16 @@ template <bool COH>
 __device__ __forceinline__ TetG gather_tet(const ffsrl_batch& b, const float* P, int64_t vb, const TetIn& t,
                                            int64_t ti, int64_t T, int nh) {
   TetG g;
-  if (nh) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) g.M[k] = b.tet_inv_rest[k * T + ti];
-  }
+  // the position gathers first: they need only the ids, and the inverse
+  // rest matrix loads then overlap them
   g.P[0] = ldp<COH>(P, vb + t.q.x);
   g.P[1] = ldp<COH>(P, vb + t.q.y);
   g.P[2] = ldp<COH>(P, vb + t.q.z);
   g.P[3] = ldp<COH>(P, vb + t.q.w);
+  if (nh) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g.M[k] = b.tet_inv_rest[k * T + ti];
+  }
   return g;
 }
 
@@ -183,7 +187,7 @@ __device__ __forceinline__ TetG gather_tet(const ffsrl_batch& b, const float* P,
 template <bool CARRY, bool COH>
 __device__ __forceinline__ void process_tet(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
                                             const FrameCtx& c, const NHConst& nhk, const TetIn& tc, const TetG& g,
-                                            int j, bool fresh) {
+                                            int j, bool fresh, bool nh) {
   f3 d[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
   uint32_t n = 0;
   if (prm.vol_mode == FFSRL_PBD) {
@@ -194,7 +198,7 @@ __device__ __forceinline__ void process_tet(const ffsrl_batch& b, const ffsrl_pa
     n += volume_xpbd<true>(g.P, tc.rest, prm.vol_k, &lam, d);
     if (CARRY) b.lam_t[e.lam_tbase + j] = lam;
   }
-  if (prm.nh_on) {
+  if (nh) {
     f3 dn[4];
     if (neohookean<true>(g.P, tc.rest, g.M, nhk, dn)) {
       n += 1;
@@ -240,6 +244,49 @@ struct Slot {
   float4 A, B;
 };
 
+// The 1024-thread projection loop for a fixed constraint set (HT: tets,
+// HE: edges, NH: Neo-Hookean): indices are clamped into range instead of
+// guarded, so every load of the loop body is unconditional and no branch
+// lies between a load and its use (a branch that issues loads on one path
+// only makes the compiler wait for every outstanding load at the join).
+// The ids are loaded one stride ahead; an item's edge gathers, tet gathers
+// and inverse rest matrix are issued together, then the edge and the tet
+// are computed (the edge's registers are free again before the tet's peak).
+template <bool HT, bool HE, bool NH, bool CARRY, bool COH>
+__device__ __forceinline__ void project_wide(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
+                                             const FrameCtx& c, const NHConst& nhk, const float* P, int64_t vb,
+                                             int64_t T, int nt, int ne, int jb, int je, bool fresh) {
+  constexpr int BLOCK = 1024;
+  const int nmax = je;
+  const int j0 = jb + threadIdx.x;
+  auto ct = [&](int jj) { return jj < nt ? jj : nt - 1; };
+  auto ce = [&](int jj) { return jj < ne ? jj : ne - 1; };
+  auto ids = [&](Slot& s, int jj) {
+    if (HT) s.t = load_tet(b, e.tbase + ct(jj));
+    if (HE) s.e = load_edge(b, e.ebase + ce(jj));
+  };
+  auto gr = [&](const Slot& s, int jj) {
+    float4 A = make_float4(0.0f, 0.0f, 0.0f, 0.0f), B = A;
+    TetG g;
+    if (HE) {
+      A = ldp<COH>(P, vb + s.e.q.x);
+      B = ldp<COH>(P, vb + s.e.q.y);
+    }
+    if (HT) g = gather_tet<COH>(b, P, vb, s.t, e.tbase + ct(jj), T, NH);
+    if (HE && jj < ne) process_edge<CARRY, COH>(b, prm, e, c, s.e, A, B, jj, fresh);
+    if (HT && jj < nt) process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, g, jj, fresh, NH);
+  };
+  Slot s0, s1;
+  ids(s0, j0);
+  for (int j = j0; j < nmax; j += 2 * BLOCK) {
+    ids(s1, j + BLOCK);
+    gr(s0, j);
+    if (j + BLOCK >= nmax) break;
+    ids(s0, j + 2 * BLOCK);
+    gr(s1, j + BLOCK);
+  }
+}
+
 // One projection pass over the env's tets and edges.  Thread j handles tet j
 // and edge j of each stride of BLOCK.  The pass is software-pipelined over
 // rotating slots (the loop is unrolled over the slots, so "next" and
@@ -270,7 +317,7 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
     }
   };
   auto run = [&](const Slot& s, int jj) {
-    if (jj < nt) process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, s.g, jj, fresh);
+    if (jj < nt) process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, s.g, jj, fresh, prm.nh_on != 0);
     if (jj < ne) process_edge<CARRY, COH>(b, prm, e, c, s.e, s.A, s.B, jj, fresh);
   };
   if constexpr (BLOCK <= 512) {
@@ -292,25 +339,20 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
       run(s2, j + 2 * BLOCK);
     }
   } else {
-    // Gather and compute one item at a time to stay within 128 VGPRs.
-    auto gr = [&](const Slot& s, int jj) {
-      if (jj < nt) {
-        TetG g = gather_tet<COH>(b, P, vb, s.t, e.tbase + jj, T, prm.nh_on);
-        process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, g, jj, fresh);
-      }
-      if (jj < ne) {
-        float4 A = ldp<COH>(P, vb + s.e.q.x), B = ldp<COH>(P, vb + s.e.q.y);
-        process_edge<CARRY, COH>(b, prm, e, c, s.e, A, B, jj, fresh);
-      }
-    };
-    Slot s0, s1;
-    ids(s0, j0);
-    for (int j = j0; j < nmax; j += 2 * BLOCK) {
-      ids(s1, j + BLOCK);
-      gr(s0, j);
-      if (j + BLOCK >= nmax) break;
-      ids(s0, j + 2 * BLOCK);
-      gr(s1, j + BLOCK);
+    // 128 VGPRs per lane: the straight-line loop of the constraint set
+    const bool nh = prm.nh_on != 0;
+    if (nt > 0 && ne > 0) {
+      if (nh)
+        project_wide<true, true, true, CARRY, COH>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, fresh);
+      else
+        project_wide<true, true, false, CARRY, COH>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, fresh);
+    } else if (nt > 0) {
+      if (nh)
+        project_wide<true, false, true, CARRY, COH>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, fresh);
+      else
+        project_wide<true, false, false, CARRY, COH>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, fresh);
+    } else if (ne > 0) {
+      project_wide<false, true, false, CARRY, COH>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, fresh);
     }
   }
 }

@@ -40,6 +40,7 @@ __global__ __launch_bounds__(kServeBlock) void k_apply_requests(ffsrl_batch b, f
     }
     if (tid < 12) b.tips[env * 12 + tid] = r.tips0[env * 12 + tid];
     if (tid < 12 && r.base0 && b.caps_base) b.caps_base[env * 12 + tid] = r.base0[env * 12 + tid];
+    if (tid == 0 && b.status) b.status[env] = 0;  // a new episode
     __syncthreads();
     if (b.bvh_lo && b.level_nodes && b.level_info) refit_env<kServeBlock>(b, e);
     return;

@@ -510,10 +510,7 @@ class SimModelDO:
             row[F["lvlbase"]], row[F["nlvl"]] = self.mesh_off["l"][k], t.num_groups
             row[F["bvh_depth"]] = t.max_level + 1
             row[F["lam_ebase"]], row[F["lam_tbase"]] = (le, lt) if carry else (0, 0)
-            # fixed-point quantum of the batched frame's Jacobi sums: 2^-27 of the
-            # mesh diagonal rounded down to a power of two (range 8..16 diagonals)
-            diag = max(m.bbox_diagonal(), 1e-6)
-            row[F["dp_exp"]] = int(math.floor(math.log2(diag))) - 27
+            row[F["dp_exp"]] = self._dp_exp(k)
             row[F["vtopo_base"]] = self.mesh_off["vt"][k]
             row[F["max_terms"]] = self._max_terms(k)
             vb += m.num_vertices
@@ -533,6 +530,15 @@ class SimModelDO:
         self.max_nv = int(max(m.num_vertices for m in self.meshes)) if self.meshes else 0
         work = np.array([self.meshes[k].num_tets * 4 + self.meshes[k].num_edges for k in self.env_mesh])
         self.work = work
+
+    def _dp_exp(self, k: int) -> int:
+        """Exponent of the fixed-point quantum q of mesh k's Jacobi sums in
+        the batched frames: q = 2^(floor(log2 D) - 26), D the mesh diagonal.
+        A term is clamped to 2^24 - 1 quanta (D/8 .. D/4); its rounding error
+        q/2 is 1/16 of the float32 spacing of a coordinate of size D.  The
+        sums of a vertex's at most 128 terms stay inside int32 for any q."""
+        diag = max(self.meshes[k].bbox_diagonal(), 1e-6)
+        return int(math.floor(math.log2(diag))) - 26
 
     def _max_terms(self, k: int) -> int:
         """Most Jacobi terms one vertex of mesh k receives in one pass of the

@@ -63,7 +63,8 @@ typedef struct ffsrl_env_desc {
   int32_t lam_ebase;       /* first edge lambda of this env (XPBD, iterations > 1) */
   int32_t lam_tbase;       /* first tet lambda of this env */
   int32_t dp_exp;          /* quantum of the fixed-point Jacobi sums of the batched
-                              frame: 2^dp_exp (about 2^-27 x bbox diagonal) */
+                              frame: 2^dp_exp (the host mirror uses 2^-26 x bbox diagonal,
+                              rounded down to a power of two: a term's range is D/8 .. D/4) */
   int32_t vtopo_base;      /* first vf_off entry of the env's mesh */
   int32_t bvh_depth;       /* levels of the env's BVH (max node level + 1) */
   int32_t max_terms;       /* most Jacobi terms of one vertex in one pass: incident tets plus
@@ -360,7 +361,8 @@ int ffsrl_instrument_update(const ffsrl_batch* b, const ffsrl_instrument* ins, v
 
 /* Applies one tick of requests: a reset restores the env like resetFixedModel
  * (modelDO.py:1589-1622: positions from x0, velocities and grasp weights 0,
- * instrument points from tips0) and refits its BVH; a step adds the action's
+ * instrument points from tips0), refits its BVH and clears the env's status
+ * word (when the batch has one: a reset starts a new episode); a step adds the action's
  * delta to the env's 4 instrument points, less the components that would
  * take the drag point out of [lo, hi] when given (dropped, as
  * transfromCartesianActionsKernelInWorkspace does, modelDO.py:97-154), and, when

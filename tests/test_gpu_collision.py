@@ -169,17 +169,17 @@ def test_reset_restores_capsule_bases(pkg, gpu):
     assert np.array_equal(model.laparoscopeBase[1].cpu().numpy(), moved[1])
 
 
-@pytest.mark.parametrize("radius,ks,saturates", [(0.12, 1.0, False), (0.6, 4.0, True)])
+@pytest.mark.parametrize("radius,ks,saturates", [(0.12, 1.0, False), (0.6, 4.0, None), (0.6, 16.0, True)])
 def test_deep_contact_is_exact_or_flagged(pkg, gpu, radius, ks, saturates):
-    """The batched frame's fixed-point terms hold +-(2^24 - 1) quanta, about
-    D/16 .. D/8 of the mesh diagonal D (0.25 on these 2.5-diagonal meshes),
-    while a collision response reaches up to 3 r ks (r the capsule radius):
-    plen <= r and the barycentric weights' squared norm is >= 1/3.  A
-    moderate press (r 0.12, ks 1) stays in range: the batched frame equals
-    the plain form within 1e-5 x diag and reports nothing.  A deep press
-    with a large stiff capsule (r 0.6, ks 4) produces terms beyond the range:
-    the batched frame clamps them and reports STATUS_SATURATED for the env,
-    where it no longer equals the plain form (which sums floats)."""
+    """The batched frame's fixed-point terms hold +-(2^24 - 1) quanta, D/8 ..
+    D/4 of the mesh diagonal D (0.5 on these 2.5-diagonal meshes), while a
+    collision response reaches up to 3 r ks (r the capsule radius): plen <= r
+    and the barycentric weights' squared norm is >= 1/3.  A moderate press (r 0.12, ks 1) stays in range: the batched
+    frame equals the plain form within 1e-5 x diag and reports nothing.  A
+    deep press with a large stiff capsule (r 0.6, ks 16) produces terms far
+    beyond the range: the batched frame clamps them and reports
+    STATUS_SATURATED for the env, where it no longer equals the plain form
+    (which sums floats).  In between (r 0.6, ks 4) either holds."""
     meshes, bases, tips, radii = _scene(pkg, 2, 5000, 31, depth=0.5, RADIUS=radius)
     kw = dict(C0, collision=True, globalKsContact=ks, tips=tips, capsuleBases=bases, capsuleRadii=radii)
     models = {f: pkg.SimModelDO(meshes, [0, 1], device=gpu, **kw) for f in ("batched", "plain")}
@@ -192,12 +192,12 @@ def test_deep_contact_is_exact_or_flagged(pkg, gpu, radius, ks, saturates):
     for e, m in enumerate(meshes):
         s = models["batched"].env_slice(e)
         err = float(np.abs(xb[s] - xp[s]).max()) / m.bbox_diagonal()
-        moved = float(np.abs(xp[s] - m.verts).max())
+        moved = float(np.nanmax(np.abs(xb[s] - m.verts)))                # the plain form may not stay finite
         print(f"\n[saturation r={radius} ks={ks}] env {e}: status {int(st[e])}, |batched-plain| {err:.2e} diag, "
               f"largest move {moved:.3f}")
         assert moved > 1e-3                                             # the capsule pressed the body
         if saturates:
             assert st[e] & pkg._lib.STATUS_SATURATED
-        else:
+        elif saturates is not None:
             assert st[e] == 0 and err <= 1e-5
         assert st[e] != 0 or err <= 1e-5                               # exact, or reported

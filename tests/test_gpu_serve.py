@@ -174,6 +174,23 @@ def test_reset_restores_the_initial_observation(pkg, gpu):
     assert float(first[2][30]) == 0.0 and float(first[2][25]) == 0.0          # at rest, nothing grasped
 
 
+def test_reset_clears_the_scene_status(pkg, gpu):
+    """A reset starts a new episode: it clears the slot's status word (a
+    saturated or incomplete frame of the previous episode) and no other."""
+    S = pkg.serve
+    srv = _server(pkg, gpu)
+    _run(srv, {a: [S.Request(a, S.REQ_RESET, seq=0)] for a in (0, 1)}, {0: 0, 1: 0})
+    s0, s1 = srv.actor_slot[0], srv.actor_slot[1]
+    srv.model.status[s0] = pkg._lib.STATUS_SATURATED
+    srv.model.status[s1] = pkg._lib.STATUS_SATURATED
+    torch.cuda.synchronize()
+    srv.submit(S.Request(0, S.REQ_RESET, seq=1))
+    srv.run_until_idle()
+    torch.cuda.synchronize()
+    st = srv.model.frame_status().cpu().numpy()
+    assert st[s0] == 0 and st[s1] == pkg._lib.STATUS_SATURATED
+
+
 def test_admission_waits_for_memory(pkg, gpu):
     S = pkg.serve
     srv = _server(pkg, gpu, capacity=(2, 4), max_batch=8)
