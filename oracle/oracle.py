@@ -41,6 +41,11 @@ Parity status of each restated function (see the tests that pin them):
                                       state table (tests/test_instrument_oracle.py);
                                       Warp's exact operation order for mat44
                                       inverse / quaternions is parity unpinned
+  remap (remapAToBKernel)           - pinned: the reference's liver scene maps
+                                      each render point to the simulation
+                                      vertex at its position (to 1.5e-5 in a
+                                      12-unit scene), the swapped map does not
+                                      (tests/test_remap_oracle.py)
 """
 from __future__ import annotations
 
@@ -51,7 +56,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# FFSRL_ORACLE_LIB selects another build of the same C (the sanitizer build)
+LIB_PATH = os.environ.get("FFSRL_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 
 def build():
@@ -424,6 +430,18 @@ def bvh_build(vertex, faces, bins=8, mutant=None):
     return {"left": left[:n].astype(np.int32), "first": first[:n].astype(np.int32),
             "count": count[:n].astype(np.int32), "level": level[:n].astype(np.int32),
             "tri_map": tri.astype(np.int32), "max_level": int(level[:n].max())}
+
+
+def remap(dst, src, pairs):
+    """remapAToBKernel (``FF_SRL/FF_SRL/utils.py:32-42``): for each pair
+    (point, vertex), in thread order, ``dst[point] = src[vertex]``; a point
+    listed twice keeps its last pair's value (the scene's maps list each
+    point once).  Returns a new array; dst is not modified."""
+    out = np.array(dst, copy=True)
+    pairs = np.asarray(pairs).reshape(-1, 2)
+    for pt, vx in pairs:
+        out[pt] = src[vx]
+    return out
 
 
 def collide(pred, tri, tri_env, inv_mass, caps_base, caps_tip, caps_radius, k=1.0, dtype=np.float64):
