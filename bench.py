@@ -15,6 +15,7 @@ import ctypes as C
 import importlib
 import json
 import os
+import re
 import sys
 import time
 
@@ -24,10 +25,46 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_frame launch measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (x2.0 and x1.0 calibration factors from a 256 MiB copy), by
-# workload; profiles/k_frame_k_render_mixed1024.md holds the runs.
-MEASURED_TRAFFIC = {"mixed-1024env-logU[1000,40000]tets": 9.91e9}
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic():
+    """HBM bytes per frame-kernel launch measured with rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE (calibrated on a device copy), by workload:
+    each summary under profiles/ states its figures on lines
+    `traffic-per-launch: <workload> <kernel> <bytes> bytes (measured <date>)`.
+    Returns {(workload, kernel): (bytes, source)}."""
+    out = {}
+    pat = re.compile(r"^traffic-per-launch:\s+(\S+)\s+(\S+)\s+([0-9.eE+]+) bytes \(measured ([^)]*)\)")
+    for name in sorted(os.listdir(PROFILES)) if os.path.isdir(PROFILES) else []:
+        if not name.endswith(".md"):
+            continue
+        for line in open(os.path.join(PROFILES, name)):
+            m = pat.match(line.strip().strip("`"))
+            if m:
+                out[(m.group(1), m.group(2))] = (float(m.group(3)), f"profiles/{name}, measured {m.group(4)}")
+    return out
+
+
+def copy_peak_gbs(dev, n_bytes=1 << 30, reps=10):
+    """Device-to-device copy bandwidth on this GPU (read + write bytes / time,
+    best of `reps`), the practical ceiling beside the nominal 8 TB/s."""
+    import torch
+    a = torch.ones(n_bytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * n_bytes / (best * 1e-3) / 1e9
 
 
 def log(msg):
@@ -58,8 +95,9 @@ def parse():
                     help="host threads of the multi-core CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
-                    help="length of the serving run after the batched measurement (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the serving run after the batched measurement (0: skip); the serving "
+                         "configuration is 60 s")
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-actors", type=int, default=4096)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -387,10 +425,17 @@ def main():
     }
     alg = algorithmic_bytes(w, model)
     achieved = alg / (frame_ms.mean() * 1e-3) / 1e9
+    kname = "k_frame_shared" if integ.shared else "k_frame"
+    traffic, source = measured_traffic().get((w.name, kname), (None, None))
+    copy_peak = copy_peak_gbs(dev)
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(w.name),
-                       "kernel": ("k_frame_shared" if integ.shared else "k_frame") + " (physics + refit)",
-                       "algorithmic_bytes_per_launch": alg}
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                       "kernel": kname + " (physics + refit)", "algorithmic_bytes_per_launch": alg,
+                       "copy_peak": copy_peak, "frac_of_copy_peak": achieved / copy_peak}
+    if traffic is not None:
+        # measured HBM bytes over this run's frame time, against the nominal peak
+        out["roofline"]["traffic_source"] = source
+        out["roofline"]["measured_hbm_frac"] = traffic / (frame_ms.mean() * 1e-3) / 1e9 / HBM_PEAK_GBS
 
     if not args.no_graph:
         log("graph replay")
