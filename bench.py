@@ -46,6 +46,16 @@ def measured_traffic():
     return out
 
 
+def one_launch_count(model, n):
+    """Kernel launches of one ffsrl_frame_do frame (with refit) over n envs."""
+    per_it = ((model.vol_mode != 0) + (model.dist_mode != 0) + bool(model.nh_on) + 2 +
+              (2 if model.collision else 0))
+    reset = model.carries_lambda and (not model.lambda_legacy or model.simConstraints > 1)
+    lam = (model.lam_e is not None) + (model.lam_t is not None) if reset else 0
+    groups = max(int(t.num_groups) for t in model.topo)
+    return model.simSubsteps * (2 + lam + model.simConstraints * per_it) + groups
+
+
 def copy_peak_gbs(dev, n_bytes=1 << 30, reps=10):
     """Device-to-device copy bandwidth on this GPU (read + write bytes / time,
     best of `reps`), the practical ceiling beside the nominal 8 TB/s."""
@@ -74,8 +84,8 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="mixed", choices=["mixed", "ellipsoids", "cube", "large", "serve", "liver"])
     ap.add_argument("--scene", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
                                                      "liver_retraction.npz"),
@@ -85,6 +95,8 @@ def parse():
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--shared", choices=["auto", "on", "off"], default="auto",
                     help="shared-env frame: autotuned (auto), forced on, or off")
+    ap.add_argument("--fold", choices=["on", "off"], default="on",
+                    help="the shared frame's folded form for envs of at most 5,104 vertices (same results)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --block and largest-first dispatch instead of the autotuned launch shape")
     ap.add_argument("--plain-frames", type=int, default=2)
@@ -95,6 +107,8 @@ def parse():
                     help="host threads of the multi-core CPU baseline")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--gen-workers", type=int, default=0,
+                    help="processes generating the meshes (1: in this process, as under a profiler; 0: up to 16)")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the serving run after the batched measurement (0: skip); the serving "
                          "configuration is 60 s")
@@ -266,6 +280,8 @@ def serving_run(pkg, wl, args, dev):
                         resolution=sw.resolution, camera_pos=sw.camera_pos, look_at=sw.look_at,
                         block=args.block, streams=args.serve_streams, max_inflight=args.serve_inflight,
                         shared=args.shared)
+    if getattr(srv.model, "split_acc", None) is not None:
+        srv.model.enable_shared(fold=args.fold == "on")
     for a in range(sw.actors):
         srv.register(a, int(cls[a]))
     log(f"serving {sw.name}: {len(reqs)} requests over {args.serve_seconds:.0f} s, "
@@ -290,6 +306,8 @@ def main():
         print(json.dumps({"error": "this benchmark runs on one GPU; several GPUs would each step their own "
                                    "share of the environments, which is not launched here", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.gen_workers:
+        os.environ["FFSRL_GEN_WORKERS"] = str(args.gen_workers)
     wl = importlib.import_module("ff-srl_amd.workloads")
     if args.workload == "serve":
         import torch
@@ -356,6 +374,8 @@ def main():
         log(f"autotuned {tuned} in {time.perf_counter() - t_tune:.1f} s")
     if args.shared != "auto":
         integ.shared = args.shared == "on"
+    if integ.shared:
+        model.enable_shared(fold=args.fold == "on")
     if args.calibrate_copy:
         a = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -413,6 +433,7 @@ def main():
                    "render": f"{w.resolution}x{w.resolution} depth{' + RGB (texture mode)' if w.rgb else ''} "
                              f"(2x2 supersampled)", "block": integ.block,
                    "dispatch": "lpt" if integ.dispatch_sorted else "table", "shared": integ.shared,
+                   **({"fold": getattr(model, "fold_ws", None) is not None} if integ.shared else {}),
                    "chunk": model.topo[0].chunk, "autotuned": tuned is not None,
                    "form": "batched (1 frame kernel + 1 render kernel per step)",
                    **({"collision": True} if w.model_kwargs.get("collision") else {})},
@@ -480,6 +501,31 @@ def main():
         out["plain_env_steps_per_s"] = n * args.plain_frames / plain_el
         out["plain_p50_step_ms"] = float(np.percentile(pms, 50))
         out["batched_over_plain"] = value / out["plain_env_steps_per_s"]
+
+        # the reference's launch model: one launch per kernel over all envs
+        # (ffsrl_frame_do), then one render launch over all envs
+        log("reference launch model")
+        restore()
+
+        def do_step(f):
+            move(f % total_frames, s)
+            lib.ffsrl_frame_do(model.batch, model.params, 0, n, 1, s)
+            render(s)
+
+        do_step(0)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        dms = []
+        for f in range(args.plain_frames * 5):
+            a = time.perf_counter()
+            do_step(f)
+            torch.cuda.synchronize()
+            dms.append((time.perf_counter() - a) * 1e3)
+        do_el = time.perf_counter() - tp
+        out["one_launch_env_steps_per_s"] = n * len(dms) / do_el
+        out["one_launch_p50_step_ms"] = float(np.percentile(dms, 50))
+        out["one_launch_launches_per_frame"] = one_launch_count(model, n)
+        out["batched_over_one_launch"] = value / out["one_launch_env_steps_per_s"]
 
     if args.serve_seconds > 0:
         log("serving")

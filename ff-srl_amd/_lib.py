@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("FFSRL_LIB") or os.path.join(HERE, "_build", "libffsrl
 
 OFF, PBD, XPBD = 0, 1, 2
 E_ARG, E_LIMIT, E_STATE = -1, -2, -3
-ABI_VERSION = 10
+ABI_VERSION = 11
 STATUS_SATURATED, STATUS_INCOMPLETE = 1, 2   # FFSRL_STATUS_* bits of ffsrl_batch.status
 MAX_TERMS = 128                              # FFSRL_MAX_TERMS
 
@@ -45,7 +45,8 @@ class Batch(C.Structure):
         ("x", _vp), ("p", _vp), ("v", _vp), ("drag", _vp), ("lam_e", _vp), ("lam_t", _vp),
         ("bvh_lo", _vp), ("bvh_hi", _vp), ("tips", _vp), ("caps_base", _vp), ("caps_radius", _vp),
         ("in_collision", _vp), ("sched", _vp),
-        ("dp", _vp), ("cnt", _vp), ("split_acc", _vp), ("split_ctr", _vp), ("status", _vp), ("host_envs", _vp),
+        ("dp", _vp), ("cnt", _vp), ("split_acc", _vp), ("split_ctr", _vp), ("status", _vp), ("fold_ws", _vp),
+        ("fold_stride", C.c_int64), ("host_envs", _vp),
         ("host_level_off", _vp),
     ]
 
@@ -123,6 +124,7 @@ SIGNATURES = {
     "ffsrl_bounds_integrate": [_P(Batch), _P(Params), _i32, _i32, _vp],
     "ffsrl_zero_lambdas": [_P(Batch), _i32, _i32, _vp],
     "ffsrl_bvh_refit_plain": [_P(Batch), _i32, _i32, _vp],
+    "ffsrl_frame_do": [_P(Batch), _P(Params), _i32, _i32, _i32, _vp],
     "ffsrl_remap": [_vp, _vp, _vp, _i64, _vp],
     "ffsrl_bvh_build": [_vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _P(_i32), _P(_i32)],
     "ffsrl_color_constraints": [_vp, _i64, _i32, _i32, _vp, _P(_i32)],

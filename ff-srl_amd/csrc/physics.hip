@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "constraints.hpp"
@@ -164,16 +165,30 @@ struct TetG {
   float M[9];
 };
 
+// Where the projection reads positions, by mesh-local vertex id: the env's
+// rows of a global array (plain, or coherent for state other workgroups
+// write), or the env's copy in LDS.
 template <bool COH>
-__device__ __forceinline__ TetG gather_tet(const ffsrl_batch& b, const float* P, int64_t vb, const TetIn& t,
-                                           int64_t ti, int64_t T, int nh) {
+struct PosGlobal {
+  const float* P;
+  int64_t vb;
+  __device__ __forceinline__ float4 operator()(int q) const { return ldp<COH>(P, vb + q); }
+};
+struct PosLds {
+  const float4* L;
+  __device__ __forceinline__ float4 operator()(int q) const { return L[q]; }
+};
+
+template <class PS>
+__device__ __forceinline__ TetG gather_tet(const ffsrl_batch& b, const PS& ps, const TetIn& t, int64_t ti, int64_t T,
+                                           int nh) {
   TetG g;
   // the position gathers first: they need only the ids, and the inverse
   // rest matrix loads then overlap them
-  g.P[0] = ldp<COH>(P, vb + t.q.x);
-  g.P[1] = ldp<COH>(P, vb + t.q.y);
-  g.P[2] = ldp<COH>(P, vb + t.q.z);
-  g.P[3] = ldp<COH>(P, vb + t.q.w);
+  g.P[0] = ps(t.q.x);
+  g.P[1] = ps(t.q.y);
+  g.P[2] = ps(t.q.z);
+  g.P[3] = ps(t.q.w);
   if (nh) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) g.M[k] = b.tet_inv_rest[k * T + ti];
@@ -252,10 +267,10 @@ struct Slot {
 // The ids are loaded one stride ahead; an item's edge gathers, tet gathers
 // and inverse rest matrix are issued together, then the edge and the tet
 // are computed (the edge's registers are free again before the tet's peak).
-template <bool HT, bool HE, bool NH, bool CARRY, bool COH>
+template <bool HT, bool HE, bool NH, bool CARRY, bool COH, class PS>
 __device__ __forceinline__ void project_wide(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
-                                             const FrameCtx& c, const NHConst& nhk, const float* P, int64_t vb,
-                                             int64_t T, int nt, int ne, int jb, int je, bool fresh) {
+                                             const FrameCtx& c, const NHConst& nhk, const PS& ps, int64_t T, int nt,
+                                             int ne, int jb, int je, bool fresh) {
   constexpr int BLOCK = 1024;
   const int nmax = je;
   const int j0 = jb + threadIdx.x;
@@ -269,10 +284,10 @@ __device__ __forceinline__ void project_wide(const ffsrl_batch& b, const ffsrl_p
     float4 A = make_float4(0.0f, 0.0f, 0.0f, 0.0f), B = A;
     TetG g;
     if (HE) {
-      A = ldp<COH>(P, vb + s.e.q.x);
-      B = ldp<COH>(P, vb + s.e.q.y);
+      A = ps(s.e.q.x);
+      B = ps(s.e.q.y);
     }
-    if (HT) g = gather_tet<COH>(b, P, vb, s.t, e.tbase + ct(jj), T, NH);
+    if (HT) g = gather_tet(b, ps, s.t, e.tbase + ct(jj), T, NH);
     if (HE && jj < ne) process_edge<CARRY, COH>(b, prm, e, c, s.e, A, B, jj, fresh);
     if (HT && jj < nt) process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, g, jj, fresh, NH);
   };
@@ -299,10 +314,10 @@ __device__ __forceinline__ void project_wide(const ffsrl_batch& b, const ffsrl_p
 //
 // Items jb .. je-1 (tets below nt, edges below ne); the batched frame passes
 // the whole range, a shared env one chunk of it.
-template <int BLOCK, bool CARRY, bool COH>
+template <int BLOCK, bool CARRY, bool COH, class PS>
 __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
-                                        const FrameCtx& c, const NHConst& nhk, const float* P, int64_t vb, int64_t T,
-                                        int nt, int ne, int jb, int je, bool fresh) {
+                                        const FrameCtx& c, const NHConst& nhk, const PS& ps, int64_t T, int nt,
+                                        int ne, int jb, int je, bool fresh) {
   const int nmax = je;
   const int j0 = jb + threadIdx.x;
   auto ids = [&](Slot& s, int jj) {
@@ -310,10 +325,10 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
     if (jj < ne) s.e = load_edge(b, e.ebase + jj);
   };
   auto gat = [&](Slot& s, int jj) {
-    if (jj < nt) s.g = gather_tet<COH>(b, P, vb, s.t, e.tbase + jj, T, prm.nh_on);
+    if (jj < nt) s.g = gather_tet(b, ps, s.t, e.tbase + jj, T, prm.nh_on);
     if (jj < ne) {
-      s.A = ldp<COH>(P, vb + s.e.q.x);
-      s.B = ldp<COH>(P, vb + s.e.q.y);
+      s.A = ps(s.e.q.x);
+      s.B = ps(s.e.q.y);
     }
   };
   auto run = [&](const Slot& s, int jj) {
@@ -343,16 +358,16 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
     const bool nh = prm.nh_on != 0;
     if (nt > 0 && ne > 0) {
       if (nh)
-        project_wide<true, true, true, CARRY, COH>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, fresh);
+        project_wide<true, true, true, CARRY, COH>(b, prm, e, c, nhk, ps, T, nt, ne, jb, je, fresh);
       else
-        project_wide<true, true, false, CARRY, COH>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, fresh);
+        project_wide<true, true, false, CARRY, COH>(b, prm, e, c, nhk, ps, T, nt, ne, jb, je, fresh);
     } else if (nt > 0) {
       if (nh)
-        project_wide<true, false, true, CARRY, COH>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, fresh);
+        project_wide<true, false, true, CARRY, COH>(b, prm, e, c, nhk, ps, T, nt, ne, jb, je, fresh);
       else
-        project_wide<true, false, false, CARRY, COH>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, fresh);
+        project_wide<true, false, false, CARRY, COH>(b, prm, e, c, nhk, ps, T, nt, ne, jb, je, fresh);
     } else if (ne > 0) {
-      project_wide<false, true, false, CARRY, COH>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, fresh);
+      project_wide<false, true, false, CARRY, COH>(b, prm, e, c, nhk, ps, T, nt, ne, jb, je, fresh);
     }
   }
 }
@@ -459,7 +474,8 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
       // ---- projection: tets (volume + Neo-Hookean) and edges (distance) ----
       {
         const int nt = do_tets ? e.nt : 0, ne = do_edges ? e.ne : 0;
-        project<BLOCK, CARRY, false>(b, prm, e, c, nhk, P, vb, T, nt, ne, 0, nt > ne ? nt : ne, false);
+        project<BLOCK, CARRY, false>(b, prm, e, c, nhk, PosGlobal<false>{P, vb}, T, nt, ne, 0, nt > ne ? nt : ne,
+                                     false);
       }
       __syncthreads();
       // ---- apply (+ grasp; + ground, integrate and next predict) ----
@@ -688,17 +704,29 @@ __device__ __forceinline__ ShareChunk share_chunk(const SharePlan& p, int c, int
 // Where the shared frame keeps an env's global sums (2 words per vertex)
 // and chunk counters (taken, done, give-up flag, exits): indexed by env and
 // vertex base (the batch's split_acc / split_ctr), or by list slot el, with
-// slot_nv vertices per slot (a workspace of the library's own).
+// slot_nv vertices per slot (a workspace of the library's own).  fold: the
+// folded form's workspace (the batch's fold_ws, by vertex base; NULL: every
+// env takes the unfolded form).
 struct ShareWs {
   unsigned long long* acc;
   uint32_t* ctr;
   int by_slot;
   int slot_nv;
+  unsigned long long* fold;
+  int64_t fold_stride;
   __device__ unsigned long long* acc_of(int el, const ffsrl_env_desc& e) const {
     return acc + 2 * (by_slot ? (int64_t)el * slot_nv : (int64_t)e.vbase);
   }
   __device__ uint32_t* ctr_of(int el, int env) const { return ctr + 4 * (int64_t)(by_slot ? el : env); }
+  // plane k of the folded form's workspace, at the env's first vertex
+  __device__ unsigned long long* fold_plane(int k, const ffsrl_env_desc& e) const {
+    return fold + 2 * ((int64_t)k * fold_stride + e.vbase);
+  }
 };
+
+// Largest env of the folded form: its positions and sums share the LDS
+// (32 B per vertex).
+constexpr int kFoldMaxVertices = (kLdsBytes - 512) / 32;
 
 // Polls ctr[1] until it reaches target; false when the wait is given up
 // (limit reached, or another workgroup of the env gave up: ctr[2] set).
@@ -805,7 +833,8 @@ __device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, const S
     if (ch.kind == kProjectPass || ch.kind == kCollidePass) {
       if (ch.kind == kProjectPass) {
         const int jb = ch.local * kShareItems, je = jb + kShareItems < nmax ? jb + kShareItems : nmax;
-        project<BLOCK, CARRY, true>(b, prm, e, c, nhk, P, vb, T, nt, ne, jb, je, reset_lambda && ch.it == 0);
+        project<BLOCK, CARRY, true>(b, prm, e, c, nhk, PosGlobal<true>{P, vb}, T, nt, ne, jb, je,
+                                    reset_lambda && ch.it == 0);
       } else {
         const int f0 = ch.local * kShareTris, f1 = f0 + kShareTris < e.nf ? f0 + kShareTris : e.nf;
         collide_env<true>(b, prm, e, env, c, P, vb, &s_contacts, BLOCK, f0, f1);
@@ -871,22 +900,289 @@ __device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, const S
   if (tid == 0 && sat_flag()) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_SATURATED);
 }
 
+
+// ---------------------------------------------------------------------------
+// Folded shared frame (an env of at most kFoldMaxVertices vertices, without
+// the collision pass).  The unfolded frame has two passes per unit (substep
+// x iteration): the projection, then an apply pass that publishes the
+// positions the next projection reads, each ended by a hand-off.  Here the
+// apply is folded into the next pass: pass u holds the projection chunks of
+// unit u and vertex chunks that publish unit u's positions; a workgroup that
+// projects in pass u first computes the env's positions of unit u into LDS
+// from the published positions of unit u - 1 and that unit's global sums
+// (the apply and grasp and, at a substep boundary, the ground clamp,
+// integrate and predict, expression by expression as the one-workgroup
+// frame), then gathers from LDS.  One hand-off per unit.  A final pass of
+// vertex chunks does the last apply and integrate and writes the model's x,
+// v and p.
+//
+// Workspace planes (fold_stride vertices x 16 B, by vertex base):
+//   0-2  the global sums of unit u in plane u % 3;
+//   3-4  the published positions of unit u in plane 3 + u % 2;
+//   5-6  the positions at the start of substep s >= 1 in plane 5 + s % 2
+//        (substep 0 starts from the model's x).
+// Every reader of a plane runs in a later pass than its writers, and no
+// plane is rewritten before every pass that reads it has completed: pass u's
+// vertex chunks clear the sums of unit u - 2 (last read in pass u - 1), and
+// the final pass clears the last two, so every frame leaves the sums at
+// zero.  Sums and published state are read with coherent loads.
+
+// Positions of unit u from unit u - 1: the apply of its sums (+ grasp) and,
+// at a substep boundary, the ground clamp, integrate and next predict.
+// *xs gets the new substep's start position (boundary only).
+__device__ __forceinline__ float4 fold_advance(const FrameCtx& c, const VertexStep& vs, const f3& tip, float4 p4,
+                                               unsigned long long w0, unsigned long long w1, float dr, bool boundary,
+                                               float4 x4, float4* xs) {
+  int32_t fx, fy, fz, n;
+  unpack2(w0, &fx, &fy);
+  unpack2(w1, &fz, &n);
+  n = sum_count(n);
+  f3 p = xyz(p4);
+  if (n > 0) {
+    const float s = c.q * __builtin_amdgcn_rcpf((float)n);
+    p = p + mk((float)fx, (float)fy, (float)fz) * s;
+  }
+  if (dr != 0.0f) p = grasp_pull<true>(p, tip, dr, vs.ks);
+  if (!boundary) return with_w(p, p4.w);
+  if (p.y < vs.ground) p.y = vs.ground;
+  const f3 v = (p - xyz(x4)) * vs.inv_dt;
+  *xs = with_w(p, x4.w);
+  return with_w(predict(p, v, x4.w, vs.g, vs.dt, vs.damp1), p4.w);
+}
+
+template <int BLOCK, bool CARRY>
+__device__ void share_fold_env(const ffsrl_batch& b, const ffsrl_params& prm, const ShareWs& ws, int el, int env,
+                               int refit, unsigned long long* lds) {
+  __shared__ int s_chunk, s_ok, s_last;
+  const ffsrl_env_desc e = b.envs[env];
+  const SharePlan pl = share_plan(e, prm, false);
+  const int unit = pl.cc + pl.cv, I = prm.iterations, U = prm.substeps * I;
+  uint32_t* ctr = ws.ctr_of(el, env);
+  const int nv = e.nv, tid = threadIdx.x;
+  const int64_t vb = e.vbase;
+  FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
+  float4* Pl = reinterpret_cast<float4*>(lds + 2 * nv);  // positions of the unit this workgroup projects
+  VertexStep vs;
+  vs.g = mk(prm.gravity[0], prm.gravity[1], prm.gravity[2]);
+  vs.dt = prm.dt;
+  vs.damp1 = 1.0f - prm.damping;
+  vs.inv_dt = 1.0f / prm.dt;
+  vs.ground = prm.ground;
+  vs.ks = prm.drag_ks;
+  const NHConst nhk = nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt);
+  const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
+  const int64_t T = b.n_tet_total;
+  const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
+  const int nt = (prm.vol_mode != FFSRL_OFF || prm.nh_on) ? e.nt : 0;
+  const int ne = prm.dist_mode != FFSRL_OFF ? e.ne : 0;
+  const int nmax = nt > ne ? nt : ne;
+  const float* X = b.x + 4 * vb;  // the env's rows of the model state
+  const float* V = b.v + 4 * vb;
+  auto sums = [&](int u) { return ws.fold_plane(u % 3, e); };
+  auto pub = [&](int u) { return reinterpret_cast<float*>(ws.fold_plane(3 + u % 2, e)); };
+  auto start = [&](int s) { return s == 0 ? X : reinterpret_cast<const float*>(ws.fold_plane(5 + s % 2, e)); };
+
+  // positions of unit u of vertex i from the published state; *xs and
+  // *boundary report a new substep's start position
+  // positions of unit u of vertices lo .. hi-1 from the published state,
+  // handed to out(i, position, new substep start).  Each thread takes K
+  // vertices at a time and issues all their loads before any use (indices
+  // clamped into range, only the hand-over guarded), so the loads of a group
+  // are in flight together.
+  constexpr int K = 4;
+  auto for_unit = [&](int u, int lo, int hi, auto&& out) {
+    const int last = hi - 1;
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (u == 0) {
+      for (int i0 = lo + tid; i0 < hi; i0 += K * BLOCK) {
+        float4 x[K], v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int i = i0 + k * BLOCK < hi ? i0 + k * BLOCK : last;
+          x[k] = ld4(X, i);
+          v[k] = ld4(V, i);
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (i0 + k * BLOCK < hi)
+            out(i0 + k * BLOCK, with_w(predict(xyz(x[k]), xyz(v[k]), x[k].w, vs.g, vs.dt, vs.damp1), x[k].w), zero);
+      }
+      return;
+    }
+    const int prev = u - 1;
+    const unsigned long long* g = sums(prev);
+    const float* pp = pub(prev);
+    const float* st = start(prev / I);
+    auto run = [&](auto boundary) {
+      constexpr bool BND = decltype(boundary)::value;
+      for (int i0 = lo + tid; i0 < hi; i0 += K * BLOCK) {
+        float4 p4[K], x4[K];
+        unsigned long long w0[K], w1[K];
+        float dr[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int i = i0 + k * BLOCK < hi ? i0 + k * BLOCK : last;
+          p4[k] = ld4c(pp, i);
+          w0[k] = ldu64c(g, 2 * i);
+          w1[k] = ldu64c(g, 2 * i + 1);
+          dr[k] = b.drag[vb + i];
+          x4[k] = BND ? ld4c(st, i) : zero;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (i0 + k * BLOCK >= hi) continue;
+          float4 xs = zero;
+          const float4 p = fold_advance(c, vs, tip, p4[k], w0[k], w1[k], dr[k], BND, x4[k], &xs);
+          out(i0 + k * BLOCK, p, xs);
+        }
+      }
+    };
+    if (prev % I == I - 1)
+      run(std::integral_constant<bool, true>());
+    else
+      run(std::integral_constant<bool, false>());
+  };
+
+  for (int i = tid; i < 2 * nv; i += BLOCK) lds[i] = 0ull;
+  if (tid == 0) sat_flag() = 0;
+  int pend_base = -1, pend = 0, pend_u = 0, lds_u = -1;
+  for (;;) {
+    __syncthreads();  // LDS and the shared words of the previous chunk are done with
+    if (tid == 0) s_chunk = (int)atomicAdd(&ctr[0], 1u);
+    __syncthreads();
+    const int ci = s_chunk;
+    int pass = 0, base = 0, local = 0;
+    bool proj = false, fin = false;
+    if (ci < pl.total) {
+      pass = ci / unit;  // U for the final pass (its cv chunks follow the U units)
+      base = pass * unit;
+      local = ci - base;
+      fin = pass == U;
+      proj = !fin && local < pl.cc;
+      if (!proj && !fin) local -= pl.cc;
+    }
+    if (pend && (ci >= pl.total || base != pend_base)) {
+      // leave the pass: add the LDS sums to the unit's sums, then count the chunks
+      unsigned long long* g = sums(pend_u);
+      for (int i = tid; i < nv; i += BLOCK) {
+        const unsigned long long w0 = lds[2 * i], w1 = lds[2 * i + 1];
+        if (w0) {
+          atomicAdd(&g[2 * i], w0);
+          lds[2 * i] = 0ull;
+        }
+        if (w1) {
+          atomicAdd(&g[2 * i + 1], w1);
+          lds[2 * i + 1] = 0ull;
+        }
+      }
+      wait_vmem();
+      __syncthreads();
+      if (tid == 0) atomicAdd(&ctr[1], (uint32_t)pend);
+      pend = 0;
+    }
+    if (ci >= pl.total) break;
+    if (tid == 0) s_ok = base == 0 || share_wait(ctr, (uint32_t)base);
+    __syncthreads();
+    if (!s_ok) {
+      if (tid == 0) atomicOr(&ctr[2], 1u);
+      break;
+    }
+    if (proj) {
+      if (lds_u != pass) {  // the first projection chunk of this pass here: the unit's positions
+        for_unit(pass, 0, nv, [&](int i, float4 p, float4) { Pl[i] = p; });
+        __syncthreads();
+        lds_u = pass;
+      }
+      const int jb = local * kShareItems, je = jb + kShareItems < nmax ? jb + kShareItems : nmax;
+      project<BLOCK, CARRY, true>(b, prm, e, c, nhk, PosLds{Pl}, T, nt, ne, jb, je,
+                                  reset_lambda && pass % I == 0);
+      pend_base = base;
+      pend_u = pass;
+      pend += 1;
+      continue;
+    }
+    const int v0 = local * kShareVerts, v1 = v0 + kShareVerts < nv ? v0 + kShareVerts : nv;
+    if (!fin) {
+      // publish unit `pass` (and a new substep's start), clear the sums of unit pass - 2
+      float* P = pub(pass);
+      float* Xs = pass > 0 && (pass - 1) % I == I - 1 ? const_cast<float*>(start(pass / I)) : nullptr;
+      unsigned long long* old = pass >= 2 ? sums(pass - 2) : nullptr;
+      for_unit(pass, v0, v1, [&](int i, float4 p, float4 xs) {
+        st4(P, i, p);
+        if (Xs) st4(Xs, i, xs);
+        if (old) {
+          old[2 * i] = 0ull;
+          old[2 * i + 1] = 0ull;
+        }
+      });
+    } else {
+      // the last apply (+ grasp), ground clamp and integrate: the model's x, v, p
+      const int last = U - 1;
+      const unsigned long long* g = sums(last);
+      const float* P = pub(last);
+      const float* Xs = start(last / I);
+      unsigned long long* g1 = sums(last);
+      unsigned long long* g2 = U >= 2 ? sums(last - 1) : nullptr;
+      for (int i = v0 + tid; i < v1; i += BLOCK) {
+        int32_t fx, fy, fz, n;
+        unpack2(ldu64c(g, 2 * i), &fx, &fy);
+        unpack2(ldu64c(g, 2 * i + 1), &fz, &n);
+        n = sum_count(n);
+        const float4 p4 = ld4c(P, i);
+        f3 p = xyz(p4);
+        if (n > 0) {
+          const float sc = c.q * __builtin_amdgcn_rcpf((float)n);
+          p = p + mk((float)fx, (float)fy, (float)fz) * sc;
+        }
+        const float dr = b.drag[vb + i];
+        if (dr != 0.0f) p = grasp_pull<true>(p, tip, dr, vs.ks);
+        if (p.y < vs.ground) p.y = vs.ground;
+        const float4 x4 = ld4c(Xs, i);
+        const f3 v = (p - xyz(x4)) * vs.inv_dt;
+        st4(b.x, vb + i, with_w(p, x4.w));
+        st4(b.v, vb + i, with_w(v, 0.0f));
+        st4(b.p, vb + i, with_w(p, p4.w));
+        g1[2 * i] = 0ull;
+        g1[2 * i + 1] = 0ull;
+        if (g2) {
+          g2[2 * i] = 0ull;
+          g2[2 * i + 1] = 0ull;
+        }
+      }
+    }
+    wait_vmem();
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(&ctr[1], 1u) + 1u == (uint32_t)pl.total;
+    __syncthreads();
+    if (refit && s_last) refit_env<BLOCK, true>(b, e);  // every chunk of the frame is done
+  }
+  __syncthreads();
+  if (tid == 0 && sat_flag()) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_SATURATED);
+}
+
 // Hardware XCD id (0..15) -> dense index 0..nxcd-1 (-1: no XCD of that id).
 struct XcdMap {
   int8_t index[16];
 };
 
-template <int BLOCK, bool CARRY, bool COLLIDE>
+// FOLD (without COLLIDE only): envs of at most kFoldMaxVertices vertices take
+// the folded form (ws.fold is set), the others the unfolded one.  The
+// unfolded kernels carry no folded code, so their registers are unchanged.
+template <int BLOCK, bool CARRY, bool COLLIDE, bool FOLD>
 __global__ __launch_bounds__(BLOCK) void k_frame_shared(ffsrl_batch b, ffsrl_params prm, const int32_t* list, int n,
                                                         int refit, int nxcd, XcdMap map, ShareWs ws) {
-  extern __shared__ unsigned long long lds[];
+  extern __shared__ __align__(16) unsigned long long lds[];
   __shared__ int s_lastwg, s_incomplete;
   const int x = map.index[xcc_id() & 15];
   const int mine = (x >= 0 && x < nxcd && x < n) ? (n - 1 - x) / nxcd + 1 : 0;  // slots x, x + nxcd, ...
   const int rank = (int)blockIdx.x / nxcd;                           // spreads the XCD's workgroups
   for (int k = 0; k < mine; ++k) {
     const int el = x + nxcd * ((rank + k) % mine);
-    share_env<BLOCK, CARRY, COLLIDE>(b, prm, ws, el, list ? list[el] : el, refit, lds);
+    const int env = list ? list[el] : el;
+    if (FOLD && !COLLIDE && b.envs[env].nv <= kFoldMaxVertices)
+      share_fold_env<BLOCK, CARRY>(b, prm, ws, el, env, refit, lds);
+    else
+      share_env<BLOCK, CARRY, COLLIDE>(b, prm, ws, el, env, refit, lds);
   }
   // The last workgroup out checks that every env's frame completed, clears
   // the global sums of any frame that did not (a workgroup that gave up
@@ -904,8 +1200,15 @@ __global__ __launch_bounds__(BLOCK) void k_frame_shared(ffsrl_batch b, ffsrl_par
     if (threadIdx.x == 0) s_incomplete = atomicAdd(&ctr[1], 0u) != (uint32_t)share_plan(e, prm, COLLIDE).total;
     __syncthreads();
     if (s_incomplete) {
-      unsigned long long* gacc = ws.acc_of(el, e);
-      for (int i = threadIdx.x; i < 2 * e.nv; i += BLOCK) atomicExch(&gacc[i], 0ull);
+      if (FOLD && !COLLIDE && e.nv <= kFoldMaxVertices) {
+        for (int k = 0; k < 3; ++k) {  // the three sum planes of the folded form
+          unsigned long long* g = ws.fold_plane(k, e);
+          for (int i = threadIdx.x; i < 2 * e.nv; i += BLOCK) atomicExch(&g[i], 0ull);
+        }
+      } else {
+        unsigned long long* gacc = ws.acc_of(el, e);
+        for (int i = threadIdx.x; i < 2 * e.nv; i += BLOCK) atomicExch(&gacc[i], 0ull);
+      }
     }
     if (threadIdx.x == 0) {
       if (s_incomplete) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_INCOMPLETE);
@@ -925,8 +1228,10 @@ __global__ void k_xcc_probe(int32_t* out) {
 template <int BLOCK>
 static int launch_shared(const ffsrl_batch& b, const ffsrl_params& prm, const int32_t* list, int n, int refit,
                          bool carry, int lds, int nxcd, const XcdMap& map, const ShareWs& ws, hipStream_t st) {
-  auto kern = prm.collide ? (carry ? k_frame_shared<BLOCK, true, true> : k_frame_shared<BLOCK, false, true>)
-                          : (carry ? k_frame_shared<BLOCK, true, false> : k_frame_shared<BLOCK, false, false>);
+  const bool fold = ws.fold != nullptr;
+  auto kern = prm.collide ? (carry ? k_frame_shared<BLOCK, true, true, false> : k_frame_shared<BLOCK, false, true, false>)
+              : fold ? (carry ? k_frame_shared<BLOCK, true, false, true> : k_frame_shared<BLOCK, false, false, true>)
+                     : (carry ? k_frame_shared<BLOCK, true, false, false> : k_frame_shared<BLOCK, false, false, false>);
   FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   int dev = 0, cus = 0, per_cu = 0;
@@ -971,14 +1276,46 @@ __global__ void k_predict(ffsrl_batch b, ffsrl_params prm, int64_t vb, int nv) {
   st4(b.p, vb + i, with_w(predict(xyz(x), xyz(v), x.w, g, prm.dt, 1.0f - prm.damping), x.w));
 }
 
+// Items of one plain launch: those of env `env` (off NULL: item = thread),
+// or of envs eb .. eb+n-1 in one launch (the reference's launch model: one
+// launch per kernel over every environment), where thread t is item
+// t - off[k] of env eb + k, off[] holding the prefix sums of the envs' item
+// counts (off[k] <= t < off[k+1], found by binary search).
+struct ItemMap {
+  const int64_t* off;
+  int eb, n, env;
+};
+__device__ __forceinline__ bool map_item(const ItemMap& m, int64_t t, int* env, int* local) {
+  if (!m.off) {
+    *env = m.env;
+    *local = (int)t;
+    return true;
+  }
+  if (t >= m.off[m.n]) return false;
+  int lo = 0, hi = m.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m.off[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  *env = m.eb + lo;
+  *local = (int)(t - m.off[lo]);
+  return true;
+}
+__device__ __forceinline__ int64_t gtid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
 __device__ __forceinline__ void g_add(const ffsrl_batch& b, int64_t V, int64_t i, f3 d) {
   unsafeAtomicAdd(&b.dp[i], d.x);
   unsafeAtomicAdd(&b.dp[V + i], d.y);
   unsafeAtomicAdd(&b.dp[2 * V + i], d.z);
 }
 
-__global__ void k_volume(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, int64_t V) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_volume(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e1, ItemMap m, int64_t V) {
+  int env, t;
+  if (!map_item(m, gtid(), &env, &t)) return;
+  const ffsrl_env_desc e = m.off ? b.envs[env] : e1;
   if (t >= e.nt) return;
   const int64_t ti = e.tbase + t, vb = e.vbase;
   const int4 q = ldi4(b.tets, ti);
@@ -1001,8 +1338,10 @@ __global__ void k_volume(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, int6
   }
 }
 
-__global__ void k_neohookean(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, int64_t V) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_neohookean(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e1, ItemMap m, int64_t V) {
+  int env, t;
+  if (!map_item(m, gtid(), &env, &t)) return;
+  const ffsrl_env_desc e = m.off ? b.envs[env] : e1;
   if (t >= e.nt) return;
   const int64_t ti = e.tbase + t, vb = e.vbase, T = b.n_tet_total;
   const int4 q = ldi4(b.tets, ti);
@@ -1020,8 +1359,10 @@ __global__ void k_neohookean(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, 
   }
 }
 
-__global__ void k_distance(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e, int64_t V) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_distance(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e1, ItemMap m, int64_t V) {
+  int env, k;
+  if (!map_item(m, gtid(), &env, &k)) return;
+  const ffsrl_env_desc e = m.off ? b.envs[env] : e1;
   if (k >= e.ne) return;
   const int64_t ei = e.ebase + k, vb = e.vbase;
   const int2 q = ldi2(b.edges, ei);
@@ -1059,9 +1400,11 @@ __global__ void k_apply(ffsrl_batch b, int64_t vb, int nv, int64_t V) {
   b.cnt[g] = 0;
 }
 
-__global__ void k_grasp(ffsrl_batch b, ffsrl_params prm, int env, int64_t vb, int nv) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
+__global__ void k_grasp(ffsrl_batch b, ffsrl_params prm, int64_t vb1, int nv1, ItemMap m) {
+  int env, i;
+  if (!map_item(m, gtid(), &env, &i)) return;
+  const int64_t vb = m.off ? (int64_t)b.envs[env].vbase : vb1;
+  if (!m.off && i >= nv1) return;
   const float dr = b.drag[vb + i];
   if (dr == 0.0f) return;
   const float* t = b.tips + (env * 4 + 2) * 3;
@@ -1073,8 +1416,10 @@ __global__ void k_grasp(ffsrl_batch b, ffsrl_params prm, int env, int64_t vb, in
 // surface triangle of env `env`; responses summed into dp (no count), then
 // k_collide_apply subtracts them, so every triangle reads the positions of
 // the start of the pass.
-__global__ void k_collide(ffsrl_batch b, ffsrl_params prm, int env, ffsrl_env_desc e, int64_t V) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_collide(ffsrl_batch b, ffsrl_params prm, ffsrl_env_desc e1, ItemMap m, int64_t V) {
+  int env, f;
+  if (!map_item(m, gtid(), &env, &f)) return;
+  const ffsrl_env_desc e = m.off ? b.envs[env] : e1;
   if (f >= e.nf) return;
   const int64_t vb = e.vbase;
   const int4 t = ldi4(b.tris, (int64_t)e.fbase + f);
@@ -1384,9 +1729,21 @@ int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int3
     map = g_xcd_map[dev];
   }
   const bool carry = carries_lambda(*prm);
-  const int lds = lds_bytes_for(b->max_nv < 1 ? 1 : b->max_nv);
+  const int max_nv = b->max_nv < 1 ? 1 : b->max_nv;
+  int lds = lds_bytes_for(max_nv);
+  const bool fold = b->fold_ws && !prm->collide;
+  if (fold) {
+    // every env with vbase + nv inside the planes (checked from the host mirror when given)
+    if (b->fold_stride < 1) return fail(FFSRL_E_ARG, "fold_ws needs fold_stride");
+    if (b->host_envs)
+      for (int i = 0; i < b->n_env; ++i)
+        if ((int64_t)b->host_envs[i].vbase + b->host_envs[i].nv > b->fold_stride)
+          return fail(FFSRL_E_ARG, "fold_stride %lld < vbase + nv of env %d", (long long)b->fold_stride, i);
+    lds = std::max(lds, 32 * std::min(max_nv, kFoldMaxVertices));  // positions + sums of a folded env
+  }
   hipStream_t st = as_stream(stream);
-  const ShareWs ws{reinterpret_cast<unsigned long long*>(b->split_acc), b->split_ctr, 0, 0};
+  const ShareWs ws{reinterpret_cast<unsigned long long*>(b->split_acc), b->split_ctr, 0, 0,
+                   fold ? reinterpret_cast<unsigned long long*>(b->fold_ws) : nullptr, b->fold_stride};
   switch (block) {
     case 256: return launch_shared<256>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, ws, st);
     case 512: return launch_shared<512>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, ws, st);
@@ -1421,7 +1778,8 @@ int ffsrl_project_volume(const ffsrl_batch* b, const ffsrl_params* prm, int32_t 
   if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nt) continue;
-    hipLaunchKernelGGL(k_volume, dim3(nblk(h[i].nt, 256)), dim3(256), 0, st, *b, *prm, h[i], V);
+    hipLaunchKernelGGL(k_volume, dim3(nblk(h[i].nt, 256)), dim3(256), 0, st, *b, *prm, h[i], ItemMap{nullptr, 0, 0, i},
+                       V);
   }
   FFSRL_HIP(hipGetLastError());
   return 0;
@@ -1438,7 +1796,8 @@ int ffsrl_project_neohookean(const ffsrl_batch* b, const ffsrl_params* prm, int3
   if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nt) continue;
-    hipLaunchKernelGGL(k_neohookean, dim3(nblk(h[i].nt, 256)), dim3(256), 0, st, *b, *prm, h[i], V);
+    hipLaunchKernelGGL(k_neohookean, dim3(nblk(h[i].nt, 256)), dim3(256), 0, st, *b, *prm, h[i],
+                       ItemMap{nullptr, 0, 0, i}, V);
   }
   FFSRL_HIP(hipGetLastError());
   return 0;
@@ -1455,7 +1814,8 @@ int ffsrl_project_distance(const ffsrl_batch* b, const ffsrl_params* prm, int32_
   if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].ne) continue;
-    hipLaunchKernelGGL(k_distance, dim3(nblk(h[i].ne, 256)), dim3(256), 0, st, *b, *prm, h[i], V);
+    hipLaunchKernelGGL(k_distance, dim3(nblk(h[i].ne, 256)), dim3(256), 0, st, *b, *prm, h[i], ItemMap{nullptr, 0, 0, i},
+                       V);
   }
   FFSRL_HIP(hipGetLastError());
   return 0;
@@ -1486,7 +1846,8 @@ int ffsrl_grasp(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32
   if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nv) continue;
-    hipLaunchKernelGGL(k_grasp, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, *prm, i, (int64_t)h[i].vbase, h[i].nv);
+    hipLaunchKernelGGL(k_grasp, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, *prm, (int64_t)h[i].vbase, h[i].nv,
+                       ItemMap{nullptr, 0, 0, i});
   }
   FFSRL_HIP(hipGetLastError());
   return 0;
@@ -1504,7 +1865,7 @@ int ffsrl_collide(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int
   if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
   for (int i = eb; i < ee; ++i) {
     if (!h[i].nv || !h[i].nf) continue;
-    hipLaunchKernelGGL(k_collide, dim3(nblk(h[i].nf, 256)), dim3(256), 0, st, *b, *prm, i, h[i], V);
+    hipLaunchKernelGGL(k_collide, dim3(nblk(h[i].nf, 256)), dim3(256), 0, st, *b, *prm, h[i], ItemMap{nullptr, 0, 0, i}, V);
     hipLaunchKernelGGL(k_collide_apply, dim3(nblk(h[i].nv, 256)), dim3(256), 0, st, *b, (int64_t)h[i].vbase, h[i].nv,
                        V);
   }
@@ -1541,6 +1902,149 @@ int ffsrl_zero_lambdas(const ffsrl_batch* b, int32_t eb, int32_t ee, void* strea
     if (b->lam_t && h[i].nt)
       hipLaunchKernelGGL(k_zero, dim3(nblk(h[i].nt, 256)), dim3(256), 0, st, b->lam_t, (int64_t)h[i].lam_tbase, h[i].nt);
   }
+  FFSRL_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The reference's launch model (ffsrl_frame_do): every plain kernel launched
+// once per substep / iteration over all envs of the range, envs found in the
+// prefix sums of their item counts (ItemMap); the refit one launch per refit
+// group over all envs.  The prefix sums live in device memory kept per env
+// range and uploaded again only when the host mirror's counts change.
+}  // extern "C"
+
+namespace {
+
+__global__ void k_refit_group_do(ffsrl_batch b, ItemMap m, int L) {
+  int env, k;
+  if (!map_item(m, gtid(), &env, &k)) return;
+  const ffsrl_env_desc e = b.envs[env];
+  refit_node(b, e, b.level_off[e.lvlbase + L] + k);
+}
+
+struct DoMaps {
+  std::vector<int64_t> host;  // [4 + groups][n + 1]: vertices, tets, edges, triangles, refit groups
+  int64_t* dev = nullptr;
+  size_t cap = 0;
+};
+std::mutex g_do_mu;
+DoMaps g_do;
+
+// Prefix sums of envs eb .. ee-1 (host mirror), uploaded when they changed.
+int do_maps(const ffsrl_batch* b, int eb, int ee, int groups, const int64_t** dev) {
+  const int n = ee - eb;
+  std::vector<int64_t> h((size_t)(4 + groups) * (n + 1), 0);
+  for (int k = 0; k < n; ++k) {
+    const ffsrl_env_desc& e = b->host_envs[eb + k];
+    const int64_t cnt[4] = {e.nv, e.nt, e.ne, e.nf};
+    for (int a = 0; a < 4; ++a) h[(size_t)a * (n + 1) + k + 1] = h[(size_t)a * (n + 1) + k] + cnt[a];
+    for (int L = 0; L < groups; ++L) {
+      const int64_t c = L < e.nlvl ? b->host_level_off[e.lvlbase + L + 1] - b->host_level_off[e.lvlbase + L] : 0;
+      const size_t r = (size_t)(4 + L) * (n + 1);
+      h[r + k + 1] = h[r + k] + c;
+    }
+  }
+  std::lock_guard<std::mutex> lock(g_do_mu);
+  if (h != g_do.host) {
+    // earlier frames (on any stream) may still read the buffer
+    FFSRL_HIP(hipDeviceSynchronize());
+    if (h.size() > g_do.cap) {
+      if (g_do.dev) (void)hipFree(g_do.dev);
+      g_do.dev = nullptr;
+      g_do.cap = 0;
+      FFSRL_HIP(hipMalloc(&g_do.dev, h.size() * sizeof(int64_t)));
+      g_do.cap = h.size();
+    }
+    FFSRL_HIP(hipMemcpy(g_do.dev, h.data(), h.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    g_do.host.swap(h);
+  }
+  *dev = g_do.dev;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffsrl_frame_do(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, int32_t refit,
+                   void* stream) {
+  int rc = check_batch(b, prm);
+  if (rc) return rc;
+  if (!b->dp || !b->cnt) return fail(FFSRL_E_ARG, "the plain form needs dp/cnt scratch");
+  const ffsrl_env_desc* h;
+  int64_t V;
+  if ((rc = plain_envs(b, eb, ee, &h, &V))) return rc;
+  if (eb == ee) return 0;
+  if (refit && (!b->level_off || !b->level_nodes || !b->level_info || !b->bvh_lo || !b->bvh_hi || !b->tris ||
+                !b->host_level_off))
+    return fail(FFSRL_E_ARG, "frame_do: the refit needs the BVH arrays and host_level_off");
+  if (prm->collide && (!b->tris || !b->caps_base || !b->caps_radius || !b->in_collision))
+    return fail(FFSRL_E_ARG, "frame_do: collision needs triangles, capsules and the in_collision counters");
+  // the envs' vertices (and carried lambdas) must be contiguous: one range per launch
+  for (int i = eb + 1; i < ee; ++i) {
+    if ((int64_t)h[i].vbase != (int64_t)h[i - 1].vbase + h[i - 1].nv)
+      return fail(FFSRL_E_ARG, "frame_do: vertices of envs %d and %d are not contiguous", i - 1, i);
+    if (b->lam_t && (int64_t)h[i].lam_tbase != (int64_t)h[i - 1].lam_tbase + h[i - 1].nt)
+      return fail(FFSRL_E_ARG, "frame_do: tet lambdas of envs %d and %d are not contiguous", i - 1, i);
+    if (b->lam_e && (int64_t)h[i].lam_ebase != (int64_t)h[i - 1].lam_ebase + h[i - 1].ne)
+      return fail(FFSRL_E_ARG, "frame_do: edge lambdas of envs %d and %d are not contiguous", i - 1, i);
+  }
+  const int n = ee - eb;
+  int groups = 0;
+  for (int i = eb; i < ee; ++i) groups = std::max(groups, (int)h[i].nlvl);
+  if (!refit) groups = 0;
+  const int64_t* off = nullptr;
+  if ((rc = do_maps(b, eb, ee, groups, &off))) return rc;
+  std::vector<int64_t> tot(4 + groups);
+  for (int a = 0; a < 4 + groups; ++a) {
+    int64_t t = 0;
+    for (int i = eb; i < ee; ++i) {
+      const ffsrl_env_desc& e = h[i];
+      if (a < 4) {
+        t += a == 0 ? e.nv : a == 1 ? e.nt : a == 2 ? e.ne : e.nf;
+      } else {
+        const int L = a - 4;
+        if (L < e.nlvl) t += b->host_level_off[e.lvlbase + L + 1] - b->host_level_off[e.lvlbase + L];
+      }
+    }
+    tot[a] = t;
+  }
+  auto map = [&](int a) { return ItemMap{off + (size_t)a * (n + 1), eb, n, 0}; };
+  auto blocks = [](int64_t items) { return dim3((unsigned)((items + 255) / 256)); };
+  const int64_t vb0 = h[eb].vbase;
+  const int nvt = (int)tot[0];
+  hipStream_t st = as_stream(stream);
+  const bool carry = carries_lambda(*prm);
+  const bool reset = carry && (!prm->lambda_legacy || prm->iterations > 1);
+  for (int s = 0; s < prm->substeps; ++s) {
+    if (nvt) hipLaunchKernelGGL(k_predict, blocks(nvt), dim3(256), 0, st, *b, *prm, vb0, nvt);
+    if (reset) {
+      if (b->lam_e && tot[2])
+        hipLaunchKernelGGL(k_zero, blocks(tot[2]), dim3(256), 0, st, b->lam_e, (int64_t)h[eb].lam_ebase, (int)tot[2]);
+      if (b->lam_t && tot[1])
+        hipLaunchKernelGGL(k_zero, blocks(tot[1]), dim3(256), 0, st, b->lam_t, (int64_t)h[eb].lam_tbase, (int)tot[1]);
+    }
+    for (int it = 0; it < prm->iterations; ++it) {
+      if (prm->vol_mode != FFSRL_OFF && tot[1])
+        hipLaunchKernelGGL(k_volume, blocks(tot[1]), dim3(256), 0, st, *b, *prm, h[eb], map(1), V);
+      if (prm->dist_mode != FFSRL_OFF && tot[2])
+        hipLaunchKernelGGL(k_distance, blocks(tot[2]), dim3(256), 0, st, *b, *prm, h[eb], map(2), V);
+      if (prm->nh_on && tot[1])
+        hipLaunchKernelGGL(k_neohookean, blocks(tot[1]), dim3(256), 0, st, *b, *prm, h[eb], map(1), V);
+      if (nvt) {
+        hipLaunchKernelGGL(k_apply, blocks(nvt), dim3(256), 0, st, *b, vb0, nvt, V);
+        hipLaunchKernelGGL(k_grasp, blocks(nvt), dim3(256), 0, st, *b, *prm, vb0, nvt, map(0));
+      }
+      if (prm->collide && tot[3] && nvt) {
+        hipLaunchKernelGGL(k_collide, blocks(tot[3]), dim3(256), 0, st, *b, *prm, h[eb], map(3), V);
+        hipLaunchKernelGGL(k_collide_apply, blocks(nvt), dim3(256), 0, st, *b, vb0, nvt, V);
+      }
+    }
+    if (nvt) hipLaunchKernelGGL(k_bounds_integrate, blocks(nvt), dim3(256), 0, st, *b, *prm, vb0, nvt);
+  }
+  for (int L = 0; L < groups; ++L)
+    if (tot[4 + L]) hipLaunchKernelGGL(k_refit_group_do, blocks(tot[4 + L]), dim3(256), 0, st, *b, map(4 + L), L);
   FFSRL_HIP(hipGetLastError());
   return 0;
 }

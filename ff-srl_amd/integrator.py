@@ -55,6 +55,15 @@ class SimIntegratorDO:
         else:
             lib.ffsrl_frame(m.batch, m.params, order, self.block, int(bool(refit)), m.stream())
 
+    def stepModelDO(self, simModel: SimModelDO, refit: bool = False) -> None:
+        """One frame in the reference's launch model: every kernel launched once
+        per substep and iteration over all environments (``ffsrl_frame_do``;
+        the plain form's kernels and float atomics), and with ``refit`` one
+        refit launch per BVH refit group over all environments."""
+        m = simModel
+        m.enable_plain()
+        lib.ffsrl_frame_do(m.batch, m.params, 0, m.numEnvs, int(bool(refit)), m.stream())
+
     def stepModelPlain(self, simModel: SimModelDO, env_begin: int = 0, env_end: int = None) -> None:
         """One frame with one launch per kernel per environment (the library
         loops over the environments of each call)."""

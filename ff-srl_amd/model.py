@@ -645,6 +645,9 @@ class SimModelDO:
         b.split_acc = _ptr(getattr(self, "split_acc", None))
         b.split_ctr = _ptr(getattr(self, "split_ctr", None))
         b.status = _ptr(self.status)
+        fw = getattr(self, "fold_ws", None)
+        b.fold_ws = _ptr(fw)
+        b.fold_stride = self.numVertices if fw is not None else 0
         b.host_envs = self.env_desc_host.ctypes.data
         b.host_level_off = self.level_off_host.ctypes.data
         return b
@@ -794,11 +797,16 @@ class SimModelDO:
             self._batch.dp = _ptr(self.dp)
             self._batch.cnt = _ptr(self.cnt)
 
-    def enable_shared(self):
+    def enable_shared(self, fold: Optional[bool] = None):
         """Allocate the global Jacobi sums and chunk counters of the shared-env
         frame (``ffsrl_frame_shared``); both start at zero and every frame
-        leaves them at zero.  Probes the device's XCD count, so call it
-        before capturing a shared frame into a graph."""
+        leaves them at zero.  With ``fold`` (the default) also the workspace
+        of its folded form (7 x 16 B per vertex), which envs of at most 5,104
+        vertices take when the frame has no collision pass; ``fold=False``
+        frees it, so every env takes the unfolded form (the results are the
+        same, bit for bit); ``None`` keeps the current choice.  Probes the
+        device's XCD count, so call it before capturing a shared frame into a
+        graph."""
         if getattr(self, "split_acc", None) is None:
             self.split_acc = torch.zeros((self.numVertices, 2), dtype=torch.int64, device=self.device)
             self.split_ctr = torch.zeros((self.numEnvs, 4), dtype=torch.int32, device=self.device)
@@ -806,6 +814,15 @@ class SimModelDO:
             self._batch.split_ctr = _ptr(self.split_ctr)
             if lib.ffsrl_xcd_count() <= 0:
                 raise RuntimeError(f"ffsrl_xcd_count: {lib.ffsrl_last_error().decode()}")
+        if fold is None:
+            fold = getattr(self, "_fold", True)
+        self._fold = bool(fold)
+        if fold and getattr(self, "fold_ws", None) is None:
+            self.fold_ws = torch.zeros((7, self.numVertices, 2), dtype=torch.int64, device=self.device)
+        elif not fold:
+            self.fold_ws = None
+        self._batch.fold_ws = _ptr(self.fold_ws)
+        self._batch.fold_stride = self.numVertices if self.fold_ws is not None else 0
 
     def frame_status(self) -> torch.Tensor:
         """Per-env status bits of the batched and shared frames, sticky until

@@ -21,13 +21,14 @@ each get a distinct mesh.
 Grasp size.  The grasp (dragConstraintsDO, integrator.py:412-442) pulls every
 grasped vertex toward one point, the tool tip.  Eight grasped vertices
 collapse onto it within a frame; with the stiff XPBD distance (compliance
-1e-6) and volume (compliance 0) constraints of the mixed and large
-workloads, the collapsed tets then drive the scheme apart: the float32 and
-float64 oracles differ by up to 2 mesh diagonals after one frame of the
-mixed workload.  One grasped vertex keeps both workloads reproducible (the
-two oracles agree to 7e-6 of the diagonal over 3 frames of the mixed
-workload and 1 frame of the large one), so those workloads grasp one vertex;
-the Neo-Hookean-only ellipsoid workload keeps its eight (`grasp_k`).
+1e-6) and volume (compliance 0) constraints of the mixed workload (volume
+and Neo-Hookean in the large one), the collapsed tets then drive the scheme
+apart: the float32 and float64 oracles differ by up to 2 mesh diagonals
+after one frame of the mixed workload.  One grasped vertex keeps both
+workloads reproducible (the two oracles agree to 7e-6 of the diagonal over
+3 frames of the mixed workload and 1 frame of the large one), so those
+workloads grasp one vertex; the Neo-Hookean-only ellipsoid workload keeps
+its eight (`grasp_k`).
 """
 from __future__ import annotations
 
@@ -86,8 +87,11 @@ def _gen(args):
 def generate(specs, workers: Optional[int] = None):
     """Generate meshes for (shape, tets, seed) specs on a process pool: forked
     before the GPU runtime is initialised, spawned after it (a forked child
-    of a process with a live GPU runtime is unsafe)."""
-    workers = workers or min(16, os.cpu_count() or 1)
+    of a process with a live GPU runtime is unsafe).  FFSRL_GEN_WORKERS sets
+    the pool size (1: generate in this process, e.g. under a profiler whose
+    preloaded library should not run in children).  The pool is closed and
+    joined, so its workers exit on their own rather than by a signal."""
+    workers = workers or int(os.environ.get("FFSRL_GEN_WORKERS", "0")) or min(16, os.cpu_count() or 1)
     if workers <= 1 or len(specs) <= 8:
         return [_gen(s) for s in specs]
     live = False
@@ -96,8 +100,15 @@ def generate(specs, workers: Optional[int] = None):
         live = torch.cuda.is_initialized()
     except Exception:
         pass
-    with mp.get_context("spawn" if live else "fork").Pool(min(workers, len(specs))) as pool:
-        return pool.map(_gen, specs, chunksize=4)
+    pool = mp.get_context("spawn" if live else "fork").Pool(min(workers, len(specs)))
+    try:
+        out = pool.map(_gen, specs, chunksize=4)
+        pool.close()
+        pool.join()
+        return out
+    except BaseException:
+        pool.terminate()
+        raise
 
 
 def _tool(mesh, rng_offset=0.02, k=8):

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 10
+#define FFSRL_ABI_VERSION 11
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -133,6 +133,12 @@ typedef struct ffsrl_batch {
                                      zero before the first frame, left zero by every frame */
   int32_t* status;                /* [n_env] FFSRL_STATUS_* bits, or'ed by the batched and shared
                                      frames (required by them); the caller clears them */
+  uint64_t* fold_ws;              /* workspace of the shared frame's folded form: 7 planes of
+                                     fold_stride x 16 B (3 sum planes, zero before the first frame
+                                     and left at zero by every frame; 2 + 2 state planes); envs of
+                                     at most 5,104 vertices without collision take the folded form
+                                     when it is given (NULL: none does) */
+  int64_t fold_stride;            /* vertices per plane of fold_ws (>= every env's vbase + nv) */
   /* host mirrors needed by the plain form, which launches per env (and per
    * BVH level) from the host: the env table and the level offsets */
   const ffsrl_env_desc* host_envs;   /* [n_env] (host memory) */
@@ -402,6 +408,19 @@ int ffsrl_bounds_integrate(const ffsrl_batch* b, const ffsrl_params* prm, int32_
 int ffsrl_zero_lambdas(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, void* stream);
 /* One launch per env and BVH level, deepest level first (refitBVHLoop). */
 int ffsrl_bvh_refit_plain(const ffsrl_batch* b, int32_t env_begin, int32_t env_end, void* stream);
+
+/* One frame in the reference's own launch model: SimIntegratorDO.stepModel
+ * (integrator.py:1640-1744) launching each DO kernel once per substep and
+ * iteration over every environment, then refitBVHLoop (bvh.py:419-454) as one
+ * launch per refit group over every environment.  The kernels are the plain
+ * form's (global float atomics; results equal the per-env plain form up to
+ * the order of the atomics); envs env_begin .. env_end-1 must have
+ * contiguous vertices (and lambdas when carried).  Needs dp/cnt and the host
+ * mirrors; keeps device prefix sums of the envs' counts (re-uploaded when
+ * the counts change), so it synchronises the host on its first call for an
+ * env range and is not for graph capture then. */
+int ffsrl_frame_do(const ffsrl_batch* b, const ffsrl_params* prm, int32_t env_begin, int32_t env_end,
+                   int32_t refit, void* stream);
 
 /* remapAToBKernel (utils.py:32-42): dst[map[2i]] = src[map[2i+1]] for float4
  * arrays, i < n. */

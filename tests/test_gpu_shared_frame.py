@@ -5,8 +5,11 @@ ffsrl_frame): positions, velocities, predicted positions, carried lambdas,
 BVH bounds and contact counts equal bit for bit (both sum the Jacobi deltas
 as integers), for mixed sizes, XPBD with carried lambdas, the grasp, the
 instrument collision, env lists, every block size, and repeated frames (the
-chunk counters must be left at zero).  The physics itself is pinned against
-the oracle by the batched frame's own parity tests.
+chunk counters and sums must be left at zero).  Envs of at most 5,104
+vertices take the folded form (one hand-off per unit, the apply folded into
+the next pass) unless the model's fold workspace is off; the tests run both
+forms, on batches that mix folded and unfolded envs.  The physics itself is
+pinned against the oracle by the batched frame's own parity tests.
 """
 import numpy as np
 import pytest
@@ -60,6 +63,23 @@ def _clean(m):
     incomplete = m.frame_status() & pkg_status_incomplete()
     assert int(incomplete.abs().sum()) == 0, m.frame_status()
     assert int(m.split_ctr.abs().sum()) == 0 and int(m.split_acc.abs().sum()) == 0
+    if getattr(m, "fold_ws", None) is not None:
+        assert int(m.fold_ws[:3].abs().sum()) == 0          # the folded form's sum planes
+
+
+FOLD_LIMIT = 5104
+
+
+def _folds(meshes):
+    """How many of the meshes the folded form takes."""
+    return sum(m.num_vertices <= FOLD_LIMIT for m in meshes)
+
+
+def _clean_sums(m):
+    torch.cuda.synchronize()
+    assert int(m.split_ctr.abs().sum()) == 0 and int(m.split_acc.abs().sum()) == 0
+    if getattr(m, "fold_ws", None) is not None:
+        assert int(m.fold_ws[:3].abs().sum()) == 0
 
 
 def pkg_status_incomplete():
@@ -67,10 +87,11 @@ def pkg_status_incomplete():
     return importlib.import_module("ff-srl_amd")._lib.STATUS_INCOMPLETE
 
 
-@pytest.mark.parametrize("block", [1024, 512, 256])
-def test_shared_equals_batched_mixed(pkg, gpu, block):
-    meshes = _meshes(pkg, 12, 1000, 20000, seed=5)
+@pytest.mark.parametrize("block,fold", [(1024, True), (512, True), (256, True), (1024, False), (256, False)])
+def test_shared_equals_batched_mixed(pkg, gpu, block, fold):
+    meshes = _meshes(pkg, 12, 1000, 30000, seed=5)
     a, b = _pair(pkg, gpu, meshes, NH_DV)
+    b.enable_shared(fold=fold)
     ref = pkg.SimIntegratorDO(gpu, block=block)
     sh = pkg.SimIntegratorDO(gpu, block=block, shared=True)
     for f in range(3):
@@ -79,12 +100,15 @@ def test_shared_equals_batched_mixed(pkg, gpu, block):
         _equal(a, b)
         _clean(b)
     assert bool(torch.isfinite(b.x).all())
-    print(f"\n[shared] block {block}: 12 envs ({sum(m.num_tets for m in meshes)} tets) equal after 3 frames")
+    print(f"\n[shared] block {block} fold {fold}: 12 envs ({sum(m.num_tets for m in meshes)} tets, "
+          f"{_folds(meshes) if fold else 0} folded) equal after 3 frames")
 
 
-def test_shared_xpbd_carried_lambdas(pkg, gpu):
+@pytest.mark.parametrize("fold", [True, False])
+def test_shared_xpbd_carried_lambdas(pkg, gpu, fold):
     meshes = _meshes(pkg, 6, 2000, 12000, seed=17)
     a, b = _pair(pkg, gpu, meshes, XPBD_I2)
+    b.enable_shared(fold=fold)
     assert a.lam_t is not None
     ref, sh = pkg.SimIntegratorDO(gpu), pkg.SimIntegratorDO(gpu, shared=True)
     for f in range(3):
@@ -94,14 +118,15 @@ def test_shared_xpbd_carried_lambdas(pkg, gpu):
     _clean(b)
 
 
-def test_shared_env_list(pkg, gpu):
+@pytest.mark.parametrize("fold", [True, False])
+def test_shared_env_list(pkg, gpu, fold):
     meshes = _meshes(pkg, 8, 1000, 15000, seed=23)
     a, b = _pair(pkg, gpu, meshes, NH_DV)
+    b.enable_shared(fold=fold)
     envs = torch.tensor([5, 0, 2], dtype=torch.int32, device=gpu)
     x0 = b.x.clone()
     for f in range(2):
         pkg.lib.ffsrl_frame_envs(a.batch, a.params, envs.data_ptr(), 3, 1024, 1, a.stream())
-        b.enable_shared()
         pkg.lib.ffsrl_frame_shared(b.batch, b.params, envs.data_ptr(), 3, 1024, 1, b.stream())
     _equal(a, b)
     _clean(b)
@@ -219,7 +244,8 @@ def test_env_at_the_vertex_limit(pkg, gpu):
             _clean(big)
 
 
-def test_abandoned_frame_is_reported_and_cleaned(pkg, gpu):
+@pytest.mark.parametrize("fold", [True, False])
+def test_abandoned_frame_is_reported_and_cleaned(pkg, gpu, fold):
     """A shared frame whose bounded waits give up: the host sets two envs'
     give-up flags before the launch, so their first wait that does not find
     the pass complete gives up and every workgroup exits normally.  The
@@ -229,8 +255,9 @@ def test_abandoned_frame_is_reported_and_cleaned(pkg, gpu):
     bit, with no status."""
     meshes = _meshes(pkg, 6, 4000, 20000, seed=29)
     a, b = _pair(pkg, gpu, meshes, NH_DV, grasp=False)
+    b.enable_shared(fold=fold)
     ref, sh = pkg.SimIntegratorDO(gpu), pkg.SimIntegratorDO(gpu, shared=True)
-    sh.stepModel(b, refit=True)                       # allocates the shared workspace
+    sh.stepModel(b, refit=True)
     ref.stepModel(a, refit=True)
     _equal(a, b)
     _clean(b)
@@ -242,7 +269,7 @@ def test_abandoned_frame_is_reported_and_cleaned(pkg, gpu):
     print(f"\n[shared] status after the abandoned frame: {st.tolist()}")
     assert st[1] & pkg._lib.STATUS_INCOMPLETE and st[4] & pkg._lib.STATUS_INCOMPLETE
     assert (st[[0, 2, 3, 5]] == 0).all()
-    assert int(b.split_ctr.abs().sum()) == 0 and int(b.split_acc.abs().sum()) == 0
+    _clean_sums(b)
     with pytest.raises(pkg.FFSRLError, match="incomplete shared frame"):
         b.check_status()
     b.clear_status()
