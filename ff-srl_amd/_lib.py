@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("FFSRL_LIB") or os.path.join(HERE, "_build", "libffsrl
 
 OFF, PBD, XPBD = 0, 1, 2
 E_ARG, E_LIMIT, E_STATE = -1, -2, -3
-ABI_VERSION = 11
+ABI_VERSION = 12
 STATUS_SATURATED, STATUS_INCOMPLETE = 1, 2   # FFSRL_STATUS_* bits of ffsrl_batch.status
 MAX_TERMS = 128                              # FFSRL_MAX_TERMS
 
@@ -66,6 +66,7 @@ class Camera(C.Structure):
     _fields_ = [
         ("origin", C.c_float * 3), ("dirs", _vp), ("height", C.c_int32), ("width", C.c_int32),
         ("mode", C.c_int32), ("light_dir", C.c_float * 3), ("light_intensity", C.c_float),
+        ("traversal", C.c_int32),
     ]
 
 

@@ -74,7 +74,8 @@ class RaycastRendererDO:
 
     def __init__(self, device, simModel: SimModelDO, cameraPos=(3.0, 3.0, 3.0), cameraRot=(0.0, 0.0, 0.0),
                  lightPos=(3.0, 0.0, 0.0), lightIntensity=0.3, resolution: int = 512, mode: str = "headless",
-                 horizontalAperture=19.2, verticalAperture=19.2, render_mode: int = RenderMode.depth_map):
+                 horizontalAperture=19.2, verticalAperture=19.2, render_mode: int = RenderMode.depth_map,
+                 traversal: int = 0):
         if resolution % 4:
             raise ValueError("resolution must be a multiple of 4")
         self.device = torch.device(device)
@@ -82,6 +83,7 @@ class RaycastRendererDO:
         self.mode = mode
         self.height = self.width = int(resolution)
         self.render_mode = render_mode
+        self.traversal = int(traversal)
         self.h_aperture, self.v_aperture = horizontalAperture, verticalAperture
         self.cameraPos = np.asarray(cameraPos, dtype=np.float64)
         self.cameraRot = np.asarray(cameraRot, dtype=np.float64)
@@ -104,6 +106,7 @@ class RaycastRendererDO:
         cam.mode = int(self.render_mode)
         cam.light_dir[:] = tuple(float(c) for c in self.lightPos)
         cam.light_intensity = self.lightIntensity
+        cam.traversal = self.traversal
         self.camera = cam
 
     def lookAt(self, at) -> None:
@@ -136,6 +139,13 @@ class RaycastRendererDO:
     def set_mode(self, render_mode: int) -> None:
         self.render_mode = render_mode
         self.camera.mode = int(render_mode)
+
+    def set_traversal(self, traversal: int) -> None:
+        """0 picks by image size (2x2-ray quads per lane from 128x128 pixels,
+        one ray per lane below), 1 one ray per lane, 2 quads.  The images are
+        the same bit for bit."""
+        self.traversal = int(traversal)
+        self.camera.traversal = self.traversal
 
     def depth_camera(self) -> Camera:
         """A copy of the camera in depth mode (the renderer keeps its mode)."""

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 11
+#define FFSRL_ABI_VERSION 12
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -174,7 +174,17 @@ typedef struct ffsrl_camera {
                                      colours where a triangle has none), 5 per-triangle colour */
   float light_dir[3];
   float light_intensity;
+  int32_t traversal;              /* FFSRL_TRAVERSE_*: 0 (zero-initialised) picks by image size */
 } ffsrl_camera;
+
+/* Ray traversal of the renderer.  Both forms give the same hits, depths and
+ * colours bit for bit; they differ only in speed.  PER_RAY: a lane traces
+ * one supersampled ray.  QUAD: a lane traces the 2x2 rays of its pixel with
+ * one stack (a BVH of at most 2^28 - 1 nodes).  AUTO: QUAD from 128x128
+ * output pixels, PER_RAY below. */
+#define FFSRL_TRAVERSE_AUTO 0
+#define FFSRL_TRAVERSE_PER_RAY 1
+#define FFSRL_TRAVERSE_QUAD 2
 
 /* One tick of serving requests (device arrays of n entries, env ids distinct). */
 #define FFSRL_REQ_STEP 0

@@ -33,7 +33,7 @@ def test_every_declared_symbol_is_exported(pkg):
 
 
 def test_abi_version(pkg):
-    assert pkg.lib.ffsrl_abi_version() == 11
+    assert pkg.lib.ffsrl_abi_version() == 12
     assert pkg.lib.ffsrl_frame_max_vertices() >= 10000
 
 
@@ -127,6 +127,19 @@ def test_argument_errors_are_reported(pkg):
         pkg.lib.ffsrl_frame_shared(b, prm, None, 0, 1024, 0, None)
     b.status = dummy.ctypes.data
     assert pkg.lib.ffsrl_frame_shared(b, prm, None, 0, 1024, 0, None) == 0      # nothing listed: no launch
+    # the render's traversal choice is checked before the empty range returns
+    for f in ("tris", "bvh_lo", "bvh_hi"):
+        setattr(b, f, dummy.ctypes.data)
+    cam = L.Camera()
+    cam.dirs = dummy.ctypes.data
+    cam.width = cam.height = 8
+    cam.mode = 4
+    cam.traversal = 3
+    with pytest.raises(pkg.FFSRLError, match="render traversal 3"):
+        pkg.lib.ffsrl_render(b, cam, 0, 0, dummy.ctypes.data, None, None, None, None, None, None, None)
+    for t in (0, 1, 2):
+        cam.traversal = t
+        assert pkg.lib.ffsrl_render(b, cam, 0, 0, dummy.ctypes.data, None, None, None, None, None, None, None) == 0
 
 
 def test_frame_rejects_too_many_terms_per_vertex(pkg):

@@ -70,15 +70,17 @@ def _bvh_dict(topo):
             "tri_map": topo.tri_map}
 
 
-def test_refit_and_depth_bit_exact(pkg, orc, gpu):
+@pytest.mark.parametrize("traversal", [1, 2])
+def test_refit_and_depth_bit_exact(pkg, orc, gpu, traversal):
     """C1 camera: 128x128 depth (256x256 rays) after 20 frames with the refit
-    fused into the frame; bounds, per-ray hits and depth equal the oracle's."""
+    fused into the frame; bounds, per-ray hits and depth equal the oracle's,
+    with one ray per lane (1) and with 2x2-ray quads per lane (2)."""
     mesh, v0 = _c1_model(pkg)
     model = pkg.SimModelDO([mesh], [0], device=gpu, initialVelocity=v0, **C1)
     integ = pkg.SimIntegratorDO(gpu)
     for _ in range(20):
         integ.stepModel(model, refit=True)
-    ren = pkg.RaycastRendererDO(gpu, model, cameraPos=(2.2, 1.8, 2.6), resolution=128)
+    ren = pkg.RaycastRendererDO(gpu, model, cameraPos=(2.2, 1.8, 2.6), resolution=128, traversal=traversal)
     ren.lookAt((0.5, 0.45, 0.5))
     hit = torch.full((4 * 128 * 128,), -7, dtype=torch.int32, device=gpu)
     thit = torch.zeros(4 * 128 * 128, dtype=torch.float32, device=gpu)
@@ -123,10 +125,13 @@ def _edge_rays(n, rng):
     return out
 
 
-def test_depth_edge_rays_bit_exact(pkg, orc, gpu):
+@pytest.mark.parametrize("traversal", [1, 2])
+def test_depth_edge_rays_bit_exact(pkg, orc, gpu, traversal):
     """Depth, hits and t for rays with zero, tiny and axis-aligned direction
     components, from origins that lie in planes of the mesh's grid, equal the
-    oracle's (the division form of the reference) bit for bit."""
+    oracle's (the division form of the reference) bit for bit.  In the quad
+    traversal (2) the four rays of a pixel point in unrelated directions
+    here, so their reach masks part at the root."""
     mesh, _ = _c1_model(pkg)
     model = pkg.SimModelDO([mesh], [0], device=gpu, **C1)
     pkg.SimBVH(gpu, model)
@@ -136,7 +141,7 @@ def test_depth_edge_rays_bit_exact(pkg, orc, gpu):
     bvh = _bvh_dict(model.topo[0])
     mn, mx = orc.refit(bvh, mesh.tris, x)
     rng = np.random.default_rng(5)
-    ren = pkg.RaycastRendererDO(gpu, model, cameraPos=(0.0, 0.5, 2.5), resolution=16)
+    ren = pkg.RaycastRendererDO(gpu, model, cameraPos=(0.0, 0.5, 2.5), resolution=16, traversal=traversal)
     for origin in ((xs[0], ys[4], 2.5), (xs[3], ys[0], 2.5), (xs[5], ys[5], 2.5)):
         ren.camera.origin[:] = tuple(float(c) for c in np.asarray(origin, np.float32))
         dirs = _edge_rays(32 * 32, rng)
@@ -484,11 +489,13 @@ def test_autotune_keeps_state_and_results(pkg, gpu, tmp_path, iters):
     assert again.tune(tuned) == cfg
 
 
+@pytest.mark.parametrize("traversal", [1, 2])
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 5])
-def test_rgb_modes_and_normals_bit_exact(pkg, orc, gpu, mode):
+def test_rgb_modes_and_normals_bit_exact(pkg, orc, gpu, mode, traversal):
     """Colour renders (grayscale, texture lookup, normal map, vertex colour,
     per-triangle colour, each Lambert-lit by the raw light position) and the
-    vertex normals equal the oracle's on the same deformed positions."""
+    vertex normals equal the oracle's on the same deformed positions, with
+    one ray per lane (1) and with 2x2-ray quads per lane (2)."""
     sc = pkg.scenes
     meshes = [sc.solid("ellipsoid", 3000, seed=41), sc.solid("box", 2000, seed=42)]
     tips = np.stack([np.repeat((sc.top_point(m) + np.array([0.0, 0.05, 0.0]))[None], 4, axis=0) for m in meshes])
@@ -499,7 +506,7 @@ def test_rgb_modes_and_normals_bit_exact(pkg, orc, gpu, mode):
     for _ in range(5):
         integ.stepModel(model, refit=True)
     ren = pkg.RaycastRendererDO(gpu, model, cameraPos=(0.0, 1.6, 5.5), resolution=64, lightPos=(3.0, 2.0, 4.0),
-                                lightIntensity=0.3, render_mode=mode)
+                                lightIntensity=0.3, render_mode=mode, traversal=traversal)
     ren.lookAt((0.0, 0.45, 0.0))
     rgb = ren.render_rgb()
     torch.cuda.synchronize()
