@@ -12,6 +12,13 @@
 // Both are within the positional tolerance of the reference; the choice only
 // moves the last bits.  This file is compiled with multiply-add contraction
 // (no bit-exact result depends on it: the refit is min/max only).
+//
+// `scale` (a power of two, 1 in the plain form) multiplies the returned
+// deltas: the batched frames pass 2^-dp_exp and get their fixed-point
+// quanta directly.  It is applied to the constraint's scalar multiplier
+// (dlambda); scaling by a power of two commutes with rounding, so the
+// deltas are exactly scale times the unscaled ones (lambda itself stays
+// unscaled).
 #pragma once
 
 #include "common.hpp"
@@ -45,7 +52,7 @@ struct Ar<true> {
 // On success returns 1 and e = edgeDP; the caller applies dP[a] -= e*wa and
 // dP[b] += e*wb and counts both ends.
 template <bool FAST>
-__device__ __forceinline__ int distance_pbd(float4 A, float4 B, float rest, float kS, f3* e) {
+__device__ __forceinline__ int distance_pbd(float4 A, float4 B, float rest, float kS, f3* e, float scale = 1.0f) {
   using M = Ar<FAST>;
   if (rest < kFeps) return 0;
   f3 dir = xyz(B) - xyz(A);
@@ -55,7 +62,7 @@ __device__ __forceinline__ int distance_pbd(float4 A, float4 B, float rest, floa
   if (w < kFeps) return 0;
   float C = len - rest;
   if (FAST) {
-    *e = dir * (M::div(-C * kS, w) * M::div(1.0f, len));
+    *e = dir * ((M::div(-C * kS, w) * M::div(1.0f, len)) * scale);
   } else {
     *e = ((dir / len) * (-C)) * kS / w;
   }
@@ -65,7 +72,8 @@ __device__ __forceinline__ int distance_pbd(float4 A, float4 B, float rest, floa
 // distanceConstraintsXPBD (integrator.py:191-240); alpha = compliance/dt^2.
 // *lam is read and updated in place.
 template <bool FAST>
-__device__ __forceinline__ int distance_xpbd(float4 A, float4 B, float rest, float alpha, float* lam, f3* e) {
+__device__ __forceinline__ int distance_xpbd(float4 A, float4 B, float rest, float alpha, float* lam, f3* e,
+                                             float scale = 1.0f) {
   using M = Ar<FAST>;
   if (rest < kFeps) return 0;
   f3 dir = xyz(B) - xyz(A);
@@ -76,7 +84,7 @@ __device__ __forceinline__ int distance_xpbd(float4 A, float4 B, float rest, flo
   float C = len - rest;
   float dl = M::div(-(C + alpha * (*lam)), w + alpha);
   *lam = *lam + dl;
-  *e = M::vdiv(dir, len) * dl;  // normalize(dir) * dlambda
+  *e = M::vdiv(dir, len) * (dl * scale);  // normalize(dir) * dlambda
   return 1;
 }
 
@@ -103,12 +111,12 @@ __device__ __forceinline__ VolTerms vol_terms(const float4 P[4]) {
 
 // volumeConstraints, PBD form (integrator.py:444-504): |V|, C = V0 - |V|.
 template <bool FAST>
-__device__ __forceinline__ int volume_pbd(const float4 P[4], float rest, float kS, f3 d[4]) {
+__device__ __forceinline__ int volume_pbd(const float4 P[4], float rest, float kS, f3 d[4], float scale = 1.0f) {
   VolTerms t = vol_terms(P);
   float volume = FAST ? fabsf(t.six_vol) * (1.0f / 6.0f) : fabsf(t.six_vol) / 6.0f;
   if (fabsf(t.w) < kFeps) return 0;
   float C = rest - volume;
-  float dl = Ar<FAST>::div(-kS * C, t.w);
+  const float dl = Ar<FAST>::div(-kS * C, t.w) * scale;
 #pragma unroll
   for (int i = 0; i < 4; ++i) d[i] = (t.q[i] * dl) * P[i].w;
   return 1;
@@ -116,15 +124,17 @@ __device__ __forceinline__ int volume_pbd(const float4 P[4], float rest, float k
 
 // volumeConstraintsXPBD (integrator.py:507-576): signed V, C = V - V0.
 template <bool FAST>
-__device__ __forceinline__ int volume_xpbd(const float4 P[4], float rest, float alpha, float* lam, f3 d[4]) {
+__device__ __forceinline__ int volume_xpbd(const float4 P[4], float rest, float alpha, float* lam, f3 d[4],
+                                           float scale = 1.0f) {
   VolTerms t = vol_terms(P);
   float volume = FAST ? t.six_vol * (1.0f / 6.0f) : t.six_vol / 6.0f;
   if (fabsf(t.w) < kFeps) return 0;
   float C = volume - rest;
   float dl = Ar<FAST>::div(-(C + alpha * (*lam)), t.w + alpha);
   *lam = *lam + dl;
+  const float ds = dl * scale;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) d[i] = (t.q[i] * dl) * P[i].w;
+  for (int i = 0; i < 4; ++i) d[i] = (t.q[i] * ds) * P[i].w;
   return 1;
 }
 
@@ -155,7 +165,7 @@ __host__ __device__ inline NHConst nh_const(float volC, float devC, float dT) {
 // additions.
 template <bool FAST>
 __device__ __forceinline__ int neohookean(const float4 P[4], float rest, const float M[9], const NHConst& k,
-                                          f3 d[4]) {
+                                          f3 d[4], float scale = 1.0f) {
   using A = Ar<FAST>;
   const f3 a = xyz(P[0]);
   const f3 p1 = xyz(P[1]) - a, p2 = xyz(P[2]) - a, p3 = xyz(P[3]) - a;
@@ -188,7 +198,7 @@ __device__ __forceinline__ int neohookean(const float4 P[4], float rest, const f
   if (w < kFeps) return 0;
   const float C_h = dot(F[0], c1) - 1.0f - k.gamma;
   const float inv_rest = FAST ? __builtin_amdgcn_rcpf(rest) : 0.0f;
-  const float dl_h = A::div(-C_h, w + (FAST ? k.alpha_h * inv_rest : k.alpha_h / rest));
+  const float dl_h = A::div(-C_h, w + (FAST ? k.alpha_h * inv_rest : k.alpha_h / rest)) * scale;
   float rsi, C_d;
   if (FAST) {
     const float r2 = (dot(F[0], F[0]) + dot(F[1], F[1])) + dot(F[2], F[2]);
@@ -209,7 +219,7 @@ __device__ __forceinline__ int neohookean(const float4 P[4], float rest, const f
   }
   w = w + A::sq_len(s) * m[0];
   if (w < kFeps) return 0;
-  const float dl_d = A::div(-C_d, w + (FAST ? k.alpha_d * inv_rest : k.alpha_d / rest)) * rsi;
+  const float dl_d = (A::div(-C_d, w + (FAST ? k.alpha_d * inv_rest : k.alpha_d / rest)) * rsi) * scale;
   s = mk(0.0f, 0.0f, 0.0f);
 #pragma unroll
   for (int j = 0; j < 3; ++j) {

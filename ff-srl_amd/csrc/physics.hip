@@ -112,10 +112,11 @@ __device__ __forceinline__ void unpack2(unsigned long long w, int32_t* lo, int32
   *lo = l;
   *hi = (int32_t)((uint32_t)(w >> 32) - (uint32_t)(l >> 31));
 }
-// Adds sign * d (fixed point) and n to the count of vertex i.
+// Adds sign * d (given in quanta: the delta times c.inv_q) to the fixed-point
+// sums of vertex i and n to its count.
 template <int SIGN>
 __device__ __forceinline__ void lds_acc(const FrameCtx& c, int i, f3 d, uint32_t n) {
-  const float sx = d.x * c.inv_q, sy = d.y * c.inv_q, sz = d.z * c.inv_q;  // scaling by 2^k is exact
+  const float sx = d.x, sy = d.y, sz = d.z;
   // IEEE 754-2019 maximum propagates NaN, which fails the comparison
   const float m = __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(sx), fabsf(sy)), fabsf(sz));
   const uint32_t mark = m <= kFixMax ? 0u : kSatMark;
@@ -206,16 +207,16 @@ __device__ __forceinline__ void process_tet(const ffsrl_batch& b, const ffsrl_pa
   f3 d[4] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
   uint32_t n = 0;
   if (prm.vol_mode == FFSRL_PBD) {
-    n += volume_pbd<true>(g.P, tc.rest, prm.vol_k, d);
+    n += volume_pbd<true>(g.P, tc.rest, prm.vol_k, d, c.inv_q);
   } else if (prm.vol_mode == FFSRL_XPBD) {
     float lam = 0.0f;
     if (CARRY && !fresh) lam = COH ? ldfc(b.lam_t, e.lam_tbase + j) : b.lam_t[e.lam_tbase + j];
-    n += volume_xpbd<true>(g.P, tc.rest, prm.vol_k, &lam, d);
+    n += volume_xpbd<true>(g.P, tc.rest, prm.vol_k, &lam, d, c.inv_q);
     if (CARRY) b.lam_t[e.lam_tbase + j] = lam;
   }
   if (nh) {
     f3 dn[4];
-    if (neohookean<true>(g.P, tc.rest, g.M, nhk, dn)) {
+    if (neohookean<true>(g.P, tc.rest, g.M, nhk, dn, c.inv_q)) {
       n += 1;
 #pragma unroll
       for (int k = 0; k < 4; ++k) d[k] = d[k] + dn[k];
@@ -237,11 +238,11 @@ __device__ __forceinline__ void process_edge(const ffsrl_batch& b, const ffsrl_p
   f3 d;
   int ok;
   if (prm.dist_mode == FFSRL_PBD) {
-    ok = distance_pbd<true>(A, B, ec.rest, prm.dist_k, &d);
+    ok = distance_pbd<true>(A, B, ec.rest, prm.dist_k, &d, c.inv_q);
   } else {
     float lam = 0.0f;
     if (CARRY && !fresh) lam = COH ? ldfc(b.lam_e, e.lam_ebase + j) : b.lam_e[e.lam_ebase + j];
-    ok = distance_xpbd<true>(A, B, ec.rest, prm.dist_k, &lam, &d);
+    ok = distance_xpbd<true>(A, B, ec.rest, prm.dist_k, &lam, &d, c.inv_q);
     // an early-out leaves lambda unchanged: a fresh one is stored as zero
     if (CARRY && (ok || fresh)) b.lam_e[e.lam_ebase + j] = lam;
   }
@@ -411,9 +412,9 @@ __device__ __forceinline__ void collide_env(const ffsrl_batch& b, const ffsrl_pa
     }
     if (len > 0.0f) {
       atomicAdd(contacts, 1);
-      if (A.w > 0.0f) lds_acc<-1>(c, t.x, r[0], 0u);
-      if (B.w > 0.0f) lds_acc<-1>(c, t.y, r[1], 0u);
-      if (Cc.w > 0.0f) lds_acc<-1>(c, t.z, r[2], 0u);
+      if (A.w > 0.0f) lds_acc<-1>(c, t.x, r[0] * c.inv_q, 0u);
+      if (B.w > 0.0f) lds_acc<-1>(c, t.y, r[1] * c.inv_q, 0u);
+      if (Cc.w > 0.0f) lds_acc<-1>(c, t.z, r[2] * c.inv_q, 0u);
     }
   }
 }
