@@ -97,6 +97,9 @@ def parse():
                     help="shared-env frame: autotuned (auto), forced on, or off")
     ap.add_argument("--fold", choices=["on", "off"], default="on",
                     help="the shared frame's folded form for envs of at most 5,104 vertices (same results)")
+    ap.add_argument("--parts", choices=["on", "off"], default="off",
+                    help="partitioned frame: each env cut into parts, one workgroup per part (same results)")
+    ap.add_argument("--tets-per-part", type=int, default=5120, help="constraints per part of the partitioned frame")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --block and largest-first dispatch instead of the autotuned launch shape")
     ap.add_argument("--plain-frames", type=int, default=2)
@@ -374,6 +377,9 @@ def main():
         log(f"autotuned {tuned} in {time.perf_counter() - t_tune:.1f} s")
     if args.shared != "auto":
         integ.shared = args.shared == "on"
+    if args.parts == "on":
+        integ.shared, integ.parts = False, True
+        model.enable_parts(tets_per_part=args.tets_per_part)
     if integ.shared:
         model.enable_shared(fold=args.fold == "on")
     if args.calibrate_copy:
@@ -434,8 +440,11 @@ def main():
                              f"(2x2 supersampled)", "block": integ.block,
                    "dispatch": "lpt" if integ.dispatch_sorted else "table", "shared": integ.shared,
                    **({"fold": getattr(model, "fold_ws", None) is not None} if integ.shared else {}),
+                   **({"parts": int(sum(model.parts_per_mesh[k] for k in model.env_mesh)),
+                       "tets_per_part": model._parts_cfg["tets_per_part"]} if integ.parts else {}),
                    "chunk": model.topo[0].chunk, "autotuned": tuned is not None,
-                   "form": "batched (1 frame kernel + 1 render kernel per step)",
+                   "form": ("partitioned (claim table + 1 frame kernel + 1 render kernel per step)" if integ.parts
+                            else "batched (1 frame kernel + 1 render kernel per step)"),
                    **({"collision": True} if w.model_kwargs.get("collision") else {})},
         "p50_step_ms": float(np.percentile(step_ms, 50)), "p99_step_ms": float(np.percentile(step_ms, 99)),
         "frame_kernel_ms": float(frame_ms.mean()), "render_kernel_ms": float(render_ms.mean()),
@@ -446,7 +455,7 @@ def main():
     }
     alg = algorithmic_bytes(w, model)
     achieved = alg / (frame_ms.mean() * 1e-3) / 1e9
-    kname = "k_frame_shared" if integ.shared else "k_frame"
+    kname = "k_frame_parts" if integ.parts else "k_frame_shared" if integ.shared else "k_frame"
     traffic, source = measured_traffic().get((w.name, kname), (None, None))
     copy_peak = copy_peak_gbs(dev)
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -15,9 +15,10 @@ LIB_PATH = os.environ.get("FFSRL_LIB") or os.path.join(HERE, "_build", "libffsrl
 
 OFF, PBD, XPBD = 0, 1, 2
 E_ARG, E_LIMIT, E_STATE = -1, -2, -3
-ABI_VERSION = 12
+ABI_VERSION = 13
 STATUS_SATURATED, STATUS_INCOMPLETE = 1, 2   # FFSRL_STATUS_* bits of ffsrl_batch.status
 MAX_TERMS = 128                              # FFSRL_MAX_TERMS
+PART_OWNER, PART_SHARED = 0x80000000, 0x40000000  # FFSRL_PART_* flags of ffsrl_parts.verts
 
 
 class EnvDesc(C.Structure):
@@ -91,6 +92,14 @@ class BvhJob(C.Structure):
     ]
 
 
+class Parts(C.Structure):
+    _fields_ = [
+        ("n_parts", C.c_int32), ("max_local", C.c_int32), ("parts", _vp), ("env_parts", _vp), ("verts", _vp),
+        ("tets16", _vp), ("edges16", _vp), ("sums", _vp), ("stride", C.c_int64), ("ctr", _vp), ("claims", _vp),
+        ("n_claims", C.c_int64), ("host_env_parts", _vp),
+    ]
+
+
 REQ_STEP, REQ_RESET = 0, 1
 
 _i32, _f32, _i64 = C.c_int32, C.c_float, C.c_int64
@@ -126,10 +135,13 @@ SIGNATURES = {
     "ffsrl_zero_lambdas": [_P(Batch), _i32, _i32, _vp],
     "ffsrl_bvh_refit_plain": [_P(Batch), _i32, _i32, _vp],
     "ffsrl_frame_do": [_P(Batch), _P(Params), _i32, _i32, _i32, _vp],
+    "ffsrl_frame_parts": [_P(Batch), _P(Params), _P(Parts), _vp, _i32, _i32, _vp],
     "ffsrl_remap": [_vp, _vp, _vp, _i64, _vp],
     "ffsrl_bvh_build": [_vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _P(_i32), _P(_i32)],
     "ffsrl_color_constraints": [_vp, _i64, _i32, _i32, _vp, _P(_i32)],
     "ffsrl_order_constraints": [_vp, _i64, _i32, _vp, _i32, _i32, _vp],
+    "ffsrl_partition_mesh": [_vp, _i64, _vp, _i64, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _P(_i64), _vp, _vp,
+                             _P(_i32)],
     "ffsrl_graph_begin": [_vp],
     "ffsrl_graph_end": [_vp, _P(_vp)],
     "ffsrl_graph_launch": [_vp, _vp],
@@ -143,6 +155,7 @@ PLAIN_SIGNATURES = {
     "ffsrl_frame_max_vertices": ([], C.c_int),
     "ffsrl_bvh_scratch_bytes": ([_i32], C.c_int64),
     "ffsrl_xcd_count": ([], C.c_int),          # count > 0, or an error code < 0
+    "ffsrl_parts_max_local": ([], C.c_int),
 }
 
 

@@ -10,7 +10,12 @@ Knobs (all leave the results unchanged; the frame's Jacobi sums are integers):
             and a wavefront's position gathers stay local);
   shared    the shared-env frame (every CU on the batch's envs, each env's
             frame cut into chunks), tried when there are at most half as
-            many envs as CUs.
+            many envs as CUs;
+  parts     the partitioned frame (each env's constraints cut into parts,
+            one 1024-thread workgroup per part with the part's state in
+            LDS): 0 (not partitioned), or the tets per part (PART_WHOLE:
+            as few parts as a workgroup's LDS allows), tried without the
+            collision pass (it has none).
 
 A trial runs whole frames on a snapshot of the state, which is restored
 afterwards, so tuning does not advance the simulation.  The cache is a JSON
@@ -28,13 +33,23 @@ from typing import Dict, Optional
 import numpy as np
 import torch
 
-from ._lib import lib
-from .model import LOCAL_CHUNK, SimModelDO
+from ._lib import FFSRLError, lib
+from .model import LOCAL_CHUNK, PART_TETS, SimModelDO
 
 BLOCKS = (256, 512, 1024)
 DISPATCH = ("lpt", "table")
 CHUNKS = (64, 128, 256)
-DEFAULT = {"block": 1024, "dispatch": "lpt", "chunk": LOCAL_CHUNK, "shared": False}
+PART_WHOLE = 1 << 30
+PART_SIZES = (PART_WHOLE, PART_TETS)
+DEFAULT = {"block": 1024, "dispatch": "lpt", "chunk": LOCAL_CHUNK, "shared": False, "parts": 0}
+
+
+def parts_candidates(model: SimModelDO) -> tuple:
+    """(0,) or (0, *PART_SIZES): the partitioned frame (tets per part; 0 =
+    not partitioned) is a candidate without the collision pass."""
+    if model.collision or lib.ffsrl_xcd_count() <= 0:
+        return (0,)
+    return (0,) + PART_SIZES
 
 
 def shared_candidates(model: SimModelDO) -> tuple:
@@ -56,7 +71,7 @@ def key_of(model: SimModelDO) -> str:
     run on this device (an XCD map)."""
     tets = np.array([model.meshes[k].num_tets for k in model.env_mesh], dtype=np.float64)
     flags = (f"d{model.dist_mode}v{model.vol_mode}n{int(model.nh_on)}s{model.simSubsteps}i{model.simConstraints}"
-             f"c{int(model.collision)}x{int(lib.ffsrl_xcd_count() > 0)}")
+             f"c{int(model.collision)}x{int(lib.ffsrl_xcd_count() > 0)}k2")
     return f"envs{_bucket(model.numEnvs)}-tets{_bucket(tets.mean())}-max{_bucket(tets.max())}-{flags}"
 
 
@@ -89,7 +104,11 @@ class Autotuner:
     def _time(model: SimModelDO, cfg: dict, frames: int, stream) -> float:
         order = model.dispatch_order().data_ptr() if cfg["dispatch"] == "lpt" else None
         s = stream.cuda_stream
-        if cfg.get("shared"):
+        if cfg.get("parts"):
+            if getattr(model, "_parts_cfg", {}).get("tets_per_part") != cfg["parts"]:
+                model.enable_parts(tets_per_part=cfg["parts"])
+            run = lambda: lib.ffsrl_frame_parts(model.batch, model.params, model.parts, order, model.numEnvs, 1, s)
+        elif cfg.get("shared"):
             model.enable_shared()
             run = lambda: lib.ffsrl_frame_shared(model.batch, model.params, order, model.numEnvs, cfg["block"], 1, s)
         else:
@@ -104,8 +123,9 @@ class Autotuner:
         return e0.elapsed_time(e1) / frames
 
     def tune(self, model: SimModelDO, frames: int = 2, force: bool = False) -> dict:
-        """Best {block, dispatch, chunk, shared} for this batch (cached), applied to
-        the model's constraint order; the state is left as it was."""
+        """Best {block, dispatch, chunk, shared, parts} for this batch (cached),
+        applied to the model's constraint order (and partition); the state is
+        left as it was."""
         key = key_of(model)
         if not force and key in self.cache:
             cfg = dict(self.cache[key]["best"])
@@ -120,9 +140,15 @@ class Autotuner:
             model.set_constraint_order("local", chunk)          # carried lambdas follow their constraints
             lam = [(t, t.clone()) for t in self._lambdas(model)]
             torch.cuda.synchronize(model.device)
-            for block, dispatch, shared in itertools.product(BLOCKS, DISPATCH, shared_candidates(model)):
-                cfg = {"block": block, "dispatch": dispatch, "chunk": chunk, "shared": shared}
-                ms = self._time(model, cfg, frames, stream)
+            for block, dispatch, shared, parts in itertools.product(BLOCKS, DISPATCH, shared_candidates(model),
+                                                                    parts_candidates(model)):
+                if parts and (block != 1024 or shared):   # the partitioned frame runs 1024-thread parts
+                    continue
+                cfg = {"block": block, "dispatch": dispatch, "chunk": chunk, "shared": shared, "parts": parts}
+                try:
+                    ms = self._time(model, cfg, frames, stream)
+                except (FFSRLError, ValueError):          # a mesh the partition cannot hold
+                    ms = float("inf")
                 for t, saved in state + lam:
                     t.copy_(saved)
                 results.append((ms, cfg))
@@ -143,3 +169,5 @@ class Autotuner:
         t = model.topo[0] if model.topo else None
         if t is not None and (t.constraint_order != "local" or t.chunk != cfg["chunk"]):
             model.set_constraint_order("local", cfg["chunk"])
+        if cfg.get("parts") and getattr(model, "_parts_cfg", {}).get("tets_per_part") != cfg["parts"]:
+            model.enable_parts(tets_per_part=cfg["parts"])

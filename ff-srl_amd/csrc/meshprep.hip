@@ -123,4 +123,108 @@ int ffsrl_order_constraints(const int32_t* elems, int64_t n, int32_t arity, cons
   return 0;
 }
 
+// Partition of one mesh for the partitioned frame: part p takes tets
+// [p nt / P, (p+1) nt / P) and edges [p ne / P, (p+1) ne / P) of the current
+// order; its vertices are those its constraints in use touch, in increasing
+// mesh id (their local ids), plus the untouched vertices, dealt round-robin
+// in increasing id.  A vertex is flagged SHARED when several parts hold it
+// and OWNER on the lowest part that holds it.  Two passes over the
+// constraints with a per-vertex stamp: O(4 nt + 2 ne + nv) plus the sort of
+// each part's vertex list.
+int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, int64_t ne, int32_t nv,
+                         int32_t parts, int32_t use_tets, int32_t use_edges, int32_t* rows, int32_t* verts,
+                         int64_t verts_cap, int64_t* n_verts, uint64_t* tets16, uint32_t* edges16,
+                         int32_t* max_local) {
+  if (parts < 1 || nv < 0 || nt < 0 || ne < 0 || !rows || !verts || !n_verts || !max_local ||
+      (use_tets && nt && (!tets || !tets16)) || (use_edges && ne && (!edges || !edges16)))
+    return fail(FFSRL_E_ARG, "partition_mesh: bad arguments");
+  const int64_t P = parts;
+  auto tbeg = [&](int64_t p) { return p * nt / P; };
+  auto ebeg = [&](int64_t p) { return p * ne / P; };
+  std::vector<int32_t> stamp((size_t)nv, -1), holders((size_t)nv, 0), first((size_t)nv, -1);
+  // visits the vertices of part p's constraints in use, each once per part
+  auto each = [&](int64_t p, auto&& fn) {
+    if (use_tets)
+      for (int64_t t = tbeg(p); t < tbeg(p + 1); ++t)
+        for (int k = 0; k < 4; ++k) fn(tets[4 * t + k]);
+    if (use_edges)
+      for (int64_t e = ebeg(p); e < ebeg(p + 1); ++e)
+        for (int k = 0; k < 2; ++k) fn(edges[2 * e + k]);
+  };
+  for (int64_t p = 0; p < P; ++p)
+    each(p, [&](int32_t v) {
+      if (v < 0 || v >= nv) return;
+      if (stamp[(size_t)v] != (int32_t)p) {
+        stamp[(size_t)v] = (int32_t)p;
+        if (holders[(size_t)v]++ == 0) first[(size_t)v] = (int32_t)p;
+      }
+    });
+  std::vector<std::vector<int32_t>> loose((size_t)P);
+  int64_t nl = 0;
+  for (int32_t v = 0; v < nv; ++v)
+    if (holders[(size_t)v] == 0) {
+      const int32_t p = (int32_t)(nl++ % P);
+      loose[(size_t)p].push_back(v);
+      first[(size_t)v] = p;
+    }
+  std::fill(stamp.begin(), stamp.end(), -1);
+  std::vector<int32_t> local((size_t)nv, 0), list;
+  int64_t off = 0;
+  int32_t most = 0;
+  for (int64_t p = 0; p < P; ++p) {
+    list.clear();
+    bool bad = false;
+    each(p, [&](int32_t v) {
+      if (v < 0 || v >= nv) {
+        bad = true;
+        return;
+      }
+      if (stamp[(size_t)v] != (int32_t)p) {
+        stamp[(size_t)v] = (int32_t)p;
+        list.push_back(v);
+      }
+    });
+    if (bad) return fail(FFSRL_E_ARG, "partition_mesh: a vertex id outside [0, %d)", nv);
+    list.insert(list.end(), loose[(size_t)p].begin(), loose[(size_t)p].end());
+    std::sort(list.begin(), list.end());
+    const int64_t n = (int64_t)list.size();
+    if (off + n > verts_cap) return fail(FFSRL_E_ARG, "partition_mesh: verts holds %lld entries", (long long)verts_cap);
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t v = list[(size_t)i];
+      local[(size_t)v] = (int32_t)i;
+      uint32_t f = (uint32_t)v;
+      if (holders[(size_t)v] > 1) f |= FFSRL_PART_SHARED;
+      if (first[(size_t)v] == (int32_t)p) f |= FFSRL_PART_OWNER;
+      verts[off + i] = (int32_t)f;
+    }
+    if (use_tets)
+      for (int64_t t = tbeg(p); t < tbeg(p + 1); ++t) {
+        uint64_t w = 0;
+        for (int k = 0; k < 4; ++k) w |= (uint64_t)(uint16_t)local[(size_t)tets[4 * t + k]] << (16 * k);
+        tets16[t] = w;
+      }
+    if (use_edges)
+      for (int64_t e = ebeg(p); e < ebeg(p + 1); ++e)
+        edges16[e] = (uint32_t)(uint16_t)local[(size_t)edges[2 * e]] |
+                     ((uint32_t)(uint16_t)local[(size_t)edges[2 * e + 1]] << 16);
+    int32_t* r = rows + 8 * p;
+    r[0] = (int32_t)tbeg(p);
+    r[1] = (int32_t)tbeg(p + 1);
+    r[2] = (int32_t)ebeg(p);
+    r[3] = (int32_t)ebeg(p + 1);
+    r[4] = (int32_t)off;
+    r[5] = (int32_t)n;
+    r[6] = r[7] = 0;
+    off += n;
+    most = std::max(most, (int32_t)n);
+  }
+  if (!use_tets && tets16)
+    for (int64_t t = 0; t < nt; ++t) tets16[t] = 0;
+  if (!use_edges && edges16)
+    for (int64_t e = 0; e < ne; ++e) edges16[e] = 0;
+  *n_verts = off;
+  *max_local = most;
+  return 0;
+}
+
 }  // extern "C"

@@ -144,17 +144,18 @@ struct EdgeIn {
 };
 
 // The frame reads the 16-bit packed ids (half the index bytes of int4 / int2;
-// an env of the batched frame has at most 10,208 vertices).
-__device__ __forceinline__ TetIn load_tet(const ffsrl_batch& b, int64_t ti) {
+// an env of the batched frame has at most 10,208 vertices, a part of the
+// partitioned frame at most kPartMaxLocal).
+__device__ __forceinline__ TetIn load_tet(const ffsrl_batch& b, const uint64_t* t16, int64_t ti) {
   TetIn t;
-  const uint64_t w = b.tets16[ti];
+  const uint64_t w = t16[ti];
   t.q = make_int4((int)(w & 0xffffu), (int)((w >> 16) & 0xffffu), (int)((w >> 32) & 0xffffu), (int)(w >> 48));
   t.rest = b.tet_rest_volume[ti];
   return t;
 }
-__device__ __forceinline__ EdgeIn load_edge(const ffsrl_batch& b, int64_t ei) {
+__device__ __forceinline__ EdgeIn load_edge(const ffsrl_batch& b, const uint32_t* e16, int64_t ei) {
   EdgeIn t;
-  const uint32_t w = b.edges16[ei];
+  const uint32_t w = e16[ei];
   t.q = make_int2((int)(w & 0xffffu), (int)(w >> 16));
   t.rest = b.edge_rest[ei];
   return t;
@@ -252,6 +253,16 @@ __device__ __forceinline__ void process_edge(const ffsrl_batch& b, const ffsrl_p
   }
 }
 
+// The constraints of a projection pass: packed ids (the batch's mesh-local
+// ids, or the partitioned frame's part-local ids) and the first tet and edge
+// of the range within the env's mesh (item j is tet t0 + j and edge e0 + j;
+// the carried lambdas are indexed the same way).
+struct Items {
+  const uint64_t* t16;
+  const uint32_t* e16;
+  int t0, e0;
+};
+
 // Work of one stride for one thread: tet j and edge j of the env.
 struct Slot {
   TetIn t;
@@ -270,16 +281,17 @@ struct Slot {
 // are computed (the edge's registers are free again before the tet's peak).
 template <bool HT, bool HE, bool NH, bool CARRY, bool COH, class PS>
 __device__ __forceinline__ void project_wide(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
-                                             const FrameCtx& c, const NHConst& nhk, const PS& ps, int64_t T, int nt,
-                                             int ne, int jb, int je, bool fresh) {
+                                             const FrameCtx& c, const NHConst& nhk, const PS& ps, const Items& it,
+                                             int64_t T, int nt, int ne, int jb, int je, bool fresh) {
   constexpr int BLOCK = 1024;
   const int nmax = je;
   const int j0 = jb + threadIdx.x;
   auto ct = [&](int jj) { return jj < nt ? jj : nt - 1; };
   auto ce = [&](int jj) { return jj < ne ? jj : ne - 1; };
+  const int tb = e.tbase + it.t0, eb = e.ebase + it.e0;  // 32-bit like the env table
   auto ids = [&](Slot& s, int jj) {
-    if (HT) s.t = load_tet(b, e.tbase + ct(jj));
-    if (HE) s.e = load_edge(b, e.ebase + ce(jj));
+    if (HT) s.t = load_tet(b, it.t16, tb + ct(jj));
+    if (HE) s.e = load_edge(b, it.e16, eb + ce(jj));
   };
   auto gr = [&](const Slot& s, int jj) {
     float4 A = make_float4(0.0f, 0.0f, 0.0f, 0.0f), B = A;
@@ -288,9 +300,9 @@ __device__ __forceinline__ void project_wide(const ffsrl_batch& b, const ffsrl_p
       A = ps(s.e.q.x);
       B = ps(s.e.q.y);
     }
-    if (HT) g = gather_tet(b, ps, s.t, e.tbase + ct(jj), T, NH);
-    if (HE && jj < ne) process_edge<CARRY, COH>(b, prm, e, c, s.e, A, B, jj, fresh);
-    if (HT && jj < nt) process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, g, jj, fresh, NH);
+    if (HT) g = gather_tet(b, ps, s.t, tb + ct(jj), T, NH);
+    if (HE && jj < ne) process_edge<CARRY, COH>(b, prm, e, c, s.e, A, B, it.e0 + jj, fresh);
+    if (HT && jj < nt) process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, g, it.t0 + jj, fresh, NH);
   };
   Slot s0, s1;
   ids(s0, j0);
@@ -317,24 +329,25 @@ __device__ __forceinline__ void project_wide(const ffsrl_batch& b, const ffsrl_p
 // the whole range, a shared env one chunk of it.
 template <int BLOCK, bool CARRY, bool COH, class PS>
 __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
-                                        const FrameCtx& c, const NHConst& nhk, const PS& ps, int64_t T, int nt,
-                                        int ne, int jb, int je, bool fresh) {
+                                        const FrameCtx& c, const NHConst& nhk, const PS& ps, const Items& it,
+                                        int64_t T, int nt, int ne, int jb, int je, bool fresh) {
   const int nmax = je;
   const int j0 = jb + threadIdx.x;
+  const int tb = e.tbase + it.t0, eb = e.ebase + it.e0;  // 32-bit like the env table
   auto ids = [&](Slot& s, int jj) {
-    if (jj < nt) s.t = load_tet(b, e.tbase + jj);
-    if (jj < ne) s.e = load_edge(b, e.ebase + jj);
+    if (jj < nt) s.t = load_tet(b, it.t16, tb + jj);
+    if (jj < ne) s.e = load_edge(b, it.e16, eb + jj);
   };
   auto gat = [&](Slot& s, int jj) {
-    if (jj < nt) s.g = gather_tet(b, ps, s.t, e.tbase + jj, T, prm.nh_on);
+    if (jj < nt) s.g = gather_tet(b, ps, s.t, tb + jj, T, prm.nh_on);
     if (jj < ne) {
       s.A = ps(s.e.q.x);
       s.B = ps(s.e.q.y);
     }
   };
   auto run = [&](const Slot& s, int jj) {
-    if (jj < nt) process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, s.g, jj, fresh, prm.nh_on != 0);
-    if (jj < ne) process_edge<CARRY, COH>(b, prm, e, c, s.e, s.A, s.B, jj, fresh);
+    if (jj < nt) process_tet<CARRY, COH>(b, prm, e, c, nhk, s.t, s.g, it.t0 + jj, fresh, prm.nh_on != 0);
+    if (jj < ne) process_edge<CARRY, COH>(b, prm, e, c, s.e, s.A, s.B, it.e0 + jj, fresh);
   };
   if constexpr (BLOCK <= 512) {
     Slot s0, s1, s2;
@@ -359,16 +372,16 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
     const bool nh = prm.nh_on != 0;
     if (nt > 0 && ne > 0) {
       if (nh)
-        project_wide<true, true, true, CARRY, COH>(b, prm, e, c, nhk, ps, T, nt, ne, jb, je, fresh);
+        project_wide<true, true, true, CARRY, COH>(b, prm, e, c, nhk, ps, it, T, nt, ne, jb, je, fresh);
       else
-        project_wide<true, true, false, CARRY, COH>(b, prm, e, c, nhk, ps, T, nt, ne, jb, je, fresh);
+        project_wide<true, true, false, CARRY, COH>(b, prm, e, c, nhk, ps, it, T, nt, ne, jb, je, fresh);
     } else if (nt > 0) {
       if (nh)
-        project_wide<true, false, true, CARRY, COH>(b, prm, e, c, nhk, ps, T, nt, ne, jb, je, fresh);
+        project_wide<true, false, true, CARRY, COH>(b, prm, e, c, nhk, ps, it, T, nt, ne, jb, je, fresh);
       else
-        project_wide<true, false, false, CARRY, COH>(b, prm, e, c, nhk, ps, T, nt, ne, jb, je, fresh);
+        project_wide<true, false, false, CARRY, COH>(b, prm, e, c, nhk, ps, it, T, nt, ne, jb, je, fresh);
     } else if (ne > 0) {
-      project_wide<false, true, false, CARRY, COH>(b, prm, e, c, nhk, ps, T, nt, ne, jb, je, fresh);
+      project_wide<false, true, false, CARRY, COH>(b, prm, e, c, nhk, ps, it, T, nt, ne, jb, je, fresh);
     }
   }
 }
@@ -475,8 +488,8 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
       // ---- projection: tets (volume + Neo-Hookean) and edges (distance) ----
       {
         const int nt = do_tets ? e.nt : 0, ne = do_edges ? e.ne : 0;
-        project<BLOCK, CARRY, false>(b, prm, e, c, nhk, PosGlobal<false>{P, vb}, T, nt, ne, 0, nt > ne ? nt : ne,
-                                     false);
+        project<BLOCK, CARRY, false>(b, prm, e, c, nhk, PosGlobal<false>{P, vb}, Items{b.tets16, b.edges16, 0, 0},
+                                     T, nt, ne, 0, nt > ne ? nt : ne, false);
       }
       __syncthreads();
       // ---- apply (+ grasp; + ground, integrate and next predict) ----
@@ -834,8 +847,8 @@ __device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, const S
     if (ch.kind == kProjectPass || ch.kind == kCollidePass) {
       if (ch.kind == kProjectPass) {
         const int jb = ch.local * kShareItems, je = jb + kShareItems < nmax ? jb + kShareItems : nmax;
-        project<BLOCK, CARRY, true>(b, prm, e, c, nhk, PosGlobal<true>{P, vb}, T, nt, ne, jb, je,
-                                    reset_lambda && ch.it == 0);
+        project<BLOCK, CARRY, true>(b, prm, e, c, nhk, PosGlobal<true>{P, vb}, Items{b.tets16, b.edges16, 0, 0}, T,
+                                    nt, ne, jb, je, reset_lambda && ch.it == 0);
       } else {
         const int f0 = ch.local * kShareTris, f1 = f0 + kShareTris < e.nf ? f0 + kShareTris : e.nf;
         collide_env<true>(b, prm, e, env, c, P, vb, &s_contacts, BLOCK, f0, f1);
@@ -1095,7 +1108,7 @@ __device__ void share_fold_env(const ffsrl_batch& b, const ffsrl_params& prm, co
         lds_u = pass;
       }
       const int jb = local * kShareItems, je = jb + kShareItems < nmax ? jb + kShareItems : nmax;
-      project<BLOCK, CARRY, true>(b, prm, e, c, nhk, PosLds{Pl}, T, nt, ne, jb, je,
+      project<BLOCK, CARRY, true>(b, prm, e, c, nhk, PosLds{Pl}, Items{b.tets16, b.edges16, 0, 0}, T, nt, ne, jb, je,
                                   reset_lambda && pass % I == 0);
       pend_base = base;
       pend_u = pass;
@@ -1194,13 +1207,33 @@ __global__ __launch_bounds__(BLOCK) void k_frame_shared(ffsrl_batch b, ffsrl_par
   if (threadIdx.x == 0) s_lastwg = atomicAdd(exits, 1u) == gridDim.x - 1;
   __syncthreads();
   if (!s_lastwg) return;
-  for (int el = 0; el < n; ++el) {
+  // one thread per env: report an incomplete frame and reset its counters,
+  // marking the frame (2 in its give-up word) for the clearing of its sums
+  // by the whole workgroup below, which only runs when a frame is incomplete
+  if (threadIdx.x == 0) s_incomplete = 0;
+  __syncthreads();
+  for (int el = threadIdx.x; el < n; el += BLOCK) {
     const int env = list ? list[el] : el;
-    const ffsrl_env_desc e = b.envs[env];
     uint32_t* ctr = ws.ctr_of(el, env);
-    if (threadIdx.x == 0) s_incomplete = atomicAdd(&ctr[1], 0u) != (uint32_t)share_plan(e, prm, COLLIDE).total;
-    __syncthreads();
-    if (s_incomplete) {
+    const bool inc = atomicAdd(&ctr[1], 0u) != (uint32_t)share_plan(b.envs[env], prm, COLLIDE).total;
+    if (inc) {
+      atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_INCOMPLETE);
+      s_incomplete = 1;
+    }
+    atomicExch(&ctr[0], 0u);
+    atomicExch(&ctr[1], 0u);
+    atomicExch(&ctr[2], inc ? 2u : 0u);
+  }
+  __syncthreads();
+  if (s_incomplete) {
+    for (int el = 0; el < n; ++el) {
+      const int env = list ? list[el] : el;
+      const ffsrl_env_desc e = b.envs[env];
+      uint32_t* ctr = ws.ctr_of(el, env);
+      __syncthreads();
+      if (threadIdx.x == 0) s_lastwg = atomicAdd(&ctr[2], 0u) == 2u;  // reused: this env's sums to clear
+      __syncthreads();
+      if (!s_lastwg) continue;
       if (FOLD && !COLLIDE && e.nv <= kFoldMaxVertices) {
         for (int k = 0; k < 3; ++k) {  // the three sum planes of the folded form
           unsigned long long* g = ws.fold_plane(k, e);
@@ -1210,16 +1243,339 @@ __global__ __launch_bounds__(BLOCK) void k_frame_shared(ffsrl_batch b, ffsrl_par
         unsigned long long* gacc = ws.acc_of(el, e);
         for (int i = threadIdx.x; i < 2 * e.nv; i += BLOCK) atomicExch(&gacc[i], 0ull);
       }
+      if (threadIdx.x == 0) atomicExch(&ctr[2], 0u);
     }
-    if (threadIdx.x == 0) {
-      if (s_incomplete) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_INCOMPLETE);
-      atomicExch(&ctr[0], 0u);
-      atomicExch(&ctr[1], 0u);
-      atomicExch(&ctr[2], 0u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) atomicExch(exits, 0u);
+}
+
+// ---------------------------------------------------------------------------
+// Partitioned frame (ffsrl_frame_parts): each env's constraints are cut into
+// parts, contiguous ranges of its tets and of its edges in their current
+// (Morton) order, so a part covers a compact region of the mesh; one
+// workgroup runs one part for the whole frame.  A part's vertices (those its
+// constraints touch, plus untouched vertices assigned to it) have part-local
+// ids; their positions, substep-start positions, grasp weights and Jacobi sums
+// stay in the workgroup's LDS for the frame, and the projection gathers from
+// LDS.  A vertex held by several parts ("shared") is advanced by each of them
+// from the same inputs with the same expressions, so the copies agree; its
+// sums are exchanged through global planes: after projecting pass u (u =
+// substep * iterations + iteration) a part adds its LDS sums of shared
+// vertices into plane u % 3, counts its arrival, waits for every part of the
+// env to arrive, and reads the complete sums back; a vertex of one part uses
+// its LDS sums directly.  Integer sums do not depend on the order of the
+// additions, so the frame equals the one-workgroup frame bit for bit.
+//
+// Plane reuse: the owner of a shared vertex clears its entry of plane
+// (u - 1) % 3 after pass u's arrivals (every part read that plane in pass
+// u - 1, before arriving at pass u); the plane is next added to in pass u + 2,
+// after pass u + 1's arrivals, which include the owner's (issued after its
+// clears completed).  One more arrival after the last pass lets the owners
+// clear the last plane, so every frame leaves the planes at zero.
+//
+// Scheduling: list slot el belongs to XCD el % nxcd, so all parts of an env
+// share one L2 (the coherence point, as in the shared frame), and the
+// workgroups of an XCD claim its parts in slot order from a counter.  A part
+// is claimed only by a running workgroup, after every part before it in the
+// order; so every env whose parts are all claimed completes, and its
+// workgroups then claim the next ones.  An env needs all its parts resident
+// at once: at most as many parts as the XCD has CUs (checked at launch), and
+// the launch should be the only persistent frame on the device (a partly
+// resident launch can leave an env's parts unclaimed until its waits give up,
+// which the frame reports as incomplete).  Waits are bounded as in the shared
+// frame; the last workgroup out checks, cleans up and resets every counter.
+struct PartsWs {
+  const int32_t* parts;      // [n_parts][8]
+  const int32_t* env_parts;  // int2 [n_env]
+  const int32_t* verts;      // per part, its vertices (mesh-local id | flags)
+  const uint64_t* t16;       // part-local packed tet ids
+  const uint32_t* e16;       // part-local packed edge ids
+  unsigned long long* sums;  // [3][stride][2]
+  int64_t stride;
+  uint32_t* ctr;             // [n_env][4]: parts finished, pass arrivals, give-up flag, 0; then XCD claim
+                             // counters [16], exits, and the claim table's first entry and size per XCD [16][2]
+  int32_t* claims;           // per XCD, its parts in claim order: list slot << 6 | part
+  int n_env;
+  __device__ unsigned long long* plane(int k, int64_t vb) const { return sums + 2 * ((int64_t)k * stride + vb); }
+  __device__ uint32_t* env_ctr(int env) const { return ctr + 4 * (int64_t)env; }
+  __device__ uint32_t* xcd_ctr(int x) const { return ctr + 4 * (int64_t)n_env + x; }
+  __device__ uint32_t* exits() const { return ctr + 4 * (int64_t)n_env + 16; }
+  __device__ uint32_t* claim_info(int x) const { return ctr + 4 * (int64_t)n_env + 32 + 2 * x; }
+};
+
+// The claim table of a launch (one workgroup, before the frame on the same
+// stream): XCD x's entries are the parts of its slots x, x + nxcd, ... in
+// order, from claim_info(x)[0], claim_info(x)[1] of them, so a workgroup's
+// k-th claim is one load.
+__global__ __launch_bounds__(1024) void k_parts_claims(const int32_t* list, int n, int nxcd, PartsWs ws) {
+  __shared__ int scan[1024];
+  __shared__ int s_base;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_base = 0;
+  __syncthreads();
+  for (int x = 0; x < nxcd; ++x) {
+    const int first = s_base;
+    const int cnt = x < n ? (n - 1 - x) / nxcd + 1 : 0;  // slots of XCD x
+    for (int j0 = 0; j0 < cnt; j0 += 1024) {
+      const int j = j0 + tid, el = x + nxcd * j;
+      const int np = j < cnt ? ws.env_parts[2 * (list ? list[el] : el) + 1] : 0;
+      scan[tid] = np;
+      __syncthreads();
+      for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
+        const int v = tid >= off ? scan[tid - off] : 0;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+      }
+      const int start = s_base + scan[tid] - np;
+      for (int q = 0; q < np; ++q) ws.claims[start + q] = (el << 6) | q;
+      __syncthreads();
+      if (tid == 0) s_base += scan[1023];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      ws.claim_info(x)[0] = (uint32_t)first;
+      ws.claim_info(x)[1] = (uint32_t)(s_base - first);
+    }
+  }
+}
+
+constexpr uint32_t kPartOwner = FFSRL_PART_OWNER, kPartShared = FFSRL_PART_SHARED;
+constexpr uint32_t kPartIdMask = kPartShared - 1u;
+// LDS per local vertex: sums 16 B, position 16 B, substep start 16 B, flags 4 B, grasp weight 4 B.
+constexpr int kPartLocalBytes = 56;
+constexpr int kPartMaxLocal = (kLdsBytes - 512) / kPartLocalBytes;
+
+template <bool CARRY>
+__device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const PartsWs& ws, int env, int pi, int np,
+                         int refit, unsigned long long* lds) {
+  constexpr int BLOCK = 1024;
+  __shared__ int s_ok, s_last;
+  const ffsrl_env_desc e = b.envs[env];
+  const int* pr = ws.parts + 8 * (int64_t)pi;
+  const int t0 = pr[0], t1 = pr[1], e0 = pr[2], e1 = pr[3], voff = pr[4], nloc = pr[5];
+  const int tid = threadIdx.x, I = prm.iterations, U = prm.substeps * I;
+  const int64_t vb = e.vbase;
+  FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
+  float4* Pl = reinterpret_cast<float4*>(lds + 2 * nloc);
+  float4* Xl = Pl + nloc;
+  uint32_t* Ml = reinterpret_cast<uint32_t*>(Xl + nloc);
+  float* Dl = reinterpret_cast<float*>(Ml + nloc);
+  VertexStep vs;
+  vs.g = mk(prm.gravity[0], prm.gravity[1], prm.gravity[2]);
+  vs.dt = prm.dt;
+  vs.damp1 = 1.0f - prm.damping;
+  vs.inv_dt = 1.0f / prm.dt;
+  vs.ground = prm.ground;
+  vs.ks = prm.drag_ks;
+  const NHConst nhk = nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt);
+  const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
+  const int64_t T = b.n_tet_total;
+  const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
+  const int nt = (prm.vol_mode != FFSRL_OFF || prm.nh_on) ? t1 - t0 : 0;
+  const int ne = prm.dist_mode != FFSRL_OFF ? e1 - e0 : 0;
+  const int nmax = nt > ne ? nt : ne;
+  uint32_t* ctr = ws.env_ctr(env);
+  const bool solo = np == 1;  // a whole env in one part: no vertex is shared, no pass waits
+
+  // the part's vertices: flags, state and the predict of substep 0
+  for (int i = tid; i < nloc; i += BLOCK) {
+    const uint32_t m = (uint32_t)ws.verts[voff + i];
+    const int64_t g = vb + (m & kPartIdMask);
+    const float4 x = ld4(b.x, g), v = ld4(b.v, g);
+    Ml[i] = m;
+    Xl[i] = x;
+    Dl[i] = b.drag[g];
+    Pl[i] = with_w(predict(xyz(x), xyz(v), x.w, vs.g, vs.dt, vs.damp1), x.w);
+    lds[2 * i] = 0ull;
+    lds[2 * i + 1] = 0ull;
+  }
+  if (tid == 0) sat_flag() = 0;
+  __syncthreads();
+  bool ok = true;
+  for (int u = 0; u < U; ++u) {
+    const int s = u / I, itn = u - s * I;
+    project<BLOCK, CARRY, false>(b, prm, e, c, nhk, PosLds{Pl}, Items{ws.t16, ws.e16, t0, e0}, T, nt, ne, 0, nmax,
+                                 reset_lambda && itn == 0);
+    __syncthreads();
+    // shared vertices: this part's sums into the pass's plane, then arrive and wait for the env's parts
+    unsigned long long* gs = ws.plane(u % 3, vb);
+    if (!solo) {
+      for (int i = tid; i < nloc; i += BLOCK) {
+      const uint32_t m = Ml[i];
+        if (m & kPartShared) {
+          const int g = (int)(m & kPartIdMask);
+          const unsigned long long w0 = lds[2 * i], w1 = lds[2 * i + 1];
+          if (w0) atomicAdd(&gs[2 * g], w0);
+          if (w1) atomicAdd(&gs[2 * g + 1], w1);
+        }
+      }
+      wait_vmem();
+      __syncthreads();
+      if (tid == 0) {
+        atomicAdd(&ctr[1], 1u);
+        s_ok = share_wait(ctr, (uint32_t)(np * (u + 1)));
+      }
+      __syncthreads();
+      if (!s_ok) {
+        ok = false;
+        break;
+      }
+    }
+    // apply (+ grasp; at the end of a substep the ground clamp, integrate and next predict), as
+    // the one-workgroup frame; the owner clears its shared vertices' entries of plane (u - 1) % 3
+    const bool last_it = itn == I - 1, next_sub = s + 1 < prm.substeps;
+    unsigned long long* old = u >= 1 ? ws.plane((u + 2) % 3, vb) : nullptr;
+    for (int i = tid; i < nloc; i += BLOCK) {
+      const uint32_t m = Ml[i];
+      unsigned long long w0 = lds[2 * i], w1 = lds[2 * i + 1];
+      lds[2 * i] = 0ull;
+      lds[2 * i + 1] = 0ull;
+      if (m & kPartShared) {
+        const int g = (int)(m & kPartIdMask);
+        w0 = ldu64c(gs, 2 * g);
+        w1 = ldu64c(gs, 2 * g + 1);
+        if (old && (m & kPartOwner)) {
+          old[2 * g] = 0ull;
+          old[2 * g + 1] = 0ull;
+        }
+      }
+      int32_t fx, fy, fz, n;
+      unpack2(w0, &fx, &fy);
+      unpack2(w1, &fz, &n);
+      n = sum_count(n);
+      const float4 p4 = Pl[i];
+      f3 p = xyz(p4);
+      if (n > 0) {
+        const float sc = c.q * __builtin_amdgcn_rcpf((float)n);
+        p = p + mk((float)fx, (float)fy, (float)fz) * sc;
+      }
+      const float dr = Dl[i];
+      if (dr != 0.0f) p = grasp_pull<true>(p, tip, dr, vs.ks);
+      if (last_it) {
+        if (p.y < vs.ground) p.y = vs.ground;
+        const float4 x4 = Xl[i];
+        const f3 v = (p - xyz(x4)) * vs.inv_dt;
+        Xl[i] = with_w(p, x4.w);
+        if (next_sub)
+          p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
+        else if (m & kPartOwner)
+          st4(b.v, vb + (m & kPartIdMask), with_w(v, 0.0f));
+      }
+      Pl[i] = with_w(p, p4.w);
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicExch(exits, 0u);
+  if (ok) {
+    // the model's state of the vertices this part owns, then one more arrival
+    for (int i = tid; i < nloc; i += BLOCK) {
+      const uint32_t m = Ml[i];
+      if (m & kPartOwner) {
+        const int64_t g = vb + (m & kPartIdMask);
+        st4(b.x, g, Xl[i]);
+        st4(b.p, g, Pl[i]);
+      }
+    }
+    wait_vmem();
+    __syncthreads();
+    if (!solo) {
+      if (tid == 0) {
+        atomicAdd(&ctr[1], 1u);
+        s_ok = share_wait(ctr, (uint32_t)(np * (U + 1)));
+      }
+      __syncthreads();
+      ok = s_ok;
+    }
+  }
+  if (ok) {
+    // every part has read the last plane: the owners clear it
+    unsigned long long* last = ws.plane((U - 1) % 3, vb);
+    for (int i = tid; i < nloc; i += BLOCK) {
+      const uint32_t m = Ml[i];
+      if ((m & kPartShared) && (m & kPartOwner)) {
+        const int g = (int)(m & kPartIdMask);
+        last[2 * g] = 0ull;
+        last[2 * g + 1] = 0ull;
+      }
+    }
+    wait_vmem();
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(&ctr[0], 1u) + 1u == (uint32_t)np;
+    __syncthreads();
+    if (refit && s_last) refit_env<BLOCK, true>(b, e);  // every part's positions are stored
+  } else if (tid == 0) {
+    atomicOr(&ctr[2], 1u);  // the env's other waits give up too
+  }
+  __syncthreads();
+  if (tid == 0 && sat_flag()) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_SATURATED);
+}
+
+template <bool CARRY>
+__global__ __launch_bounds__(1024) void k_frame_parts(ffsrl_batch b, ffsrl_params prm, const int32_t* list, int n,
+                                                      int refit, int nxcd, XcdMap map, PartsWs ws) {
+  extern __shared__ __align__(16) unsigned long long lds[];
+  __shared__ int s_claim, s_lastwg, s_incomplete;
+  const int x = map.index[xcc_id() & 15];
+  if (x >= 0 && x < nxcd) {
+    const uint32_t c0 = ws.claim_info(x)[0], cn = ws.claim_info(x)[1];
+    for (;;) {
+      if (threadIdx.x == 0) s_claim = (int)atomicAdd(ws.xcd_ctr(x), 1u);
+      __syncthreads();
+      const uint32_t k = (uint32_t)__builtin_amdgcn_readfirstlane(s_claim);
+      __syncthreads();  // s_claim is written again by the next claim
+      if (k >= cn) break;
+      // the k-th part of this XCD's slots x, x + nxcd, ... in order
+      const int entry = ws.claims[c0 + k], el = entry >> 6, part = entry & 63;
+      const int env = list ? list[el] : el;
+      part_env<CARRY>(b, prm, ws, env, ws.env_parts[2 * env] + part, ws.env_parts[2 * env + 1], refit, lds);
+      __syncthreads();  // LDS is reused by the next part
+    }
+  }
+  // The last workgroup out checks that every env's frame completed, clears
+  // the planes of any that did not (a part that gave up leaves partial
+  // sums), reports it, and resets the counters for the next launch.
+  __syncthreads();
+  if (threadIdx.x == 0) s_lastwg = atomicAdd(ws.exits(), 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_lastwg) return;
+  // one thread per env, then the planes of incomplete frames (marked 2 in
+  // the give-up word) by the whole workgroup, as in the shared frame
+  if (threadIdx.x == 0) s_incomplete = 0;
+  __syncthreads();
+  for (int el = threadIdx.x; el < n; el += 1024) {
+    const int env = list ? list[el] : el;
+    uint32_t* ctr = ws.env_ctr(env);
+    const bool inc =
+        atomicAdd(&ctr[0], 0u) != (uint32_t)ws.env_parts[2 * env + 1] || atomicAdd(&ctr[2], 0u) != 0u;
+    if (inc) {
+      atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_INCOMPLETE);
+      s_incomplete = 1;
+    }
+    atomicExch(&ctr[0], 0u);
+    atomicExch(&ctr[1], 0u);
+    atomicExch(&ctr[2], inc ? 2u : 0u);
+  }
+  __syncthreads();
+  if (s_incomplete) {
+    for (int el = 0; el < n; ++el) {
+      const int env = list ? list[el] : el;
+      const ffsrl_env_desc e = b.envs[env];
+      uint32_t* ctr = ws.env_ctr(env);
+      __syncthreads();
+      if (threadIdx.x == 0) s_lastwg = atomicAdd(&ctr[2], 0u) == 2u;  // reused: this env's planes to clear
+      __syncthreads();
+      if (!s_lastwg) continue;
+      for (int k = 0; k < 3; ++k) {
+        unsigned long long* g = ws.plane(k, e.vbase);
+        for (int i = threadIdx.x; i < 2 * e.nv; i += 1024) atomicExch(&g[i], 0ull);
+      }
+      if (threadIdx.x == 0) atomicExch(&ctr[2], 0u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) atomicExch(ws.xcd_ctr(threadIdx.x), 0u);
+  if (threadIdx.x == 0) atomicExch(ws.exits(), 0u);
 }
 
 __global__ void k_xcc_probe(int32_t* out) {
@@ -1751,6 +2107,73 @@ int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int3
     case 1024: return launch_shared<1024>(*b, *prm, envs, n, refit, carry, lds, nxcd, map, ws, st);
     default: return fail(FFSRL_E_ARG, "block must be 256, 512 or 1024 (got %d)", block);
   }
+}
+
+int ffsrl_parts_max_local(void) { return kPartMaxLocal; }
+
+int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl_parts* pt, const int32_t* envs,
+                      int32_t n, int32_t refit, void* stream) {
+  if (!prm) return fail(FFSRL_E_ARG, "params NULL");
+  int rc = check_batch(b, prm);
+  if (rc) return rc;
+  if (!pt) return fail(FFSRL_E_ARG, "parts NULL");
+  if (prm->collide) return fail(FFSRL_E_ARG, "the partitioned frame has no collision pass");
+  const bool carry = carries_lambda(*prm);
+  if (carry && ((prm->dist_mode == FFSRL_XPBD && !b->lam_e) || (prm->vol_mode == FFSRL_XPBD && !b->lam_t)))
+    return fail(FFSRL_E_ARG, "XPBD with iterations > 1 needs lambda arrays");
+  if (refit && (!b->tris || !b->level_nodes || !b->level_info || !b->level_off || !b->bvh_lo || !b->bvh_hi))
+    return fail(FFSRL_E_ARG, "refit requested without BVH arrays");
+  if ((prm->vol_mode || prm->nh_on) && !pt->tets16) return fail(FFSRL_E_ARG, "partitioned frame needs part tet ids");
+  if (prm->dist_mode && !pt->edges16) return fail(FFSRL_E_ARG, "partitioned frame needs part edge ids");
+  if (!pt->parts || !pt->env_parts || !pt->verts || !pt->sums || !pt->ctr || !pt->claims || !pt->host_env_parts)
+    return fail(FFSRL_E_ARG, "parts: NULL table, sums, counters or claims");
+  if (pt->max_local < 1 || pt->max_local > kPartMaxLocal)
+    return fail(FFSRL_E_LIMIT, "a part of %d local vertices (LDS holds %d)", pt->max_local, kPartMaxLocal);
+  if (n < 0 || n > b->n_env) return fail(FFSRL_E_ARG, "%d environments listed, batch has %d", n, b->n_env);
+  if (!b->status) return fail(FFSRL_E_ARG, "the batched frames need the per-env status array");
+  if (n == 0) return 0;
+  const int nxcd = ffsrl_xcd_count();
+  if (nxcd <= 0) return nxcd < 0 ? nxcd : fail(FFSRL_E_LIMIT, "no XCD found");
+  int dev = 0, cus = 0, per_cu = 0;
+  FFSRL_HIP(hipGetDevice(&dev));
+  FFSRL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  XcdMap map;
+  {
+    std::lock_guard<std::mutex> lock(g_xcd_mu);
+    map = g_xcd_map[dev];
+  }
+  const int per_xcd = std::min(cus / nxcd, 63);
+  int64_t total = 0;
+  for (int i = 0; i < b->n_env; ++i) {
+    const int np = pt->host_env_parts[2 * i + 1];
+    total += np;
+    if (np < 1 || np > per_xcd)
+      return fail(FFSRL_E_LIMIT, "env %d has %d parts; they must run at once on one XCD (%d CUs)", i, np, per_xcd);
+    if (b->host_envs) {
+      const ffsrl_env_desc& h = b->host_envs[i];
+      if (h.max_terms > FFSRL_MAX_TERMS)
+        return fail(FFSRL_E_LIMIT, "env %d: a vertex receives %d Jacobi terms in one pass (limit %d)", i,
+                    h.max_terms, FFSRL_MAX_TERMS);
+      if ((int64_t)h.vbase + h.nv > pt->stride)
+        return fail(FFSRL_E_ARG, "parts stride %lld < vbase + nv of env %d", (long long)pt->stride, i);
+    }
+  }
+  if (total > pt->n_claims) return fail(FFSRL_E_ARG, "parts: %lld claim entries for %lld parts", (long long)pt->n_claims,
+                                        (long long)total);
+  if (n >= (1 << 25)) return fail(FFSRL_E_LIMIT, "%d environments listed (claim entries hold 2^25)", n);
+  const int lds = kPartLocalBytes * pt->max_local;
+  auto kern = carry ? k_frame_parts<true> : k_frame_parts<false>;
+  FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  FFSRL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 1024, lds));
+  const int grid = std::max(nxcd, cus * std::max(1, per_cu));
+  const PartsWs ws{pt->parts,  pt->env_parts, pt->verts, pt->tets16, pt->edges16,
+                   reinterpret_cast<unsigned long long*>(pt->sums), pt->stride, pt->ctr, pt->claims, b->n_env};
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_parts_claims, dim3(1), dim3(1024), 0, st, envs, n, nxcd, ws);
+  FFSRL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(1024), lds, st, *b, *prm, envs, n, refit, nxcd, map, ws);
+  FFSRL_HIP(hipGetLastError());
+  return 0;
 }
 
 int ffsrl_predict(const ffsrl_batch* b, const ffsrl_params* prm, int32_t eb, int32_t ee, void* stream) {

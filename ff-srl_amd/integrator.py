@@ -12,6 +12,11 @@ Three launch forms run the same arithmetic:
   CUs sharing the environments (each env's frame cut into chunks that the
   workgroups of one XCD take in turn), for batches with fewer environments
   than CUs; bit-identical to the batched form;
+* partitioned (``parts=True``): ``ffsrl_frame_parts``, each env's constraints
+  cut into parts with one workgroup per part for the whole frame (state in
+  LDS, only shared vertices exchanged per pass); bit-identical to the
+  batched form, and it also runs envs above the batched frame's vertex
+  limit;
 * plain: the reference's launch model with one launch per kernel per
   environment (``integrator.py:1779-1929`` is the reference's per-env
   precedent), used as the comparison baseline.
@@ -25,17 +30,20 @@ from .model import SimModelDO
 
 
 class SimIntegratorDO:
-    def __init__(self, device="cuda:0", block: int = 1024, dispatch_sorted: bool = True, shared: bool = False):
+    def __init__(self, device="cuda:0", block: int = 1024, dispatch_sorted: bool = True, shared: bool = False,
+                 parts: bool = False):
         self.device = device
         self.block = block
         self.dispatch_sorted = dispatch_sorted
         self.shared = shared
+        self.parts = parts
 
     def configure(self, cfg: dict) -> None:
         """Launch shape from the autotuner: {block, dispatch ("lpt"/"table"), shared, ...}."""
         self.block = int(cfg["block"])
         self.dispatch_sorted = cfg["dispatch"] == "lpt"
         self.shared = bool(cfg.get("shared", False))
+        self.parts = bool(cfg.get("parts", False))
 
     def autotune(self, simModel: SimModelDO, cache_path=None, frames: int = 2) -> dict:
         """Tune (or look up) the launch shape for this batch and use it."""
@@ -46,10 +54,13 @@ class SimIntegratorDO:
 
     def stepModel(self, simModel: SimModelDO, refit: bool = False) -> None:
         """One frame of all environments in one launch (``ffsrl_frame``, or
-        ``ffsrl_frame_shared`` when the integrator is shared)."""
+        ``ffsrl_frame_shared`` / ``ffsrl_frame_parts`` when the integrator is
+        shared / partitioned)."""
         m = simModel
         order = m.dispatch_order().data_ptr() if self.dispatch_sorted else None
-        if self.shared:
+        if self.parts:
+            lib.ffsrl_frame_parts(m.batch, m.params, m.parts, order, m.numEnvs, int(bool(refit)), m.stream())
+        elif self.shared:
             m.enable_shared()
             lib.ffsrl_frame_shared(m.batch, m.params, order, m.numEnvs, self.block, int(bool(refit)), m.stream())
         else:

@@ -25,8 +25,8 @@ from typing import Optional, Sequence
 import numpy as np
 import torch
 
-from ._lib import (Batch, BvhJob, EnvDesc, ENV_DESC_FIELDS, FFSRLError, Instrument, OFF, PBD, XPBD, Params,
-                   STATUS_INCOMPLETE, STATUS_SATURATED, lib)
+from ._lib import (Batch, BvhJob, EnvDesc, ENV_DESC_FIELDS, FFSRLError, Instrument, OFF, PART_OWNER, PART_SHARED,
+                   PBD, XPBD, Params, Parts, STATUS_INCOMPLETE, STATUS_SATURATED, lib)
 from .scenes import TetMesh
 
 MAX_STACK = 32  # traversal stack entries per ray (render.py:424, 924)
@@ -56,6 +56,37 @@ def _parallel(fn, items):
     with concurrent.futures.ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
         return list(ex.map(fn, items))
 LOCAL_CHUNK = 128  # constraints per colour-sorted run of the "local" order
+PART_TETS = 5120   # constraints per part of the partitioned frame (tets, or edges without tet constraints)
+PART_MAX = 32      # most parts per mesh (one XCD's CUs run an env's parts at once)
+
+
+def partition_mesh(tets: np.ndarray, edges: np.ndarray, nv: int, parts: int, use_tets: bool = True,
+                   use_edges: bool = True) -> dict:
+    """Cuts a mesh's constraints (in their current order) into ``parts``
+    ranges of tets and of edges for the partitioned frame
+    (``ffsrl_partition_mesh``).  Returns the part rows (tet begin / end, edge
+    begin / end, first vertex entry, vertex count, 0, 0; vertex entries
+    relative to this mesh), each part's vertices (mesh-local id |
+    PART_SHARED when several parts hold it | PART_OWNER on the lowest part
+    holding it), the part-local ids of every tet and edge (packed as the
+    frame reads them: 4 x 16 bits, 2 x 16 bits), and the largest part.  Only the constraint families in use decide a part's
+    vertices; a vertex that no constraint in use touches goes to part
+    i mod parts (i its rank among such vertices)."""
+    P = int(parts)
+    tets = np.ascontiguousarray(np.asarray(tets).reshape(-1, 4), dtype=np.int32)
+    edges = np.ascontiguousarray(np.asarray(edges).reshape(-1, 2), dtype=np.int32)
+    nt, ne = tets.shape[0], edges.shape[0]
+    rows = np.zeros((P, 8), np.int32)
+    cap = 4 * nt + 2 * ne + nv
+    verts = np.zeros(max(cap, 1), np.int32)
+    t16 = np.zeros(max(nt, 1), np.uint64)
+    e16 = np.zeros(max(ne, 1), np.uint32)
+    n_verts, most = C.c_int64(0), C.c_int32(0)
+    lib.ffsrl_partition_mesh(tets.ctypes.data, nt, edges.ctypes.data, ne, int(nv), P, int(bool(use_tets)),
+                             int(bool(use_edges)), rows.ctypes.data, verts.ctypes.data, cap, C.byref(n_verts),
+                             t16.ctypes.data, e16.ctypes.data, C.byref(most))
+    return {"rows": rows.astype(np.int64), "verts": verts[:n_verts.value], "tets16": t16[:nt].view(np.int64),
+            "edges16": e16[:ne].view(np.int32), "max_local": int(most.value), "parts": P}
 
 _MODES = {None: OFF, "off": OFF, "pbd": PBD, "xpbd": XPBD}
 
@@ -824,6 +855,78 @@ class SimModelDO:
         self._batch.fold_ws = _ptr(self.fold_ws)
         self._batch.fold_stride = self.numVertices if self.fold_ws is not None else 0
 
+    def enable_parts(self, tets_per_part: int = PART_TETS, parts: Optional[Sequence[int]] = None) -> Parts:
+        """Partition every mesh's constraints for the partitioned frame
+        (``ffsrl_frame_parts``) and allocate its sums and counters (zero, and
+        left at zero by every frame).  A mesh gets ceil(tets / tets_per_part)
+        parts (edges when the scheme has no tet constraints), more while a
+        part has more vertices than one workgroup's LDS holds; ``parts``
+        gives the count per mesh instead.  The partition follows the current
+        constraint order: ``set_constraint_order`` rebuilds it."""
+        use_t = self.vol_mode != OFF or self.nh_on
+        use_e = self.dist_mode != OFF
+        cap = int(lib.ffsrl_parts_max_local())
+
+        def build(k):
+            t = self.topo[k]
+            m = t.mesh
+            items = m.num_tets if use_t else m.num_edges
+            P = int(parts[k]) if parts is not None else max(1, -(-items // int(tets_per_part)))
+            # a part's vertices grow with its share of the mesh: start near the LDS bound
+            if parts is None:
+                P = max(P, min(PART_MAX, -(-m.num_vertices * 5 // (4 * cap))))
+            while True:
+                r = partition_mesh(t.tets, t.edges, m.num_vertices, P, use_t, use_e)
+                if r["max_local"] <= cap or parts is not None or P >= PART_MAX:
+                    break
+                P += 1
+            if r["max_local"] > cap:
+                raise ValueError(f"mesh {k}: a part of {r['max_local']} vertices exceeds the {cap} that one "
+                                 f"workgroup holds ({P} parts)")
+            return r
+
+        built = _parallel(build, range(len(self.topo)))
+        first = np.cumsum([0] + [r["parts"] for r in built])
+        voff = np.cumsum([0] + [r["verts"].size for r in built])
+        rows = np.concatenate([r["rows"] for r in built])
+        rows[:, 4] += np.repeat(voff[:-1], [r["parts"] for r in built])
+        env_parts = np.stack([first[self.env_mesh], first[self.env_mesh + 1] - first[self.env_mesh]], axis=1)
+        self._parts_host = np.ascontiguousarray(env_parts.astype(np.int32))
+        dev = self.device
+        cat = np.concatenate
+        self.p_rows = torch.from_numpy(rows.astype(np.int32)).to(dev)
+        self.p_env = torch.from_numpy(self._parts_host).to(dev)
+        self.p_verts = torch.from_numpy(cat([r["verts"] for r in built])).to(dev)
+        self.p_tets16 = torch.from_numpy(cat([r["tets16"] for r in built])).to(dev)
+        self.p_edges16 = torch.from_numpy(cat([r["edges16"] for r in built])).to(dev)
+        if getattr(self, "p_sums", None) is None:
+            self.p_sums = torch.zeros((3, self.numVertices, 2), dtype=torch.int64, device=dev)
+            self.p_ctr = torch.zeros(4 * self.numEnvs + 64, dtype=torch.int32, device=dev)
+            if lib.ffsrl_xcd_count() <= 0:
+                raise RuntimeError(f"ffsrl_xcd_count: {lib.ffsrl_last_error().decode()}")
+        pt = Parts()
+        pt.n_parts = int(rows.shape[0])
+        pt.max_local = max(r["max_local"] for r in built)
+        pt.parts, pt.env_parts, pt.verts = _ptr(self.p_rows), _ptr(self.p_env), _ptr(self.p_verts)
+        pt.tets16, pt.edges16 = _ptr(self.p_tets16), _ptr(self.p_edges16)
+        pt.sums, pt.stride, pt.ctr = _ptr(self.p_sums), self.numVertices, _ptr(self.p_ctr)
+        n_claims = int(self._parts_host[:, 1].sum())
+        if getattr(self, "p_claims", None) is None or self.p_claims.numel() < n_claims:
+            self.p_claims = torch.zeros(n_claims, dtype=torch.int32, device=dev)
+        pt.claims, pt.n_claims = _ptr(self.p_claims), int(self.p_claims.numel())
+        pt.host_env_parts = self._parts_host.ctypes.data
+        self._parts = pt
+        self._parts_cfg = dict(tets_per_part=tets_per_part, parts=parts)
+        self.parts_per_mesh = [r["parts"] for r in built]
+        return pt
+
+    @property
+    def parts(self) -> Parts:
+        """The partition of ``enable_parts`` (built with its defaults on first use)."""
+        if getattr(self, "_parts", None) is None:
+            self.enable_parts()
+        return self._parts
+
     def frame_status(self) -> torch.Tensor:
         """Per-env status bits of the batched and shared frames, sticky until
         ``clear_status``: STATUS_SATURATED (a fixed-point Jacobi term left
@@ -884,6 +987,8 @@ class SimModelDO:
                     src = (base + torch.arange(o.size, device=dev)[None]).reshape(-1)
                     lam[(base + dst[None]).reshape(-1)] = lam[src].clone()
         self.constraint_order = constraint_order
+        if getattr(self, "_parts", None) is not None:
+            self.enable_parts(**self._parts_cfg)   # the partition follows the constraint order
 
     def dispatch_order(self) -> torch.Tensor:
         """Environments largest first: the hardware dispatcher then assigns

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define FFSRL_ABI_VERSION 12
+#define FFSRL_ABI_VERSION 13
 
 #define FFSRL_E_ARG (-1)      /* invalid argument */
 #define FFSRL_E_LIMIT (-2)    /* a size exceeds a documented kernel limit */
@@ -258,6 +258,54 @@ int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int3
 int ffsrl_frame_shared_noalloc(const ffsrl_batch* b, const ffsrl_params* prm, const int32_t* envs, int32_t n,
                                int32_t block, int32_t refit, void* stream);
 
+/* Partition of each environment's constraints for ffsrl_frame_parts (built
+ * on the host from the current constraint order; rebuild it when that order
+ * changes).  A part is a range of its mesh's tets and a range of its edges;
+ * its vertices are numbered locally. */
+#define FFSRL_PART_OWNER 0x80000000u   /* verts entry: this part writes the vertex back */
+#define FFSRL_PART_SHARED 0x40000000u  /* verts entry: other parts hold the vertex too */
+typedef struct ffsrl_parts {
+  int32_t n_parts;
+  int32_t max_local;              /* most local vertices of a part (sizes LDS); at most
+                                     ffsrl_parts_max_local() */
+  const int32_t* parts;           /* int [n_parts][8]: tet begin, tet end, edge begin, edge end (within
+                                     the mesh's tets / edges, current order), first verts entry, local
+                                     vertex count, 0, 0 */
+  const int32_t* env_parts;       /* int2 [n_env]: first part and part count of each env (its mesh's) */
+  const int32_t* verts;           /* per part, its local vertices in local-id order: mesh-local id,
+                                     | FFSRL_PART_SHARED, | FFSRL_PART_OWNER on exactly one part */
+  const uint64_t* tets16;         /* [T] each tet's ids local to its part, 4 x 16 bits (a low) */
+  const uint32_t* edges16;        /* [E] each edge's ids local to its part, 2 x 16 bits */
+  uint64_t* sums;                 /* [3][stride][2] fixed-point sums of shared vertices, by state
+                                     vertex; zero before the first frame, left zero by every frame */
+  int64_t stride;                 /* vertices per plane of sums (>= every env's vbase + nv) */
+  uint32_t* ctr;                  /* [4 n_env + 64]: counters in the first 4 n_env + 17 words (zero
+                                     before the first frame, left zero by every frame), then the
+                                     claim table's extent per XCD (written by every launch) */
+  int32_t* claims;                /* [n_claims] workspace: each launch's claim table */
+  int64_t n_claims;               /* >= the total part count of all envs */
+  const int32_t* host_env_parts;  /* host mirror of env_parts (launch checks), required */
+} ffsrl_parts;
+
+/* ffsrl_frame_envs with each listed environment cut into parts, one
+ * workgroup (1024 threads) per part for the whole frame: positions and sums
+ * of a part's vertices stay in LDS, and only vertices shared between parts
+ * exchange sums through global memory, once per pass.  Results equal
+ * ffsrl_frame_envs bit for bit (integer Jacobi sums); envs larger than the
+ * batched frame's ffsrl_frame_max_vertices() run too, when their parts fit
+ * (ffsrl_parts_max_local()).  No collision pass (FFSRL_E_ARG).  All parts of
+ * an env run on one XCD at once: at most CUs / XCDs (and 63) parts per env
+ * (FFSRL_E_LIMIT).  Two launches: the claim table, then the frame.  The launch should be the only persistent frame on the
+ * device: waits are bounded, and a frame that could not complete sets
+ * FFSRL_STATUS_INCOMPLETE (its sums are cleared and its counters reset).
+ * envs NULL: environments 0 .. n-1.  Same reference counterpart as
+ * ffsrl_frame. */
+int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl_parts* parts,
+                      const int32_t* envs, int32_t n, int32_t refit, void* stream);
+
+/* Most local vertices of one part of ffsrl_frame_parts (56 B of LDS each). */
+int ffsrl_parts_max_local(void);
+
 /* Number of XCDs of the current device (probed once with one workgroup per
  * CU; the shared frame assigns environments to XCDs).  Allocates on its
  * first call per device, so call it before capturing a shared frame into a
@@ -478,6 +526,23 @@ int ffsrl_color_constraints(const int32_t* elems, int64_t n, int32_t arity, int3
  * make the result independent of this order.  No reference counterpart. */
 int ffsrl_order_constraints(const int32_t* elems, int64_t n, int32_t arity, const float* verts, int32_t nv,
                             int32_t chunk, int64_t* perm);
+
+/* Partition of one mesh for ffsrl_frame_parts (host memory): part p takes
+ * tets [p nt / parts, (p+1) nt / parts) and edges [p ne / parts, (p+1) ne /
+ * parts) of the current order (tets / edges: 4 / 2 mesh-local ids each).
+ * Its vertices (local ids 0 ..) are those its constraints in use touch, in
+ * increasing mesh id, plus the untouched vertices dealt round-robin in
+ * increasing id; verts gets each part's vertices (mesh-local id |
+ * FFSRL_PART_SHARED when several parts hold it | FFSRL_PART_OWNER on the
+ * lowest part holding it), rows the parts' [8] rows with vertex entries
+ * relative to this mesh, tets16 / edges16 each constraint's local ids
+ * (zero for a family not in use), *n_verts the entries written (verts_cap
+ * >= 4 nt + 2 ne + nv always suffices) and *max_local the largest part.
+ * No reference counterpart (set-up of this library's partitioned frame). */
+int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, int64_t ne, int32_t nv,
+                         int32_t parts, int32_t use_tets, int32_t use_edges, int32_t* rows, int32_t* verts,
+                         int64_t verts_cap, int64_t* n_verts, uint64_t* tets16, uint32_t* edges16,
+                         int32_t* max_local);
 
 /* ---- HIP graphs --------------------------------------------------------- */
 /* Capture everything issued on `stream` between begin and end into a graph

@@ -33,7 +33,7 @@ def test_every_declared_symbol_is_exported(pkg):
 
 
 def test_abi_version(pkg):
-    assert pkg.lib.ffsrl_abi_version() == 12
+    assert pkg.lib.ffsrl_abi_version() == 13
     assert pkg.lib.ffsrl_frame_max_vertices() >= 10000
 
 
@@ -50,6 +50,7 @@ int main(void) {
   printf("ffsrl_requests size %zu\n", sizeof(ffsrl_requests));
   printf("ffsrl_instrument size %zu\n", sizeof(ffsrl_instrument));
   printf("ffsrl_bvh_job size %zu\n", sizeof(ffsrl_bvh_job));
+  printf("ffsrl_parts size %zu\n", sizeof(ffsrl_parts));
   %FIELDS%
   return 0;
 }
@@ -60,7 +61,7 @@ def test_struct_layouts_match_header(pkg, tmp_path):
     L = pkg._lib
     structs = {"ffsrl_batch": L.Batch, "ffsrl_params": L.Params, "ffsrl_camera": L.Camera,
                "ffsrl_env_desc": L.EnvDesc, "ffsrl_requests": L.Requests, "ffsrl_instrument": L.Instrument,
-               "ffsrl_bvh_job": L.BvhJob}
+               "ffsrl_bvh_job": L.BvhJob, "ffsrl_parts": L.Parts}
     fields = "\n".join(f"F({n}, {f[0]})" for n, S in structs.items() for f in S._fields_)
     src = tmp_path / "layout.c"
     src.write_text(_LAYOUT_C.replace("%FIELDS%", fields))
@@ -140,6 +141,26 @@ def test_argument_errors_are_reported(pkg):
     for t in (0, 1, 2):
         cam.traversal = t
         assert pkg.lib.ffsrl_render(b, cam, 0, 0, dummy.ctypes.data, None, None, None, None, None, None, None) == 0
+    # the partitioned frame checks its tables before any device call
+    pt = L.Parts()
+    with pytest.raises(pkg.FFSRLError, match="parts NULL"):
+        pkg.lib.ffsrl_frame_parts(b, prm, None, None, 0, 0, None)
+    with pytest.raises(pkg.FFSRLError, match="NULL table"):
+        pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None)
+    for f in ("parts", "env_parts", "verts", "sums", "ctr", "claims", "host_env_parts"):
+        setattr(pt, f, dummy.ctypes.data)
+    pt.max_local = pkg.lib.ffsrl_parts_max_local() + 1
+    with pytest.raises(pkg.FFSRLError, match="local vertices"):
+        pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None)
+    pt.max_local = 100
+    prm.collide = 1
+    with pytest.raises(pkg.FFSRLError, match="no collision pass"):
+        pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None)
+    prm.collide = 0
+    with pytest.raises(pkg.FFSRLError, match="5 environments listed"):
+        pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 5, 0, None)
+    assert pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None) == 0      # nothing listed: no launch
+    assert 2000 <= pkg.lib.ffsrl_parts_max_local() < 3000
 
 
 def test_frame_rejects_too_many_terms_per_vertex(pkg):
