@@ -99,7 +99,10 @@ def parse():
                     help="the shared frame's folded form for envs of at most 5,104 vertices (same results)")
     ap.add_argument("--parts", choices=["on", "off"], default="off",
                     help="partitioned frame: each env cut into parts, one workgroup per part (same results)")
-    ap.add_argument("--tets-per-part", type=int, default=5120, help="constraints per part of the partitioned frame")
+    ap.add_argument("--tets-per-part", type=int, default=5120,
+                    help="constraints per part of the partitioned frame (1073741824: as few parts as LDS allows)")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="colour-run length of the constraint order with --no-autotune (0: the model's default)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --block and largest-first dispatch instead of the autotuned launch shape")
     ap.add_argument("--plain-frames", type=int, default=2)
@@ -377,6 +380,8 @@ def main():
         log(f"autotuned {tuned} in {time.perf_counter() - t_tune:.1f} s")
     if args.shared != "auto":
         integ.shared = args.shared == "on"
+    if args.chunk and args.no_autotune:
+        model.set_constraint_order("local", args.chunk)
     if args.parts == "on":
         integ.shared, integ.parts = False, True
         model.enable_parts(tets_per_part=args.tets_per_part)
