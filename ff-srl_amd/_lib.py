@@ -94,7 +94,8 @@ class BvhJob(C.Structure):
 
 class Parts(C.Structure):
     _fields_ = [
-        ("n_parts", C.c_int32), ("max_local", C.c_int32), ("parts", _vp), ("env_parts", _vp), ("verts", _vp),
+        ("n_parts", C.c_int32), ("max_local", C.c_int32), ("max_solo", C.c_int32), ("reserved", C.c_int32),
+        ("parts", _vp), ("env_parts", _vp), ("verts", _vp),
         ("tets16", _vp), ("edges16", _vp), ("sums", _vp), ("stride", C.c_int64), ("ctr", _vp), ("claims", _vp),
         ("n_claims", C.c_int64), ("host_env_parts", _vp),
     ]
@@ -156,6 +157,7 @@ PLAIN_SIGNATURES = {
     "ffsrl_bvh_scratch_bytes": ([_i32], C.c_int64),
     "ffsrl_xcd_count": ([], C.c_int),          # count > 0, or an error code < 0
     "ffsrl_parts_max_local": ([], C.c_int),
+    "ffsrl_parts_max_solo": ([], C.c_int),
 }
 
 

@@ -1346,6 +1346,99 @@ constexpr uint32_t kPartIdMask = kPartShared - 1u;
 // LDS per local vertex: sums 16 B, position 16 B, substep start 16 B, flags 4 B, grasp weight 4 B.
 constexpr int kPartLocalBytes = 56;
 constexpr int kPartMaxLocal = (kLdsBytes - 512) / kPartLocalBytes;
+// An env in one part: sums and positions only (32 B per vertex).
+constexpr int kSoloBytes = 32;
+constexpr int kSoloMaxVertices = (kLdsBytes - 512) / kSoloBytes;
+
+// An env held by one part: its vertex list is the identity and no vertex is
+// shared, so this is the one-workgroup frame (frame_env) with the positions
+// in LDS: the projection gathers from LDS, the apply reads and writes LDS,
+// and the substep-start positions, velocities and grasp weights stay in the
+// model's arrays as there.  Each pass has no exchange and no wait; the model's
+// p is written once, after the last pass.
+template <bool CARRY>
+__device__ void solo_env(const ffsrl_batch& b, const ffsrl_params& prm, const PartsWs& ws, int env, int refit,
+                         unsigned long long* lds) {
+  constexpr int BLOCK = 1024;
+  const ffsrl_env_desc e = b.envs[env];
+  const int nv = e.nv, tid = threadIdx.x;
+  FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
+  float4* Pl = reinterpret_cast<float4*>(lds + 2 * nv);
+  const int64_t vb = e.vbase;
+  float* X = b.x;
+  float* Vv = b.v;
+  VertexStep vs;
+  vs.g = mk(prm.gravity[0], prm.gravity[1], prm.gravity[2]);
+  vs.dt = prm.dt;
+  vs.damp1 = 1.0f - prm.damping;
+  vs.inv_dt = 1.0f / prm.dt;
+  vs.ground = prm.ground;
+  vs.ks = prm.drag_ks;
+  const NHConst nhk = nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt);
+  const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
+  const int64_t T = b.n_tet_total;
+  const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
+  const int nt = (prm.vol_mode != FFSRL_OFF || prm.nh_on) ? e.nt : 0;
+  const int ne = prm.dist_mode != FFSRL_OFF ? e.ne : 0;
+  const int nmax = nt > ne ? nt : ne;
+  const int I = prm.iterations;
+
+  // predict of substep 0; each thread notes which of its vertices (i = tid +
+  // k * BLOCK, k < 5 since nv <= 5,104) are grasped
+  uint32_t grasped = 0;
+  for (int i = tid, k = 0; i < nv; i += BLOCK, ++k) {
+    const float4 x = ld4(X, vb + i), v = ld4(Vv, vb + i);
+    Pl[i] = with_w(predict(xyz(x), xyz(v), x.w, vs.g, vs.dt, vs.damp1), x.w);
+    if (b.drag[vb + i] != 0.0f) grasped |= 1u << k;
+    lds[2 * i] = 0ull;
+    lds[2 * i + 1] = 0ull;
+  }
+  if (tid == 0) sat_flag() = 0;
+  __syncthreads();
+  for (int s = 0; s < prm.substeps; ++s) {
+    for (int it = 0; it < I; ++it) {
+      project<BLOCK, CARRY, false>(b, prm, e, c, nhk, PosLds{Pl}, Items{b.tets16, b.edges16, 0, 0}, T, nt, ne, 0,
+                                   nmax, reset_lambda && it == 0);
+      __syncthreads();
+      const bool last_it = it == I - 1, next_sub = s + 1 < prm.substeps;
+      for (int i = tid, k = 0; i < nv; i += BLOCK, ++k) {
+        const unsigned long long w0 = lds[2 * i], w1 = lds[2 * i + 1];
+        lds[2 * i] = 0ull;
+        lds[2 * i + 1] = 0ull;
+        int32_t fx, fy, fz, n;
+        unpack2(w0, &fx, &fy);
+        unpack2(w1, &fz, &n);
+        n = sum_count(n);
+        const float4 p4 = Pl[i];
+        f3 p = xyz(p4);
+        if (n > 0) {
+          const float sc = c.q * __builtin_amdgcn_rcpf((float)n);
+          p = p + mk((float)fx, (float)fy, (float)fz) * sc;
+        }
+        if ((grasped >> k) & 1u) p = grasp_pull<true>(p, tip, b.drag[vb + i], vs.ks);
+        if (last_it) {
+          if (p.y < vs.ground) p.y = vs.ground;
+          const float4 x4 = ld4(X, vb + i);
+          const f3 v = (p - xyz(x4)) * vs.inv_dt;
+          st4(X, vb + i, with_w(p, x4.w));
+          if (next_sub)
+            p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
+          else
+            st4(Vv, vb + i, with_w(v, 0.0f));
+        }
+        Pl[i] = with_w(p, p4.w);
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < nv; i += BLOCK) st4(b.p, vb + i, Pl[i]);
+  if (tid == 0) {
+    atomicAdd(&ws.env_ctr(env)[0], 1u);  // the env's one part is done
+    if (sat_flag()) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_SATURATED);
+  }
+  __syncthreads();
+  if (refit) refit_env<BLOCK>(b, e);
+}
 
 template <bool CARRY>
 __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const PartsWs& ws, int env, int pi, int np,
@@ -1377,7 +1470,6 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
   const int ne = prm.dist_mode != FFSRL_OFF ? e1 - e0 : 0;
   const int nmax = nt > ne ? nt : ne;
   uint32_t* ctr = ws.env_ctr(env);
-  const bool solo = np == 1;  // a whole env in one part: no vertex is shared, no pass waits
 
   // the part's vertices: flags, state and the predict of substep 0
   for (int i = tid; i < nloc; i += BLOCK) {
@@ -1401,27 +1493,25 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
     __syncthreads();
     // shared vertices: this part's sums into the pass's plane, then arrive and wait for the env's parts
     unsigned long long* gs = ws.plane(u % 3, vb);
-    if (!solo) {
-      for (int i = tid; i < nloc; i += BLOCK) {
+    for (int i = tid; i < nloc; i += BLOCK) {
       const uint32_t m = Ml[i];
-        if (m & kPartShared) {
-          const int g = (int)(m & kPartIdMask);
-          const unsigned long long w0 = lds[2 * i], w1 = lds[2 * i + 1];
-          if (w0) atomicAdd(&gs[2 * g], w0);
-          if (w1) atomicAdd(&gs[2 * g + 1], w1);
-        }
+      if (m & kPartShared) {
+        const int g = (int)(m & kPartIdMask);
+        const unsigned long long w0 = lds[2 * i], w1 = lds[2 * i + 1];
+        if (w0) atomicAdd(&gs[2 * g], w0);
+        if (w1) atomicAdd(&gs[2 * g + 1], w1);
       }
-      wait_vmem();
-      __syncthreads();
-      if (tid == 0) {
-        atomicAdd(&ctr[1], 1u);
-        s_ok = share_wait(ctr, (uint32_t)(np * (u + 1)));
-      }
-      __syncthreads();
-      if (!s_ok) {
-        ok = false;
-        break;
-      }
+    }
+    wait_vmem();
+    __syncthreads();
+    if (tid == 0) {
+      atomicAdd(&ctr[1], 1u);
+      s_ok = share_wait(ctr, (uint32_t)(np * (u + 1)));
+    }
+    __syncthreads();
+    if (!s_ok) {
+      ok = false;
+      break;
     }
     // apply (+ grasp; at the end of a substep the ground clamp, integrate and next predict), as
     // the one-workgroup frame; the owner clears its shared vertices' entries of plane (u - 1) % 3
@@ -1479,14 +1569,12 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
     }
     wait_vmem();
     __syncthreads();
-    if (!solo) {
-      if (tid == 0) {
-        atomicAdd(&ctr[1], 1u);
-        s_ok = share_wait(ctr, (uint32_t)(np * (U + 1)));
-      }
-      __syncthreads();
-      ok = s_ok;
+    if (tid == 0) {
+      atomicAdd(&ctr[1], 1u);
+      s_ok = share_wait(ctr, (uint32_t)(np * (U + 1)));
     }
+    __syncthreads();
+    ok = s_ok;
   }
   if (ok) {
     // every part has read the last plane: the owners clear it
@@ -1528,7 +1616,11 @@ __global__ __launch_bounds__(1024) void k_frame_parts(ffsrl_batch b, ffsrl_param
       // the k-th part of this XCD's slots x, x + nxcd, ... in order
       const int entry = ws.claims[c0 + k], el = entry >> 6, part = entry & 63;
       const int env = list ? list[el] : el;
-      part_env<CARRY>(b, prm, ws, env, ws.env_parts[2 * env] + part, ws.env_parts[2 * env + 1], refit, lds);
+      const int np = ws.env_parts[2 * env + 1];
+      if (np == 1)
+        solo_env<CARRY>(b, prm, ws, env, refit, lds);
+      else
+        part_env<CARRY>(b, prm, ws, env, ws.env_parts[2 * env] + part, np, refit, lds);
       __syncthreads();  // LDS is reused by the next part
     }
   }
@@ -2110,6 +2202,7 @@ int ffsrl_frame_shared(const ffsrl_batch* b, const ffsrl_params* prm, const int3
 }
 
 int ffsrl_parts_max_local(void) { return kPartMaxLocal; }
+int ffsrl_parts_max_solo(void) { return kSoloMaxVertices; }
 
 int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl_parts* pt, const int32_t* envs,
                       int32_t n, int32_t refit, void* stream) {
@@ -2123,12 +2216,17 @@ int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl
     return fail(FFSRL_E_ARG, "XPBD with iterations > 1 needs lambda arrays");
   if (refit && (!b->tris || !b->level_nodes || !b->level_info || !b->level_off || !b->bvh_lo || !b->bvh_hi))
     return fail(FFSRL_E_ARG, "refit requested without BVH arrays");
-  if ((prm->vol_mode || prm->nh_on) && !pt->tets16) return fail(FFSRL_E_ARG, "partitioned frame needs part tet ids");
-  if (prm->dist_mode && !pt->edges16) return fail(FFSRL_E_ARG, "partitioned frame needs part edge ids");
+  if ((prm->vol_mode || prm->nh_on) && (!pt->tets16 || !b->tets16))
+    return fail(FFSRL_E_ARG, "partitioned frame needs part and packed tet ids");
+  if (prm->dist_mode && (!pt->edges16 || !b->edges16))
+    return fail(FFSRL_E_ARG, "partitioned frame needs part and packed edge ids");
   if (!pt->parts || !pt->env_parts || !pt->verts || !pt->sums || !pt->ctr || !pt->claims || !pt->host_env_parts)
     return fail(FFSRL_E_ARG, "parts: NULL table, sums, counters or claims");
-  if (pt->max_local < 1 || pt->max_local > kPartMaxLocal)
+  if (pt->max_local < 0 || pt->max_local > kPartMaxLocal)
     return fail(FFSRL_E_LIMIT, "a part of %d local vertices (LDS holds %d)", pt->max_local, kPartMaxLocal);
+  if (pt->max_solo < 0 || pt->max_solo > kSoloMaxVertices)
+    return fail(FFSRL_E_LIMIT, "an env of %d vertices in one part (LDS holds %d)", pt->max_solo, kSoloMaxVertices);
+  if (pt->max_local < 1 && pt->max_solo < 1) return fail(FFSRL_E_ARG, "parts: no part sizes given");
   if (n < 0 || n > b->n_env) return fail(FFSRL_E_ARG, "%d environments listed, batch has %d", n, b->n_env);
   if (!b->status) return fail(FFSRL_E_ARG, "the batched frames need the per-env status array");
   if (n == 0) return 0;
@@ -2151,6 +2249,8 @@ int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl
       return fail(FFSRL_E_LIMIT, "env %d has %d parts; they must run at once on one XCD (%d CUs)", i, np, per_xcd);
     if (b->host_envs) {
       const ffsrl_env_desc& h = b->host_envs[i];
+      if (np == 1 && h.nv > pt->max_solo)
+        return fail(FFSRL_E_ARG, "env %d of %d vertices in one part, max_solo %d", i, h.nv, pt->max_solo);
       if (h.max_terms > FFSRL_MAX_TERMS)
         return fail(FFSRL_E_LIMIT, "env %d: a vertex receives %d Jacobi terms in one pass (limit %d)", i,
                     h.max_terms, FFSRL_MAX_TERMS);
@@ -2161,7 +2261,7 @@ int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl
   if (total > pt->n_claims) return fail(FFSRL_E_ARG, "parts: %lld claim entries for %lld parts", (long long)pt->n_claims,
                                         (long long)total);
   if (n >= (1 << 25)) return fail(FFSRL_E_LIMIT, "%d environments listed (claim entries hold 2^25)", n);
-  const int lds = kPartLocalBytes * pt->max_local;
+  const int lds = std::max(kPartLocalBytes * pt->max_local, kSoloBytes * pt->max_solo);
   auto kern = carry ? k_frame_parts<true> : k_frame_parts<false>;
   FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   FFSRL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 1024, lds));

@@ -860,29 +860,33 @@ class SimModelDO:
         (``ffsrl_frame_parts``) and allocate its sums and counters (zero, and
         left at zero by every frame).  A mesh gets ceil(tets / tets_per_part)
         parts (edges when the scheme has no tet constraints), more while a
-        part has more vertices than one workgroup's LDS holds; ``parts``
+        part has more vertices than one workgroup's LDS holds (an env in one
+        part needs 32 B of LDS per vertex, a part of several 56 B); ``parts``
         gives the count per mesh instead.  The partition follows the current
         constraint order: ``set_constraint_order`` rebuilds it."""
         use_t = self.vol_mode != OFF or self.nh_on
         use_e = self.dist_mode != OFF
-        cap = int(lib.ffsrl_parts_max_local())
+        cap, solo_cap = int(lib.ffsrl_parts_max_local()), int(lib.ffsrl_parts_max_solo())
 
         def build(k):
             t = self.topo[k]
             m = t.mesh
             items = m.num_tets if use_t else m.num_edges
             P = int(parts[k]) if parts is not None else max(1, -(-items // int(tets_per_part)))
+            if parts is None and P == 1 and m.num_vertices > solo_cap:
+                P = 2
             # a part's vertices grow with its share of the mesh: start near the LDS bound
-            if parts is None:
+            if parts is None and P > 1:
                 P = max(P, min(PART_MAX, -(-m.num_vertices * 5 // (4 * cap))))
             while True:
                 r = partition_mesh(t.tets, t.edges, m.num_vertices, P, use_t, use_e)
-                if r["max_local"] <= cap or parts is not None or P >= PART_MAX:
+                limit = solo_cap if P == 1 else cap
+                if r["max_local"] <= limit or parts is not None or P >= PART_MAX:
                     break
                 P += 1
-            if r["max_local"] > cap:
-                raise ValueError(f"mesh {k}: a part of {r['max_local']} vertices exceeds the {cap} that one "
-                                 f"workgroup holds ({P} parts)")
+            if r["max_local"] > (solo_cap if P == 1 else cap):
+                raise ValueError(f"mesh {k}: a part of {r['max_local']} vertices exceeds the "
+                                 f"{solo_cap if P == 1 else cap} that one workgroup holds ({P} parts)")
             return r
 
         built = _parallel(build, range(len(self.topo)))
@@ -906,7 +910,8 @@ class SimModelDO:
                 raise RuntimeError(f"ffsrl_xcd_count: {lib.ffsrl_last_error().decode()}")
         pt = Parts()
         pt.n_parts = int(rows.shape[0])
-        pt.max_local = max(r["max_local"] for r in built)
+        pt.max_local = max([r["max_local"] for r in built if r["parts"] > 1], default=0)
+        pt.max_solo = max([r["max_local"] for r in built if r["parts"] == 1], default=0)
         pt.parts, pt.env_parts, pt.verts = _ptr(self.p_rows), _ptr(self.p_env), _ptr(self.p_verts)
         pt.tets16, pt.edges16 = _ptr(self.p_tets16), _ptr(self.p_edges16)
         pt.sums, pt.stride, pt.ctr = _ptr(self.p_sums), self.numVertices, _ptr(self.p_ctr)

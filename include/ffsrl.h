@@ -266,8 +266,11 @@ int ffsrl_frame_shared_noalloc(const ffsrl_batch* b, const ffsrl_params* prm, co
 #define FFSRL_PART_SHARED 0x40000000u  /* verts entry: other parts hold the vertex too */
 typedef struct ffsrl_parts {
   int32_t n_parts;
-  int32_t max_local;              /* most local vertices of a part (sizes LDS); at most
-                                     ffsrl_parts_max_local() */
+  int32_t max_local;              /* most local vertices of a part of an env with several parts
+                                     (sizes LDS); at most ffsrl_parts_max_local() */
+  int32_t max_solo;               /* most vertices of an env held by one part (its vertex list is the
+                                     identity; sizes LDS); at most ffsrl_parts_max_solo() */
+  int32_t reserved;
   const int32_t* parts;           /* int [n_parts][8]: tet begin, tet end, edge begin, edge end (within
                                      the mesh's tets / edges, current order), first verts entry, local
                                      vertex count, 0, 0 */
@@ -303,8 +306,10 @@ typedef struct ffsrl_parts {
 int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl_parts* parts,
                       const int32_t* envs, int32_t n, int32_t refit, void* stream);
 
-/* Most local vertices of one part of ffsrl_frame_parts (56 B of LDS each). */
+/* Most local vertices of one part of an env cut into several parts (56 B of
+ * LDS each), and most vertices of an env held by one part (32 B each). */
 int ffsrl_parts_max_local(void);
+int ffsrl_parts_max_solo(void);
 
 /* Number of XCDs of the current device (probed once with one workgroup per
  * CU; the shared frame assigns environments to XCDs).  Allocates on its

@@ -152,7 +152,10 @@ def test_argument_errors_are_reported(pkg):
     pt.max_local = pkg.lib.ffsrl_parts_max_local() + 1
     with pytest.raises(pkg.FFSRLError, match="local vertices"):
         pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None)
-    pt.max_local = 100
+    pt.max_local, pt.max_solo = 100, pkg.lib.ffsrl_parts_max_solo() + 1
+    with pytest.raises(pkg.FFSRLError, match="in one part"):
+        pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None)
+    pt.max_solo = 0
     prm.collide = 1
     with pytest.raises(pkg.FFSRLError, match="no collision pass"):
         pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None)
@@ -161,6 +164,7 @@ def test_argument_errors_are_reported(pkg):
         pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 5, 0, None)
     assert pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None) == 0      # nothing listed: no launch
     assert 2000 <= pkg.lib.ffsrl_parts_max_local() < 3000
+    assert pkg.lib.ffsrl_parts_max_solo() == 5104
 
 
 def test_frame_rejects_too_many_terms_per_vertex(pkg):

@@ -67,8 +67,11 @@ def _shared_fraction(m):
     return float(((v & 0x40000000) != 0).mean())
 
 
-@pytest.mark.parametrize("tets_per_part", [1500, 5120])
+@pytest.mark.parametrize("tets_per_part", [1500, 5120, 1 << 30])
 def test_parts_equal_batched_mixed(pkg, gpu, tets_per_part):
+    """1 << 30 tets per part: every env of at most 5,104 vertices in one part
+    (the positions-in-LDS form of the one-workgroup frame), larger envs in as
+    few parts as LDS allows."""
     meshes = _meshes(pkg, 12, 1000, 30000, seed=5)
     a, b = _pair(pkg, gpu, meshes, NH_DV)
     b.enable_parts(tets_per_part=tets_per_part)
@@ -152,6 +155,7 @@ def test_parts_env_above_the_vertex_limit(pkg, orc, gpu):
     m.enable_parts()
     print(f"\n[parts] {mesh.num_tets} tets, {mesh.num_vertices} vertices: {m.parts_per_mesh[0]} parts, "
           f"largest {m.parts.max_local} local vertices")
+    assert m.parts.max_solo == 0
     cfg = dict(substeps=10, iterations=1, fps=60, gravity=(0.0, -9.81, 0.0), damping=0.1, dist_mode=2,
                vol_mode=2, nh_on=0, nh_vol_c=1.0 / 5e4, nh_dev_c=1.0 / 1e4, dist_compliance=1e-6,
                vol_compliance=0.0, drag_ks=1.0)
