@@ -121,8 +121,11 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-actors", type=int, default=4096)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
-    ap.add_argument("--serve-streams", type=int, default=2)
-    ap.add_argument("--serve-inflight", type=int, default=2)
+    ap.add_argument("--serve-streams", type=int, default=1)
+    ap.add_argument("--serve-inflight", type=int, default=1)
+    ap.add_argument("--serve-parts", type=int, default=1280,
+                    help="serve every tick with the partitioned frame, this many tets per part (one tick in flight); "
+                         "0: the shared-env frame (--serve-inflight 2 --serve-streams 2 measured best with it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy before timing (HBM counter calibration)")
     ap.add_argument("--collision", action="store_true",
@@ -285,7 +288,7 @@ def serving_run(pkg, wl, args, dev):
     srv = S.SceneServer(sw.classes, capacity, args.serve_budget_gb * 1e9, sw.model_kwargs, sw.tips, device=dev,
                         resolution=sw.resolution, camera_pos=sw.camera_pos, look_at=sw.look_at,
                         block=args.block, streams=args.serve_streams, max_inflight=args.serve_inflight,
-                        shared=args.shared)
+                        shared=args.shared, parts=args.serve_parts)
     if getattr(srv.model, "split_acc", None) is not None:
         srv.model.enable_shared(fold=args.fold == "on")
     for a in range(sw.actors):
@@ -297,7 +300,7 @@ def serving_run(pkg, wl, args, dev):
     out.update(workload=sw.name, duration_s=args.serve_seconds, requests=len(reqs), actors=sw.actors,
                rate_req_per_s=sw.rate, ticks=int(st["ticks"]),
                mean_tick_requests=st["tick_requests"] / max(st["ticks"], 1), admitted=int(st["admitted"]),
-               streams=args.serve_streams, max_inflight=args.serve_inflight,
+               streams=args.serve_streams, max_inflight=args.serve_inflight, parts=args.serve_parts,
                admission_waits=int(st["admission_waits"]), budget_gb=args.serve_budget_gb,
                resident_gb=srv.resident_bytes / 1e9,
                obs="64x64 fp32 depth + 32 scalars per request" if sw.resolution == 64 else f"{sw.resolution}^2 depth")

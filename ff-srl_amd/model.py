@@ -872,7 +872,7 @@ class SimModelDO:
             t = self.topo[k]
             m = t.mesh
             items = m.num_tets if use_t else m.num_edges
-            P = int(parts[k]) if parts is not None else max(1, -(-items // int(tets_per_part)))
+            P = int(parts[k]) if parts is not None else min(PART_MAX, max(1, -(-items // int(tets_per_part))))
             if parts is None and P == 1 and m.num_vertices > solo_cap:
                 P = 2
             # a part's vertices grow with its share of the mesh: start near the LDS bound

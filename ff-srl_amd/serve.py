@@ -14,7 +14,9 @@ mesh's topology).  The loop
   waiting for earlier ticks: ``ffsrl_apply_requests`` (resets, tool moves,
   grasp), ``ffsrl_frame_envs`` (one frame of the stepped scenes, refit
   included; ``ffsrl_frame_shared``, every CU on the tick's scenes, when the
-  tick steps few scenes), ``ffsrl_render_envs`` (depth of every scene of the tick) and
+  tick steps few scenes; ``ffsrl_frame_parts``, each scene in parts of
+  ``parts`` tets with one tick in flight, when ``parts`` is given),
+  ``ffsrl_render_envs`` (depth of every scene of the tick) and
   ``ffsrl_observe_envs`` (32 scalars).  Up to ``max_inflight`` ticks are on
   the GPU at once, on ``streams`` HIP streams, so requests that arrive while
   others run are admitted into the next tick and their work overlaps on the
@@ -104,7 +106,7 @@ class SceneServer:
                  model_kwargs: dict, tips: np.ndarray, device="cuda:0", resolution: int = 64,
                  camera_pos=(0.0, 1.6, 5.5), look_at=(0.0, 0.45, 0.0), max_batch: int = 1024,
                  max_inflight: int = 2, streams: int = 2, block: int = 1024, grasp_radius: float = 0.05,
-                 workspace: Optional[tuple] = None, shared: str = "auto"):
+                 workspace: Optional[tuple] = None, shared: str = "auto", parts: int = 0):
         self.device = torch.device(device)
         self.classes = list(classes)
         self.capacity = [int(c) for c in capacity]
@@ -132,6 +134,13 @@ class SceneServer:
             self.shared_max = 0
         if self.shared_max > 0:
             m.enable_shared()
+        # partitioned frame (tets per part; 0: off) for every tick: its parts
+        # must be resident together, so one tick at a time
+        self.parts = int(parts)
+        if self.parts:
+            if int(max_inflight) != 1:
+                raise ValueError("the partitioned frame serves one tick at a time (max_inflight=1)")
+            m.enable_parts(tets_per_part=self.parts)
         self.radius = float(grasp_radius)
         self.max_batch = int(max_batch)
         self.max_inflight = int(max_inflight)
@@ -296,7 +305,9 @@ class SceneServer:
             r.lo = None if self._lo is None else self._lo.data_ptr()
             r.hi = None if self._hi is None else self._hi.data_ptr()
             lib.ffsrl_apply_requests(model.batch, r, self.radius, s)
-            if m and m <= self.shared_max:
+            if m and self.parts:
+                lib.ffsrl_frame_parts(model.batch, model.params, model.parts, p_step, m, 1, s)
+            elif m and m <= self.shared_max:
                 lib.ffsrl_frame_shared(model.batch, model.params, p_step, m, self.block, 1, s)
             elif m:
                 lib.ffsrl_frame_envs(model.batch, model.params, p_step, m, self.block, 1, s)

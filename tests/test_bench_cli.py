@@ -39,4 +39,5 @@ def test_defaults_are_one_gpu_and_a_short_run():
         sys.path.remove(ROOT)
     assert a.gpus == 1 and a.workload == "mixed"
     assert 1 <= a.steps <= 100 and 0 <= a.warmup <= 10
-    assert a.serve_inflight == 2 and a.serve_streams == 2
+    # serving defaults: partitioned ticks (1,280 tets per part), one tick in flight
+    assert a.serve_inflight == 1 and a.serve_streams == 1 and a.serve_parts == 1280

@@ -24,8 +24,10 @@ def _classes(pkg):
 def _server(pkg, gpu, capacity=(4, 4), budget=1e12, max_batch=4, **kw):
     S = pkg.serve
     meshes, tips = _classes(pkg)
+    kw.setdefault("max_inflight", 3)
+    kw.setdefault("streams", 2)
     return S.SceneServer(meshes, capacity, budget, PHYS, tips, device=gpu, resolution=32, max_batch=max_batch,
-                         max_inflight=3, streams=2, grasp_radius=0.08, **kw)
+                         grasp_radius=0.08, **kw)
 
 
 def _script(pkg, actors, steps=6, seed=3):
@@ -159,6 +161,28 @@ def test_shared_ticks_equal_batched_ticks(pkg, gpu):
         assert len(runs["on"][a]) == len(scripts[a])
         for (s1, d1, o1), (s2, d2, o2) in zip(runs["off"][a], runs["on"][a]):
             assert s1 == s2 and torch.equal(d1, d2) and torch.equal(o1, o2), (a, s1)
+
+
+def test_partitioned_ticks_equal_batched_ticks(pkg, gpu):
+    """The same request stream served with the partitioned frame in every
+    tick (500 tets per part: the scene classes in 3 and 7 parts, one tick at
+    a time) and with one workgroup per scene: identical observations.  The
+    partitioned frame refuses more than one tick in flight."""
+    actors = list(range(6))
+    classes = {a: a % 2 for a in actors}
+    scripts = _script(pkg, actors, seed=13)
+    ref = _run(_server(pkg, gpu, shared="off"), scripts, classes)
+    srv = _server(pkg, gpu, shared="off", parts=500, max_inflight=1, streams=1)
+    assert sorted(set(srv.model.parts_per_mesh)) == [3, 7]
+    got = _run(srv, scripts, classes)
+    assert int(srv.model.frame_status().abs().sum()) == 0
+    assert int(srv.model.p_sums.abs().sum()) == 0
+    for a in actors:
+        assert len(got[a]) == len(scripts[a])
+        for (s1, d1, o1), (s2, d2, o2) in zip(ref[a], got[a]):
+            assert s1 == s2 and torch.equal(d1, d2) and torch.equal(o1, o2), (a, s1)
+    with pytest.raises(ValueError, match="one tick at a time"):
+        _server(pkg, gpu, parts=500, max_inflight=2)
 
 
 def test_reset_restores_the_initial_observation(pkg, gpu):
