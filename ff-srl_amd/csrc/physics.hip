@@ -1307,37 +1307,47 @@ struct PartsWs {
 // The claim table of a launch (one workgroup, before the frame on the same
 // stream): XCD x's entries are the parts of its slots x, x + nxcd, ... in
 // order, from claim_info(x)[0], claim_info(x)[1] of them, so a workgroup's
-// k-th claim is one load.
+// k-th claim is one load.  Wavefront x counts XCD x's parts (a sweep of 64
+// slots at a time, lanes summed by shuffles), the XCDs' first entries follow
+// from the counts, and wavefront x writes its entries in a second sweep with
+// an inclusive scan over its lanes.
 __global__ __launch_bounds__(1024) void k_parts_claims(const int32_t* list, int n, int nxcd, PartsWs ws) {
-  __shared__ int scan[1024];
-  __shared__ int s_base;
-  const int tid = threadIdx.x;
-  if (tid == 0) s_base = 0;
+  __shared__ int count[16], first[16];
+  const int lane = threadIdx.x & 63, x = threadIdx.x >> 6;
+  const int cnt = x < nxcd && x < n ? (n - 1 - x) / nxcd + 1 : 0;  // slots of XCD x
+  auto parts_of = [&](int j) { return j < cnt ? ws.env_parts[2 * (list ? list[x + nxcd * j] : x + nxcd * j) + 1] : 0; };
+  int total = 0;
+  for (int j0 = 0; j0 < cnt; j0 += 64) {
+    int v = parts_of(j0 + lane);
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    total += v;
+  }
+  if (lane == 0 && x < 16) count[x] = total;
   __syncthreads();
-  for (int x = 0; x < nxcd; ++x) {
-    const int first = s_base;
-    const int cnt = x < n ? (n - 1 - x) / nxcd + 1 : 0;  // slots of XCD x
-    for (int j0 = 0; j0 < cnt; j0 += 1024) {
-      const int j = j0 + tid, el = x + nxcd * j;
-      const int np = j < cnt ? ws.env_parts[2 * (list ? list[el] : el) + 1] : 0;
-      scan[tid] = np;
-      __syncthreads();
-      for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
-        const int v = tid >= off ? scan[tid - off] : 0;
-        __syncthreads();
-        scan[tid] += v;
-        __syncthreads();
-      }
-      const int start = s_base + scan[tid] - np;
-      for (int q = 0; q < np; ++q) ws.claims[start + q] = (el << 6) | q;
-      __syncthreads();
-      if (tid == 0) s_base += scan[1023];
-      __syncthreads();
+  if (threadIdx.x == 0) {
+    int base = 0;
+    for (int k = 0; k < nxcd && k < 16; ++k) {
+      first[k] = base;
+      base += count[k];
     }
-    if (tid == 0) {
-      ws.claim_info(x)[0] = (uint32_t)first;
-      ws.claim_info(x)[1] = (uint32_t)(s_base - first);
+  }
+  __syncthreads();
+  if (x >= nxcd) return;
+  int base = first[x];
+  for (int j0 = 0; j0 < cnt; j0 += 64) {
+    const int j = j0 + lane, np = parts_of(j);
+    int incl = np;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(incl, off);
+      if (lane >= off) incl += t;
     }
+    const int el = x + nxcd * j;
+    for (int q = 0; q < np; ++q) ws.claims[base + incl - np + q] = (el << 6) | q;
+    base += __shfl(incl, 63);
+  }
+  if (lane == 0) {
+    ws.claim_info(x)[0] = (uint32_t)first[x];
+    ws.claim_info(x)[1] = (uint32_t)count[x];
   }
 }
 
