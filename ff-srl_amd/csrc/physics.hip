@@ -1354,11 +1354,15 @@ __global__ __launch_bounds__(1024) void k_parts_claims(const int32_t* list, int 
 constexpr uint32_t kPartOwner = FFSRL_PART_OWNER, kPartShared = FFSRL_PART_SHARED;
 constexpr uint32_t kPartIdMask = kPartShared - 1u;
 // LDS per local vertex: sums 16 B, position 16 B, substep start 16 B, flags 4 B, grasp weight 4 B.
+// 1 KB is left for the partitioned kernel's static LDS (544 B today: the
+// paths' pass words, the saturation flag and the refit's group offsets;
+// checked at launch).
+constexpr int kPartStaticLds = 1024;
 constexpr int kPartLocalBytes = 56;
-constexpr int kPartMaxLocal = (kLdsBytes - 512) / kPartLocalBytes;
+constexpr int kPartMaxLocal = (kLdsBytes - kPartStaticLds) / kPartLocalBytes;
 // An env in one part: sums and positions only (32 B per vertex).
 constexpr int kSoloBytes = 32;
-constexpr int kSoloMaxVertices = (kLdsBytes - 512) / kSoloBytes;
+constexpr int kSoloMaxVertices = (kLdsBytes - kPartStaticLds) / kSoloBytes;
 
 // An env held by one part: its vertex list is the identity and no vertex is
 // shared, so this is the one-workgroup frame (frame_env) with the positions
@@ -2273,6 +2277,11 @@ int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl
   if (n >= (1 << 25)) return fail(FFSRL_E_LIMIT, "%d environments listed (claim entries hold 2^25)", n);
   const int lds = std::max(kPartLocalBytes * pt->max_local, kSoloBytes * pt->max_solo);
   auto kern = carry ? k_frame_parts<true> : k_frame_parts<false>;
+  hipFuncAttributes fa;
+  FFSRL_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kern)));
+  if ((int)fa.sharedSizeBytes > kPartStaticLds)
+    return fail(FFSRL_E_LIMIT, "partitioned frame: %d B of static LDS exceed the %d B reserved", (int)fa.sharedSizeBytes,
+                kPartStaticLds);
   FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   FFSRL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 1024, lds));
   const int grid = std::max(nxcd, cus * std::max(1, per_cu));

@@ -866,7 +866,8 @@ class SimModelDO:
         constraint order: ``set_constraint_order`` rebuilds it."""
         use_t = self.vol_mode != OFF or self.nh_on
         use_e = self.dist_mode != OFF
-        cap, solo_cap = int(lib.ffsrl_parts_max_local()), int(lib.ffsrl_parts_max_solo())
+        cap = int(lib.ffsrl_parts_max_local())
+        solo_cap = int(lib.ffsrl_parts_max_solo())
 
         def build(k):
             t = self.topo[k]

@@ -164,7 +164,7 @@ def test_argument_errors_are_reported(pkg):
         pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 5, 0, None)
     assert pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None) == 0      # nothing listed: no launch
     assert 2000 <= pkg.lib.ffsrl_parts_max_local() < 3000
-    assert pkg.lib.ffsrl_parts_max_solo() == 5104
+    assert pkg.lib.ffsrl_parts_max_solo() == 5088
 
 
 def test_frame_rejects_too_many_terms_per_vertex(pkg):

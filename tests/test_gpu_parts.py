@@ -69,7 +69,7 @@ def _shared_fraction(m):
 
 @pytest.mark.parametrize("tets_per_part", [1500, 5120, 1 << 30])
 def test_parts_equal_batched_mixed(pkg, gpu, tets_per_part):
-    """1 << 30 tets per part: every env of at most 5,104 vertices in one part
+    """1 << 30 tets per part: every env of at most 5,088 vertices in one part
     (the positions-in-LDS form of the one-workgroup frame), larger envs in as
     few parts as LDS allows."""
     meshes = _meshes(pkg, 12, 1000, 30000, seed=5)
