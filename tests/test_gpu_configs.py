@@ -183,26 +183,32 @@ def test_c2_ellipsoids_64_envs(pkg, orc, gpu, wl):
     """64 seeded ~20k-tet ellipsoids, Neo-Hookean (mu 1e4, lambda 5e4), 8
     grasped surface vertices pulled toward a tool tip on a 2 mm random walk,
     10 substeps, 2 frames, then RGB-D from one traversal (per-triangle
-    colours, 64x64).  The batched (one workgroup per env) and shared-env
-    frames give bit-identical states for all 64 envs; 16 sampled envs
-    against the oracle, RGB-D bit-exact on 4 of them."""
+    colours, 64x64).  The batched (one workgroup per env), shared-env and
+    partitioned (4 parts per env) frames give bit-identical states for all
+    64 envs; 16 sampled envs against the oracle, RGB-D bit-exact on 4 of
+    them."""
     w = wl.ellipsoids(64)
     frames = 2
     walk = _walk(w.num_envs, frames, w.walk_sigma, seed=321)
     models = {}
-    for f in ("batched", "shared"):
+    for f in ("batched", "shared", "parts"):
         model = wl.build_model(pkg, w, gpu)
-        integ = pkg.SimIntegratorDO(gpu, shared=(f == "shared"))
+        integ = pkg.SimIntegratorDO(gpu, shared=(f == "shared"), parts=(f == "parts"))
+        if f == "parts":
+            model.enable_parts()
+            assert model.parts_per_mesh == [4] * 64
         pkg.SimBVH(gpu, model)
         _run_frames(pkg, model, integ, walk, gpu)
         models[f] = model
     torch.cuda.synchronize()
-    a, b = models["batched"], models["shared"]
-    for t1, t2 in ((a.x, b.x), (a.v, b.v), (a.bvh_lo, b.bvh_lo), (a.bvh_hi, b.bvh_hi)):
-        assert torch.equal(t1, t2)
-    assert int(b.shared_status().abs().sum()) == 0
-    del models["batched"], a
-    model = b
+    a = models["batched"]
+    for f in ("shared", "parts"):
+        b = models[f]
+        for t1, t2 in ((a.x, b.x), (a.v, b.v), (a.bvh_lo, b.bvh_lo), (a.bvh_hi, b.bvh_hi), (a.status, b.status)):
+            assert torch.equal(t1, t2), f
+        assert int((b.frame_status() & pkg._lib.STATUS_INCOMPLETE).sum()) == 0
+    del models["batched"], models["parts"], a
+    model = models["shared"]
     ren = pkg.RaycastRendererDO(gpu, model, cameraPos=w.camera_pos, resolution=w.resolution,
                                 render_mode=pkg.RenderMode.texture)
     ren.lookAt(w.look_at)
@@ -230,23 +236,36 @@ def test_c2_ellipsoids_64_envs(pkg, orc, gpu, wl):
                           hit_all[e * n_ray:(e + 1) * n_ray], depth_all[e], frames,
                           rgb=pix_all[e] if colour else None, normals=nrm_all[s] if colour else None)
 
-    _map_report(one, list(enumerate(sample)), f"[C2] batched = shared frame on all 64 envs; {len(sample)} sampled envs:")
+    _map_report(one, list(enumerate(sample)), f"[C2] batched = shared = partitioned frame on all 64 envs; "
+                f"{len(sample)} sampled envs:")
 
 
 def test_c5_large_8192_envs(pkg, orc, gpu, wl):
     """8192 envs of the 20k-tet ellipsoid (one shared mesh), Neo-Hookean +
     XPBD volume + one grasped vertex, 20 substeps x 2 iterations with the
     volume lambdas carried between iterations, one frame, then a 128x128
-    RGB-D render of every env.  16 sampled envs against the oracle, RGB-D
-    bit-exact on 4 of them."""
+    RGB-D render of every env.  The partitioned frame (each env in one part,
+    positions in LDS) equals the batched frame bit for bit on all 8192 envs;
+    16 sampled envs against the oracle, RGB-D bit-exact on 4 of them."""
     w = wl.large(8192)
+    frames = 1
+    walk = _walk(w.num_envs, frames, w.walk_sigma, seed=555)
+    part = wl.build_model(pkg, w, gpu)
+    part.enable_parts(tets_per_part=1 << 30)
+    assert part.parts_per_mesh == [1]
+    pkg.SimBVH(gpu, part)
+    _run_frames(pkg, part, pkg.SimIntegratorDO(gpu, parts=True), walk, gpu)
     model = wl.build_model(pkg, w, gpu)
     assert model.carries_lambda
     integ = pkg.SimIntegratorDO(gpu)
     pkg.SimBVH(gpu, model)
-    frames = 1
-    walk = _walk(w.num_envs, frames, w.walk_sigma, seed=555)
     _run_frames(pkg, model, integ, walk, gpu)
+    torch.cuda.synchronize()
+    for t1, t2 in ((model.x, part.x), (model.v, part.v), (model.lam_t, part.lam_t), (model.bvh_hi, part.bvh_hi),
+                   (model.status, part.status)):
+        assert torch.equal(t1, t2)
+    del part
+    torch.cuda.empty_cache()
     ren = pkg.RaycastRendererDO(gpu, model, cameraPos=w.camera_pos, resolution=w.resolution,
                                 render_mode=pkg.RenderMode.texture)
     ren.lookAt(w.look_at)
@@ -287,9 +306,11 @@ def test_c5_large_8192_envs(pkg, orc, gpu, wl):
 def test_c4_serving_4096_actors(pkg, gpu, wl):
     """3 s of the serving stream (4096 actors, Poisson 20k requests/s, scenes
     of 2k/8k/20k/40k tets with p = 0.4/0.3/0.2/0.1, 200 GB budget, 64x64
-    depth + 32 scalars).  Every request completes; 8 sampled actors (two per
-    size class) get the same observation after every request as when their
-    request script is served alone."""
+    depth + 32 scalars), served as the bench serves it: each tick's frame
+    partitioned (1,280 tets per part), one tick in flight.  Every request
+    completes; 8 sampled actors (two per size class) get the same
+    observation after every request as when their request script is served
+    alone (one workgroup per scene)."""
     S = pkg.serve
     sw = wl.serving()
     load = S.ServingLoad(rate=sw.rate, actors=sw.actors, class_p=sw.class_p, seed=0)
@@ -297,7 +318,8 @@ def test_c4_serving_4096_actors(pkg, gpu, wl):
     cls, reqs = load.generate(duration)
     capacity = np.bincount(cls, minlength=len(sw.classes))
     kw = dict(resolution=sw.resolution, camera_pos=sw.camera_pos, look_at=sw.look_at)
-    srv = S.SceneServer(sw.classes, capacity, 200e9, sw.model_kwargs, sw.tips, device=gpu, **kw)
+    srv = S.SceneServer(sw.classes, capacity, 200e9, sw.model_kwargs, sw.tips, device=gpu, parts=1280,
+                        max_inflight=1, streams=1, **kw)
     for a in range(sw.actors):
         srv.register(a, int(cls[a]))
     active = np.bincount([r.actor for r in reqs], minlength=sw.actors)
@@ -317,13 +339,12 @@ def test_c4_serving_4096_actors(pkg, gpu, wl):
     assert out["completed_steps"] + out["completed_resets"] == len(reqs)
     assert srv.stats["admission_waits"] == 0 and srv.stats["admitted"] == len(np.nonzero(active)[0])
     assert 0.0 < out["p50_step_ms"] <= out["p99_step_ms"]
-    if srv.shared_max > 0:
-        assert int(srv.model.shared_status().abs().sum()) == 0
+    assert int(srv.model.frame_status().abs().sum()) == 0
     del srv
     torch.cuda.empty_cache()
 
     alone = S.SceneServer(sw.classes, (2,) * len(sw.classes), 200e9, sw.model_kwargs, sw.tips, device=gpu,
-                          max_batch=1, **kw)
+                          max_batch=1, shared="off", **kw)
     mine = {a: [r for r in reqs if r.actor == a] for a in watch}
     for a in watch:
         alone.register(a, int(cls[a]))
