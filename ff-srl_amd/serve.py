@@ -135,11 +135,13 @@ class SceneServer:
         if self.shared_max > 0:
             m.enable_shared()
         # partitioned frame (tets per part; 0: off) for every tick: its parts
-        # must be resident together, so one tick at a time
+        # must be resident together, so the frames of two ticks must not run
+        # at once: every tick goes to one stream (ticks in flight then queue
+        # on the device while the host prepares the next)
         self.parts = int(parts)
         if self.parts:
-            if int(max_inflight) != 1:
-                raise ValueError("the partitioned frame serves one tick at a time (max_inflight=1)")
+            if int(streams) != 1:
+                raise ValueError("the partitioned frame serves its ticks on one stream (streams=1)")
             m.enable_parts(tets_per_part=self.parts)
         self.radius = float(grasp_radius)
         self.max_batch = int(max_batch)

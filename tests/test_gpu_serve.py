@@ -167,7 +167,8 @@ def test_partitioned_ticks_equal_batched_ticks(pkg, gpu):
     """The same request stream served with the partitioned frame in every
     tick (500 tets per part: the scene classes in 3 and 7 parts, one tick at
     a time) and with one workgroup per scene: identical observations.  The
-    partitioned frame refuses more than one tick in flight."""
+    partitioned frame refuses ticks on more than one stream (two frames
+    could then run at once)."""
     actors = list(range(6))
     classes = {a: a % 2 for a in actors}
     scripts = _script(pkg, actors, seed=13)
@@ -181,8 +182,8 @@ def test_partitioned_ticks_equal_batched_ticks(pkg, gpu):
         assert len(got[a]) == len(scripts[a])
         for (s1, d1, o1), (s2, d2, o2) in zip(ref[a], got[a]):
             assert s1 == s2 and torch.equal(d1, d2) and torch.equal(o1, o2), (a, s1)
-    with pytest.raises(ValueError, match="one tick at a time"):
-        _server(pkg, gpu, parts=500, max_inflight=2)
+    with pytest.raises(ValueError, match="one stream"):
+        _server(pkg, gpu, parts=500, max_inflight=2, streams=2)
 
 
 def test_reset_restores_the_initial_observation(pkg, gpu):
