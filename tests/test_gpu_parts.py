@@ -212,3 +212,38 @@ def test_parts_abandoned_frame_is_reported_and_cleaned(pkg, gpu):
     pt.stepModel(b, refit=True)
     _equal(a, b)
     _clean(pkg, b)
+
+
+def test_parts_graph_replays_equal_eager(pkg, gpu):
+    """The partitioned frame (claim table and frame) captured in a HIP graph
+    and replayed four times, with eager work and synchronisations between
+    the replays, gives after every replay the state of the same number of
+    eager batched frames; the frame resets its own counters (no memset)."""
+    import ctypes as C
+    lib = pkg.lib
+    meshes = _meshes(pkg, 8, 1000, 12000, seed=41)
+    a, b = _pair(pkg, gpu, meshes, NH_DV)
+    b.enable_parts(tets_per_part=3000)          # partition, allocation and the XCD probe outside the capture
+    assert max(b.parts_per_mesh) > 1 and min(b.parts_per_mesh) == 1
+    integ = pkg.SimIntegratorDO(gpu)
+    order = b.dispatch_order()
+    torch.cuda.synchronize()
+    g = C.c_void_p()
+    cs = torch.cuda.Stream()
+    with torch.cuda.stream(cs):
+        lib.ffsrl_graph_begin(cs.cuda_stream)
+        lib.ffsrl_frame_parts(b.batch, b.params, b.parts, order.data_ptr(), b.numEnvs, 1, cs.cuda_stream)
+        lib.ffsrl_graph_end(cs.cuda_stream, C.byref(g))
+    torch.cuda.synchronize()
+    assert torch.equal(a.x, b.x)                   # capturing ran nothing
+    try:
+        for k in range(4):
+            integ.stepModel(a, refit=True)
+            with torch.cuda.stream(cs):
+                lib.ffsrl_graph_launch(g, cs.cuda_stream)
+            torch.cuda.synchronize()
+            _ = float(b.x[:, :3].sum())                # eager work between the replays
+            _equal(a, b)
+            _clean(pkg, b)
+    finally:
+        lib.ffsrl_graph_destroy(g)
