@@ -96,7 +96,8 @@ class Parts(C.Structure):
     _fields_ = [
         ("n_parts", C.c_int32), ("max_local", C.c_int32), ("max_solo", C.c_int32), ("reserved", C.c_int32),
         ("parts", _vp), ("env_parts", _vp), ("verts", _vp),
-        ("tets16", _vp), ("edges16", _vp), ("sums", _vp), ("stride", C.c_int64), ("ctr", _vp), ("claims", _vp),
+        ("tets16", _vp), ("edges16", _vp), ("tris16", _vp), ("sums", _vp), ("stride", C.c_int64), ("ctr", _vp),
+        ("claims", _vp),
         ("n_claims", C.c_int64), ("host_env_parts", _vp),
     ]
 
@@ -141,8 +142,8 @@ SIGNATURES = {
     "ffsrl_bvh_build": [_vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _P(_i32), _P(_i32)],
     "ffsrl_color_constraints": [_vp, _i64, _i32, _i32, _vp, _P(_i32)],
     "ffsrl_order_constraints": [_vp, _i64, _i32, _vp, _i32, _i32, _vp],
-    "ffsrl_partition_mesh": [_vp, _i64, _vp, _i64, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _P(_i64), _vp, _vp,
-                             _P(_i32)],
+    "ffsrl_partition_mesh": [_vp, _i64, _vp, _i64, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i64,
+                             _P(_i64), _vp, _vp, _vp, _P(_i32)],
     "ffsrl_graph_begin": [_vp],
     "ffsrl_graph_end": [_vp, _P(_vp)],
     "ffsrl_graph_launch": [_vp, _vp],

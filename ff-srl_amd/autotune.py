@@ -14,8 +14,7 @@ Knobs (all leave the results unchanged; the frame's Jacobi sums are integers):
   parts     the partitioned frame (each env's constraints cut into parts,
             one 1024-thread workgroup per part with the part's state in
             LDS): 0 (not partitioned), or the tets per part (PART_WHOLE:
-            as few parts as a workgroup's LDS allows), tried without the
-            collision pass (it has none).
+            as few parts as a workgroup's LDS allows).
 
 A trial runs whole frames on a snapshot of the state, which is restored
 afterwards, so tuning does not advance the simulation.  The cache is a JSON
@@ -46,8 +45,8 @@ DEFAULT = {"block": 1024, "dispatch": "lpt", "chunk": LOCAL_CHUNK, "shared": Fal
 
 def parts_candidates(model: SimModelDO) -> tuple:
     """(0,) or (0, *PART_SIZES): the partitioned frame (tets per part; 0 =
-    not partitioned) is a candidate without the collision pass."""
-    if model.collision or lib.ffsrl_xcd_count() <= 0:
+    not partitioned) is a candidate where the device has an XCD map."""
+    if lib.ffsrl_xcd_count() <= 0:
         return (0,)
     return (0,) + PART_SIZES
 

@@ -125,24 +125,40 @@ int ffsrl_order_constraints(const int32_t* elems, int64_t n, int32_t arity, cons
 
 // Partition of one mesh for the partitioned frame: part p takes tets
 // [p nt / P, (p+1) nt / P) and edges [p ne / P, (p+1) ne / P) of the current
-// order; its vertices are those its constraints in use touch, in increasing
-// mesh id (their local ids), plus the untouched vertices, dealt round-robin
-// in increasing id.  A vertex is flagged SHARED when several parts hold it
-// and OWNER on the lowest part that holds it.  Two passes over the
-// constraints with a per-vertex stamp: O(4 nt + 2 ne + nv) plus the sort of
-// each part's vertex list.
-int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, int64_t ne, int32_t nv,
-                         int32_t parts, int32_t use_tets, int32_t use_edges, int32_t* rows, int32_t* verts,
-                         int64_t verts_cap, int64_t* n_verts, uint64_t* tets16, uint32_t* edges16,
-                         int32_t* max_local) {
-  if (parts < 1 || nv < 0 || nt < 0 || ne < 0 || !rows || !verts || !n_verts || !max_local ||
-      (use_tets && nt && (!tets || !tets16)) || (use_edges && ne && (!edges || !edges16)))
+// order; its vertices are those its constraints in use touch, plus the
+// untouched vertices, dealt round-robin in increasing id, and, with the
+// collision pass, the vertices of the surface triangles it is given: a
+// triangle goes to the lowest part holding its first vertex.  A part's
+// vertices are numbered locally in increasing mesh id.  A vertex is flagged
+// SHARED when several parts hold it and OWNER on the lowest part that holds
+// it.  Three sweeps over the constraints with a per-vertex stamp (membership
+// counts, then triangle extras, then the lists): O(4 nt + 2 ne + 3 nf + nv)
+// plus the sort of each part's vertex list.
+int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, int64_t ne, const int32_t* tris,
+                         int64_t nf, int32_t nv, int32_t parts, int32_t use_tets, int32_t use_edges,
+                         int32_t use_tris, int32_t* rows, int32_t* verts, int64_t verts_cap, int64_t* n_verts,
+                         uint64_t* tets16, uint32_t* edges16, uint64_t* tris16, int32_t* max_local) {
+  if (parts < 1 || nv < 0 || nt < 0 || ne < 0 || nf < 0 || !rows || !verts || !n_verts || !max_local ||
+      (use_tets && nt && (!tets || !tets16)) || (use_edges && ne && (!edges || !edges16)) ||
+      (use_tris && nf && (!tris || !tris16)))
     return fail(FFSRL_E_ARG, "partition_mesh: bad arguments");
   const int64_t P = parts;
   auto tbeg = [&](int64_t p) { return p * nt / P; };
   auto ebeg = [&](int64_t p) { return p * ne / P; };
+  auto in_range = [&](int32_t v) { return v >= 0 && v < nv; };
+  if (use_tets)
+    for (int64_t i = 0; i < 4 * nt; ++i)
+      if (!in_range(tets[i])) return fail(FFSRL_E_ARG, "partition_mesh: a tet vertex id outside [0, %d)", nv);
+  if (use_edges)
+    for (int64_t i = 0; i < 2 * ne; ++i)
+      if (!in_range(edges[i])) return fail(FFSRL_E_ARG, "partition_mesh: an edge vertex id outside [0, %d)", nv);
+  if (use_tris)
+    for (int64_t f = 0; f < nf; ++f)
+      for (int k = 0; k < 3; ++k)
+        if (!in_range(tris[4 * f + k]))
+          return fail(FFSRL_E_ARG, "partition_mesh: a triangle vertex id outside [0, %d)", nv);
   std::vector<int32_t> stamp((size_t)nv, -1), holders((size_t)nv, 0), first((size_t)nv, -1);
-  // visits the vertices of part p's constraints in use, each once per part
+  // visits the vertices of part p's constraints in use (a vertex once per constraint)
   auto each = [&](int64_t p, auto&& fn) {
     if (use_tets)
       for (int64_t t = tbeg(p); t < tbeg(p + 1); ++t)
@@ -151,9 +167,9 @@ int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, 
       for (int64_t e = ebeg(p); e < ebeg(p + 1); ++e)
         for (int k = 0; k < 2; ++k) fn(edges[2 * e + k]);
   };
+  // memberships from the constraints, then the untouched vertices
   for (int64_t p = 0; p < P; ++p)
     each(p, [&](int32_t v) {
-      if (v < 0 || v >= nv) return;
       if (stamp[(size_t)v] != (int32_t)p) {
         stamp[(size_t)v] = (int32_t)p;
         if (holders[(size_t)v]++ == 0) first[(size_t)v] = (int32_t)p;
@@ -165,27 +181,50 @@ int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, 
     if (holders[(size_t)v] == 0) {
       const int32_t p = (int32_t)(nl++ % P);
       loose[(size_t)p].push_back(v);
+      holders[(size_t)v] = 1;
       first[(size_t)v] = p;
     }
+  // triangles by part (the lowest part holding the first vertex), in leaf order within a part
+  std::vector<int64_t> tri_off((size_t)P + 1, 0), tri_list;
+  if (use_tris) {
+    for (int64_t f = 0; f < nf; ++f) tri_off[(size_t)first[(size_t)tris[4 * f]] + 1] += 1;
+    for (int64_t p = 0; p < P; ++p) tri_off[(size_t)p + 1] += tri_off[(size_t)p];
+    tri_list.resize((size_t)nf);
+    std::vector<int64_t> fill(tri_off.begin(), tri_off.end() - 1);
+    for (int64_t f = 0; f < nf; ++f) tri_list[(size_t)fill[(size_t)first[(size_t)tris[4 * f]]]++] = f;
+    // the triangles' vertices join their part
+    std::fill(stamp.begin(), stamp.end(), -1);
+    for (int64_t p = 0; p < P; ++p) {
+      each(p, [&](int32_t v) { stamp[(size_t)v] = (int32_t)p; });
+      for (int32_t v : loose[(size_t)p]) stamp[(size_t)v] = (int32_t)p;
+      for (int64_t j = tri_off[(size_t)p]; j < tri_off[(size_t)p + 1]; ++j)
+        for (int k = 0; k < 3; ++k) {
+          const int32_t v = tris[4 * tri_list[(size_t)j] + k];
+          if (stamp[(size_t)v] != (int32_t)p) {
+            stamp[(size_t)v] = (int32_t)p;
+            holders[(size_t)v] += 1;
+            first[(size_t)v] = std::min(first[(size_t)v], (int32_t)p);
+          }
+        }
+    }
+  }
   std::fill(stamp.begin(), stamp.end(), -1);
   std::vector<int32_t> local((size_t)nv, 0), list;
   int64_t off = 0;
   int32_t most = 0;
   for (int64_t p = 0; p < P; ++p) {
     list.clear();
-    bool bad = false;
-    each(p, [&](int32_t v) {
-      if (v < 0 || v >= nv) {
-        bad = true;
-        return;
-      }
+    auto add = [&](int32_t v) {
       if (stamp[(size_t)v] != (int32_t)p) {
         stamp[(size_t)v] = (int32_t)p;
         list.push_back(v);
       }
-    });
-    if (bad) return fail(FFSRL_E_ARG, "partition_mesh: a vertex id outside [0, %d)", nv);
-    list.insert(list.end(), loose[(size_t)p].begin(), loose[(size_t)p].end());
+    };
+    each(p, add);
+    for (int32_t v : loose[(size_t)p]) add(v);
+    if (use_tris)
+      for (int64_t j = tri_off[(size_t)p]; j < tri_off[(size_t)p + 1]; ++j)
+        for (int k = 0; k < 3; ++k) add(tris[4 * tri_list[(size_t)j] + k]);
     std::sort(list.begin(), list.end());
     const int64_t n = (int64_t)list.size();
     if (off + n > verts_cap) return fail(FFSRL_E_ARG, "partition_mesh: verts holds %lld entries", (long long)verts_cap);
@@ -207,6 +246,12 @@ int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, 
       for (int64_t e = ebeg(p); e < ebeg(p + 1); ++e)
         edges16[e] = (uint32_t)(uint16_t)local[(size_t)edges[2 * e]] |
                      ((uint32_t)(uint16_t)local[(size_t)edges[2 * e + 1]] << 16);
+    if (use_tris)
+      for (int64_t j = tri_off[(size_t)p]; j < tri_off[(size_t)p + 1]; ++j) {
+        uint64_t w = 0;
+        for (int k = 0; k < 3; ++k) w |= (uint64_t)(uint16_t)local[(size_t)tris[4 * tri_list[(size_t)j] + k]] << (16 * k);
+        tris16[j] = w;
+      }
     int32_t* r = rows + 8 * p;
     r[0] = (int32_t)tbeg(p);
     r[1] = (int32_t)tbeg(p + 1);
@@ -214,7 +259,8 @@ int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, 
     r[3] = (int32_t)ebeg(p + 1);
     r[4] = (int32_t)off;
     r[5] = (int32_t)n;
-    r[6] = r[7] = 0;
+    r[6] = use_tris ? (int32_t)tri_off[(size_t)p] : 0;
+    r[7] = use_tris ? (int32_t)(tri_off[(size_t)p + 1] - tri_off[(size_t)p]) : 0;
     off += n;
     most = std::max(most, (int32_t)n);
   }
@@ -222,6 +268,8 @@ int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, 
     for (int64_t t = 0; t < nt; ++t) tets16[t] = 0;
   if (!use_edges && edges16)
     for (int64_t e = 0; e < ne; ++e) edges16[e] = 0;
+  if (!use_tris && tris16)
+    for (int64_t f = 0; f < nf; ++f) tris16[f] = 0;
   *n_verts = off;
   *max_local = most;
   return 0;

@@ -386,16 +386,37 @@ __device__ __forceinline__ void project(const ffsrl_batch& b, const ffsrl_params
   }
 }
 
+// Surface triangles of an env: mesh-local (a, b, c) from the batch's int4
+// triangles, or part-local ids packed 3 x 16 bits (the partitioned frame).
+struct TrisGlobal {
+  const int32_t* tris;
+  int64_t base;
+  __device__ __forceinline__ int3 operator()(int f) const {
+    const int4 t = ldi4(tris, base + f);
+    return make_int3(t.x, t.y, t.z);
+  }
+};
+struct TrisPacked {
+  const uint64_t* t16;
+  int64_t base;
+  __device__ __forceinline__ int3 operator()(int f) const {
+    const uint64_t w = t16[base + f];
+    return make_int3((int)(w & 0xffffu), (int)((w >> 16) & 0xffffu), (int)((w >> 32) & 0xffffu));
+  }
+};
+
 // collideLaparoscopeDO (integrator.py:981-1050) for one env: every surface
 // triangle against capsules 0..2 of the env's instrument, responses read from
 // the positions at the start of the pass and summed (negated) into the LDS
 // accumulators for the vertices with inverse mass > 0; *contacts counts the
-// triangles with a response.
-// Triangles f0 .. f1-1 (the whole surface, or one chunk of a shared env).
-template <bool COH>
-__device__ __forceinline__ void collide_env(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
-                                            int env, const FrameCtx& c, const float* P, int64_t vb, int* contacts,
-                                            int block, int f0, int f1) {
+// triangles with a response.  ps(q) is the position (w: inverse mass) of
+// vertex q and ts(f) triangle f's vertex ids in ps's and the accumulators'
+// numbering; triangles f0 .. f1-1 (the whole surface, one chunk of a shared
+// env, or a part's triangles).
+template <class PS, class TS>
+__device__ __forceinline__ void collide_tris(const ffsrl_batch& b, const ffsrl_params& prm, int env,
+                                             const FrameCtx& c, const PS& ps, const TS& ts, int* contacts, int block,
+                                             int f0, int f1) {
   CapsuleBox box[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -405,8 +426,8 @@ __device__ __forceinline__ void collide_env(const ffsrl_batch& b, const ffsrl_pa
                          b.caps_radius[ci]);
   }
   for (int f = f0 + (int)threadIdx.x; f < f1; f += block) {
-    const int4 t = ldi4(b.tris, (int64_t)e.fbase + f);
-    const float4 A = ldp<COH>(P, vb + t.x), B = ldp<COH>(P, vb + t.y), Cc = ldp<COH>(P, vb + t.z);
+    const int3 t = ts(f);
+    const float4 A = ps(t.x), B = ps(t.y), Cc = ps(t.z);
     const f3 pA = xyz(A), pB = xyz(B), pC = xyz(Cc);
     const bool near0 = tri_meets_box(box[0], pA, pB, pC), near1 = tri_meets_box(box[1], pA, pB, pC),
                near2 = tri_meets_box(box[2], pA, pB, pC);
@@ -430,6 +451,15 @@ __device__ __forceinline__ void collide_env(const ffsrl_batch& b, const ffsrl_pa
       if (Cc.w > 0.0f) lds_acc<-1>(c, t.z, r[2] * c.inv_q, 0u);
     }
   }
+}
+
+// The collision pass of env `env` over triangles f0 .. f1-1, positions from
+// the model's p array.
+template <bool COH>
+__device__ __forceinline__ void collide_env(const ffsrl_batch& b, const ffsrl_params& prm, const ffsrl_env_desc& e,
+                                            int env, const FrameCtx& c, const float* P, int64_t vb, int* contacts,
+                                            int block, int f0, int f1) {
+  collide_tris(b, prm, env, c, PosGlobal<COH>{P, vb}, TrisGlobal{b.tris, (int64_t)e.fbase}, contacts, block, f0, f1);
 }
 
 // One frame of env `env` by the calling workgroup (COLLIDE: with the
@@ -1291,6 +1321,7 @@ struct PartsWs {
   const int32_t* verts;      // per part, its vertices (mesh-local id | flags)
   const uint64_t* t16;       // part-local packed tet ids
   const uint32_t* e16;       // part-local packed edge ids
+  const uint64_t* f16;       // by part, its surface triangles' local ids (collision)
   unsigned long long* sums;  // [3][stride][2]
   int64_t stride;
   uint32_t* ctr;             // [n_env][4]: parts finished, pass arrivals, give-up flag, 0; then XCD claim
@@ -1366,11 +1397,11 @@ constexpr int kSoloMaxVertices = (kLdsBytes - kPartStaticLds) / kSoloBytes;
 
 // An env held by one part: its vertex list is the identity and no vertex is
 // shared, so this is the one-workgroup frame (frame_env) with the positions
-// in LDS: the projection gathers from LDS, the apply reads and writes LDS,
-// and the substep-start positions, velocities and grasp weights stay in the
-// model's arrays as there.  Each pass has no exchange and no wait; the model's
-// p is written once, after the last pass.
-template <bool CARRY>
+// in LDS: the projection and the collision pass gather from LDS, the applies
+// read and write LDS, and the substep-start positions, velocities and grasp
+// weights stay in the model's arrays as there.  Each pass has no exchange and
+// no wait; the model's p is written once, after the last pass.
+template <bool CARRY, bool COLLIDE>
 __device__ void solo_env(const ffsrl_batch& b, const ffsrl_params& prm, const PartsWs& ws, int env, int refit,
                          unsigned long long* lds) {
   constexpr int BLOCK = 1024;
@@ -1396,9 +1427,22 @@ __device__ void solo_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
   const int ne = prm.dist_mode != FFSRL_OFF ? e.ne : 0;
   const int nmax = nt > ne ? nt : ne;
   const int I = prm.iterations;
+  __shared__ int s_contacts;  // surface triangles in contact during the frame
+  // ground clamp, integrate and the next substep's predict of vertex i
+  auto finish = [&](int i, f3 p, float w, bool next_sub) {
+    if (p.y < vs.ground) p.y = vs.ground;
+    const float4 x4 = ld4(X, vb + i);
+    const f3 v = (p - xyz(x4)) * vs.inv_dt;
+    st4(X, vb + i, with_w(p, x4.w));
+    if (next_sub)
+      p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
+    else
+      st4(Vv, vb + i, with_w(v, 0.0f));
+    Pl[i] = with_w(p, w);
+  };
 
   // predict of substep 0; each thread notes which of its vertices (i = tid +
-  // k * BLOCK, k < 5 since nv <= 5,104) are grasped
+  // k * BLOCK, k < 5 since nv <= 5,088) are grasped
   uint32_t grasped = 0;
   for (int i = tid, k = 0; i < nv; i += BLOCK, ++k) {
     const float4 x = ld4(X, vb + i), v = ld4(Vv, vb + i);
@@ -1407,7 +1451,10 @@ __device__ void solo_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
     lds[2 * i] = 0ull;
     lds[2 * i + 1] = 0ull;
   }
-  if (tid == 0) sat_flag() = 0;
+  if (tid == 0) {
+    sat_flag() = 0;
+    s_contacts = 0;
+  }
   __syncthreads();
   for (int s = 0; s < prm.substeps; ++s) {
     for (int it = 0; it < I; ++it) {
@@ -1430,7 +1477,7 @@ __device__ void solo_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
           p = p + mk((float)fx, (float)fy, (float)fz) * sc;
         }
         if ((grasped >> k) & 1u) p = grasp_pull<true>(p, tip, b.drag[vb + i], vs.ks);
-        if (last_it) {
+        if (last_it && !COLLIDE) {  // with collision, the integrate follows it
           if (p.y < vs.ground) p.y = vs.ground;
           const float4 x4 = ld4(X, vb + i);
           const f3 v = (p - xyz(x4)) * vs.inv_dt;
@@ -1443,10 +1490,33 @@ __device__ void solo_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
         Pl[i] = with_w(p, p4.w);
       }
       __syncthreads();
+      if (COLLIDE) {
+        // ---- instrument collision (collideLaparoscopeDO), summed in LDS ----
+        collide_tris(b, prm, env, c, PosLds{Pl}, TrisGlobal{b.tris, (int64_t)e.fbase}, &s_contacts, BLOCK, 0, e.nf);
+        __syncthreads();
+        for (int i = tid; i < nv; i += BLOCK) {
+          const unsigned long long w0 = lds[2 * i], w1 = lds[2 * i + 1];
+          lds[2 * i] = 0ull;
+          lds[2 * i + 1] = 0ull;
+          if (!last_it && (w0 | w1) == 0ull) continue;
+          int32_t fx, fy, fz, n;
+          unpack2(w0, &fx, &fy);
+          unpack2(w1, &fz, &n);
+          (void)sum_count(n);  // collision terms carry no count, only saturation marks
+          const float4 p4 = Pl[i];
+          const f3 p = xyz(p4) + mk((float)fx, (float)fy, (float)fz) * c.q;
+          if (last_it)
+            finish(i, p, p4.w, next_sub);
+          else
+            Pl[i] = with_w(p, p4.w);
+        }
+        __syncthreads();
+      }
     }
   }
   for (int i = tid; i < nv; i += BLOCK) st4(b.p, vb + i, Pl[i]);
   if (tid == 0) {
+    if (COLLIDE && s_contacts) atomicAdd(&b.in_collision[env], s_contacts);  // counts accumulate until cleared
     atomicAdd(&ws.env_ctr(env)[0], 1u);  // the env's one part is done
     if (sat_flag()) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_SATURATED);
   }
@@ -1454,11 +1524,18 @@ __device__ void solo_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
   if (refit) refit_env<BLOCK>(b, e);
 }
 
-template <bool CARRY>
+// A part of a multi-part env.  Each pass (projection; with COLLIDE also the
+// collision pass) sums into LDS, and the shared vertices' sums then cross one
+// exchange: exchange x adds them into plane x % 3, arrives and waits for the
+// env's np parts (np (x + 1) arrivals); every part applies the same totals to
+// its copy of a shared vertex, and the owner clears plane (x - 1) % 3, which
+// every part has read before arriving at exchange x.
+template <bool CARRY, bool COLLIDE>
 __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const PartsWs& ws, int env, int pi, int np,
                          int refit, unsigned long long* lds) {
   constexpr int BLOCK = 1024;
-  __shared__ int s_ok, s_last;
+  constexpr int E = COLLIDE ? 2 : 1;  // exchanges per pass
+  __shared__ int s_ok, s_last, s_contacts;
   const ffsrl_env_desc e = b.envs[env];
   const int* pr = ws.parts + 8 * (int64_t)pi;
   const int t0 = pr[0], t1 = pr[1], e0 = pr[2], e1 = pr[3], voff = pr[4], nloc = pr[5];
@@ -1483,30 +1560,12 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
   const int nt = (prm.vol_mode != FFSRL_OFF || prm.nh_on) ? t1 - t0 : 0;
   const int ne = prm.dist_mode != FFSRL_OFF ? e1 - e0 : 0;
   const int nmax = nt > ne ? nt : ne;
+  const int X = E * U;  // exchanges in the frame
   uint32_t* ctr = ws.env_ctr(env);
-
-  // the part's vertices: flags, state and the predict of substep 0
-  for (int i = tid; i < nloc; i += BLOCK) {
-    const uint32_t m = (uint32_t)ws.verts[voff + i];
-    const int64_t g = vb + (m & kPartIdMask);
-    const float4 x = ld4(b.x, g), v = ld4(b.v, g);
-    Ml[i] = m;
-    Xl[i] = x;
-    Dl[i] = b.drag[g];
-    Pl[i] = with_w(predict(xyz(x), xyz(v), x.w, vs.g, vs.dt, vs.damp1), x.w);
-    lds[2 * i] = 0ull;
-    lds[2 * i + 1] = 0ull;
-  }
-  if (tid == 0) sat_flag() = 0;
-  __syncthreads();
-  bool ok = true;
-  for (int u = 0; u < U; ++u) {
-    const int s = u / I, itn = u - s * I;
-    project<BLOCK, CARRY, false>(b, prm, e, c, nhk, PosLds{Pl}, Items{ws.t16, ws.e16, t0, e0}, T, nt, ne, 0, nmax,
-                                 reset_lambda && itn == 0);
-    __syncthreads();
-    // shared vertices: this part's sums into the pass's plane, then arrive and wait for the env's parts
-    unsigned long long* gs = ws.plane(u % 3, vb);
+  // this part's sums of its shared vertices into plane x % 3, then arrive and
+  // wait for the env's parts; false when the wait gave up
+  auto exchange = [&](int x) -> bool {
+    unsigned long long* gs = ws.plane(x % 3, vb);
     for (int i = tid; i < nloc; i += BLOCK) {
       const uint32_t m = Ml[i];
       if (m & kPartShared) {
@@ -1520,31 +1579,80 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
     __syncthreads();
     if (tid == 0) {
       atomicAdd(&ctr[1], 1u);
-      s_ok = share_wait(ctr, (uint32_t)(np * (u + 1)));
+      s_ok = share_wait(ctr, (uint32_t)(np * (x + 1)));
     }
     __syncthreads();
-    if (!s_ok) {
+    return s_ok != 0;
+  };
+  // the sums of local vertex i after an exchange into plane gs (LDS cleared
+  // for the next pass); the owner of a shared vertex clears its entries of
+  // the exchange's previous plane old (none before the first exchange)
+  auto take = [&](int i, uint32_t m, const unsigned long long* gs, unsigned long long* old, unsigned long long* w0,
+                  unsigned long long* w1) {
+    *w0 = lds[2 * i];
+    *w1 = lds[2 * i + 1];
+    lds[2 * i] = 0ull;
+    lds[2 * i + 1] = 0ull;
+    if (m & kPartShared) {
+      const int g = (int)(m & kPartIdMask);
+      *w0 = ldu64c(gs, 2 * g);
+      *w1 = ldu64c(gs, 2 * g + 1);
+      if (old && (m & kPartOwner)) {
+        old[2 * g] = 0ull;
+        old[2 * g + 1] = 0ull;
+      }
+    }
+  };
+  // ground clamp, integrate and the next substep's predict of local vertex i
+  auto finish = [&](int i, uint32_t m, f3 p, float w, bool next_sub) {
+    if (p.y < vs.ground) p.y = vs.ground;
+    const float4 x4 = Xl[i];
+    const f3 v = (p - xyz(x4)) * vs.inv_dt;
+    Xl[i] = with_w(p, x4.w);
+    if (next_sub)
+      p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
+    else if (m & kPartOwner)
+      st4(b.v, vb + (m & kPartIdMask), with_w(v, 0.0f));
+    Pl[i] = with_w(p, w);
+  };
+
+  // the part's vertices: flags, state and the predict of substep 0
+  for (int i = tid; i < nloc; i += BLOCK) {
+    const uint32_t m = (uint32_t)ws.verts[voff + i];
+    const int64_t g = vb + (m & kPartIdMask);
+    const float4 x = ld4(b.x, g), v = ld4(b.v, g);
+    Ml[i] = m;
+    Xl[i] = x;
+    Dl[i] = b.drag[g];
+    Pl[i] = with_w(predict(xyz(x), xyz(v), x.w, vs.g, vs.dt, vs.damp1), x.w);
+    lds[2 * i] = 0ull;
+    lds[2 * i + 1] = 0ull;
+  }
+  if (tid == 0) {
+    sat_flag() = 0;
+    s_contacts = 0;
+  }
+  __syncthreads();
+  bool ok = true;
+  for (int u = 0; u < U; ++u) {
+    const int s = u / I, itn = u - s * I;
+    project<BLOCK, CARRY, false>(b, prm, e, c, nhk, PosLds{Pl}, Items{ws.t16, ws.e16, t0, e0}, T, nt, ne, 0, nmax,
+                                 reset_lambda && itn == 0);
+    __syncthreads();
+    if (!exchange(E * u)) {
       ok = false;
       break;
     }
     // apply (+ grasp; at the end of a substep the ground clamp, integrate and next predict), as
-    // the one-workgroup frame; the owner clears its shared vertices' entries of plane (u - 1) % 3
+    // the one-workgroup frame
     const bool last_it = itn == I - 1, next_sub = s + 1 < prm.substeps;
-    unsigned long long* old = u >= 1 ? ws.plane((u + 2) % 3, vb) : nullptr;
+    const int x = E * u;
+    const unsigned long long* gs = ws.plane(x % 3, vb);
+    unsigned long long* old = x >= 1 ? ws.plane((x + 2) % 3, vb) : nullptr;
     for (int i = tid; i < nloc; i += BLOCK) {
       const uint32_t m = Ml[i];
-      unsigned long long w0 = lds[2 * i], w1 = lds[2 * i + 1];
-      lds[2 * i] = 0ull;
-      lds[2 * i + 1] = 0ull;
-      if (m & kPartShared) {
-        const int g = (int)(m & kPartIdMask);
-        w0 = ldu64c(gs, 2 * g);
-        w1 = ldu64c(gs, 2 * g + 1);
-        if (old && (m & kPartOwner)) {
-          old[2 * g] = 0ull;
-          old[2 * g + 1] = 0ull;
-        }
-      }
+      unsigned long long w0, w1;
+      take(i, m, gs, old, &w0, &w1);
       int32_t fx, fy, fz, n;
       unpack2(w0, &fx, &fy);
       unpack2(w1, &fz, &n);
@@ -1557,19 +1665,40 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
       }
       const float dr = Dl[i];
       if (dr != 0.0f) p = grasp_pull<true>(p, tip, dr, vs.ks);
-      if (last_it) {
-        if (p.y < vs.ground) p.y = vs.ground;
-        const float4 x4 = Xl[i];
-        const f3 v = (p - xyz(x4)) * vs.inv_dt;
-        Xl[i] = with_w(p, x4.w);
-        if (next_sub)
-          p = predict(p, v, x4.w, vs.g, vs.dt, vs.damp1);
-        else if (m & kPartOwner)
-          st4(b.v, vb + (m & kPartIdMask), with_w(v, 0.0f));
-      }
-      Pl[i] = with_w(p, p4.w);
+      if (last_it && !COLLIDE)  // with collision, the integrate follows it
+        finish(i, m, p, p4.w, next_sub);
+      else
+        Pl[i] = with_w(p, p4.w);
     }
     __syncthreads();
+    if (COLLIDE) {
+      // ---- instrument collision over the part's triangles, summed in LDS, then exchanged ----
+      collide_tris(b, prm, env, c, PosLds{Pl}, TrisPacked{ws.f16, (int64_t)pr[6]}, &s_contacts, BLOCK, 0, pr[7]);
+      __syncthreads();
+      if (!exchange(x + 1)) {
+        ok = false;
+        break;
+      }
+      const unsigned long long* gc = ws.plane((x + 1) % 3, vb);
+      unsigned long long* oc = ws.plane(x % 3, vb);
+      for (int i = tid; i < nloc; i += BLOCK) {
+        const uint32_t m = Ml[i];
+        unsigned long long w0, w1;
+        take(i, m, gc, oc, &w0, &w1);
+        if (!last_it && (w0 | w1) == 0ull) continue;
+        int32_t fx, fy, fz, n;
+        unpack2(w0, &fx, &fy);
+        unpack2(w1, &fz, &n);
+        (void)sum_count(n);  // collision terms carry no count, only saturation marks
+        const float4 p4 = Pl[i];
+        const f3 p = xyz(p4) + mk((float)fx, (float)fy, (float)fz) * c.q;
+        if (last_it)
+          finish(i, m, p, p4.w, next_sub);
+        else
+          Pl[i] = with_w(p, p4.w);
+      }
+      __syncthreads();
+    }
   }
   if (ok) {
     // the model's state of the vertices this part owns, then one more arrival
@@ -1585,14 +1714,14 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
     __syncthreads();
     if (tid == 0) {
       atomicAdd(&ctr[1], 1u);
-      s_ok = share_wait(ctr, (uint32_t)(np * (U + 1)));
+      s_ok = share_wait(ctr, (uint32_t)(np * (X + 1)));
     }
     __syncthreads();
     ok = s_ok;
   }
   if (ok) {
     // every part has read the last plane: the owners clear it
-    unsigned long long* last = ws.plane((U - 1) % 3, vb);
+    unsigned long long* last = ws.plane((X - 1) % 3, vb);
     for (int i = tid; i < nloc; i += BLOCK) {
       const uint32_t m = Ml[i];
       if ((m & kPartShared) && (m & kPartOwner)) {
@@ -1603,7 +1732,10 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
     }
     wait_vmem();
     __syncthreads();
-    if (tid == 0) s_last = atomicAdd(&ctr[0], 1u) + 1u == (uint32_t)np;
+    if (tid == 0) {
+      if (COLLIDE && s_contacts) atomicAdd(&b.in_collision[env], s_contacts);  // counts accumulate until cleared
+      s_last = atomicAdd(&ctr[0], 1u) + 1u == (uint32_t)np;
+    }
     __syncthreads();
     if (refit && s_last) refit_env<BLOCK, true>(b, e);  // every part's positions are stored
   } else if (tid == 0) {
@@ -1613,7 +1745,7 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
   if (tid == 0 && sat_flag()) atomicOr(&b.status[env], (int32_t)FFSRL_STATUS_SATURATED);
 }
 
-template <bool CARRY>
+template <bool CARRY, bool COLLIDE>
 __global__ __launch_bounds__(1024) void k_frame_parts(ffsrl_batch b, ffsrl_params prm, const int32_t* list, int n,
                                                       int refit, int nxcd, XcdMap map, PartsWs ws) {
   extern __shared__ __align__(16) unsigned long long lds[];
@@ -1632,9 +1764,9 @@ __global__ __launch_bounds__(1024) void k_frame_parts(ffsrl_batch b, ffsrl_param
       const int env = list ? list[el] : el;
       const int np = ws.env_parts[2 * env + 1];
       if (np == 1)
-        solo_env<CARRY>(b, prm, ws, env, refit, lds);
+        solo_env<CARRY, COLLIDE>(b, prm, ws, env, refit, lds);
       else
-        part_env<CARRY>(b, prm, ws, env, ws.env_parts[2 * env] + part, np, refit, lds);
+        part_env<CARRY, COLLIDE>(b, prm, ws, env, ws.env_parts[2 * env] + part, np, refit, lds);
       __syncthreads();  // LDS is reused by the next part
     }
   }
@@ -2224,7 +2356,9 @@ int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl
   int rc = check_batch(b, prm);
   if (rc) return rc;
   if (!pt) return fail(FFSRL_E_ARG, "parts NULL");
-  if (prm->collide) return fail(FFSRL_E_ARG, "the partitioned frame has no collision pass");
+  if (prm->collide && (!b->tris || !b->caps_base || !b->caps_radius || !b->in_collision || !pt->tris16))
+    return fail(FFSRL_E_ARG, "collision needs triangles, capsule bases / radii, the in_collision counters and the "
+                             "parts' triangles");
   const bool carry = carries_lambda(*prm);
   if (carry && ((prm->dist_mode == FFSRL_XPBD && !b->lam_e) || (prm->vol_mode == FFSRL_XPBD && !b->lam_t)))
     return fail(FFSRL_E_ARG, "XPBD with iterations > 1 needs lambda arrays");
@@ -2276,7 +2410,8 @@ int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl
                                         (long long)total);
   if (n >= (1 << 25)) return fail(FFSRL_E_LIMIT, "%d environments listed (claim entries hold 2^25)", n);
   const int lds = std::max(kPartLocalBytes * pt->max_local, kSoloBytes * pt->max_solo);
-  auto kern = carry ? k_frame_parts<true> : k_frame_parts<false>;
+  auto kern = prm->collide ? (carry ? k_frame_parts<true, true> : k_frame_parts<false, true>)
+                           : (carry ? k_frame_parts<true, false> : k_frame_parts<false, false>);
   hipFuncAttributes fa;
   FFSRL_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kern)));
   if ((int)fa.sharedSizeBytes > kPartStaticLds)
@@ -2285,8 +2420,17 @@ int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl
   FFSRL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   FFSRL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 1024, lds));
   const int grid = std::max(nxcd, cus * std::max(1, per_cu));
-  const PartsWs ws{pt->parts,  pt->env_parts, pt->verts, pt->tets16, pt->edges16,
-                   reinterpret_cast<unsigned long long*>(pt->sums), pt->stride, pt->ctr, pt->claims, b->n_env};
+  const PartsWs ws{pt->parts,
+                   pt->env_parts,
+                   pt->verts,
+                   pt->tets16,
+                   pt->edges16,
+                   pt->tris16,
+                   reinterpret_cast<unsigned long long*>(pt->sums),
+                   pt->stride,
+                   pt->ctr,
+                   pt->claims,
+                   b->n_env};
   hipStream_t st = as_stream(stream);
   hipLaunchKernelGGL(k_parts_claims, dim3(1), dim3(1024), 0, st, envs, n, nxcd, ws);
   FFSRL_HIP(hipGetLastError());

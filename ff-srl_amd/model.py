@@ -61,32 +61,42 @@ PART_MAX = 32      # most parts per mesh (one XCD's CUs run an env's parts at on
 
 
 def partition_mesh(tets: np.ndarray, edges: np.ndarray, nv: int, parts: int, use_tets: bool = True,
-                   use_edges: bool = True) -> dict:
+                   use_edges: bool = True, tris: Optional[np.ndarray] = None) -> dict:
     """Cuts a mesh's constraints (in their current order) into ``parts``
     ranges of tets and of edges for the partitioned frame
     (``ffsrl_partition_mesh``).  Returns the part rows (tet begin / end, edge
-    begin / end, first vertex entry, vertex count, 0, 0; vertex entries
-    relative to this mesh), each part's vertices (mesh-local id |
+    begin / end, first vertex entry, vertex count, first tris16 entry,
+    triangle count; entries relative to this mesh), each part's vertices (mesh-local id |
     PART_SHARED when several parts hold it | PART_OWNER on the lowest part
     holding it), the part-local ids of every tet and edge (packed as the
-    frame reads them: 4 x 16 bits, 2 x 16 bits), and the largest part.  Only the constraint families in use decide a part's
-    vertices; a vertex that no constraint in use touches goes to part
-    i mod parts (i its rank among such vertices)."""
+    frame reads them: 4 x 16 bits, 2 x 16 bits), and the largest part.
+    With ``tris`` (the surface triangles, int4 in BVH leaf order: the
+    collision pass), each triangle goes to the lowest part holding its first
+    vertex, its vertices join that part, and ``tris16`` holds every part's
+    triangles' local ids (3 x 16 bits), by part.  Only the constraint
+    families in use (and the triangles) decide a part's vertices; a vertex
+    that none of them touches goes to part i mod parts (i its rank among
+    such vertices)."""
     P = int(parts)
     tets = np.ascontiguousarray(np.asarray(tets).reshape(-1, 4), dtype=np.int32)
     edges = np.ascontiguousarray(np.asarray(edges).reshape(-1, 2), dtype=np.int32)
-    nt, ne = tets.shape[0], edges.shape[0]
+    use_tris = tris is not None
+    tris = np.ascontiguousarray(np.asarray(tris if use_tris else np.zeros((0, 4))).reshape(-1, 4), dtype=np.int32)
+    nt, ne, nf = tets.shape[0], edges.shape[0], tris.shape[0]
     rows = np.zeros((P, 8), np.int32)
-    cap = 4 * nt + 2 * ne + nv
+    cap = 4 * nt + 2 * ne + 3 * nf + nv
     verts = np.zeros(max(cap, 1), np.int32)
     t16 = np.zeros(max(nt, 1), np.uint64)
     e16 = np.zeros(max(ne, 1), np.uint32)
+    f16 = np.zeros(max(nf, 1), np.uint64)
     n_verts, most = C.c_int64(0), C.c_int32(0)
-    lib.ffsrl_partition_mesh(tets.ctypes.data, nt, edges.ctypes.data, ne, int(nv), P, int(bool(use_tets)),
-                             int(bool(use_edges)), rows.ctypes.data, verts.ctypes.data, cap, C.byref(n_verts),
-                             t16.ctypes.data, e16.ctypes.data, C.byref(most))
+    lib.ffsrl_partition_mesh(tets.ctypes.data, nt, edges.ctypes.data, ne, tris.ctypes.data, nf, int(nv), P,
+                             int(bool(use_tets)), int(bool(use_edges)), int(use_tris), rows.ctypes.data,
+                             verts.ctypes.data, cap, C.byref(n_verts), t16.ctypes.data, e16.ctypes.data,
+                             f16.ctypes.data, C.byref(most))
     return {"rows": rows.astype(np.int64), "verts": verts[:n_verts.value], "tets16": t16[:nt].view(np.int64),
-            "edges16": e16[:ne].view(np.int32), "max_local": int(most.value), "parts": P}
+            "edges16": e16[:ne].view(np.int32), "tris16": f16[:nf].view(np.int64), "max_local": int(most.value),
+            "parts": P}
 
 _MODES = {None: OFF, "off": OFF, "pbd": PBD, "xpbd": XPBD}
 
@@ -880,7 +890,8 @@ class SimModelDO:
             if parts is None and P > 1:
                 P = max(P, min(PART_MAX, -(-m.num_vertices * 5 // (4 * cap))))
             while True:
-                r = partition_mesh(t.tets, t.edges, m.num_vertices, P, use_t, use_e)
+                r = partition_mesh(t.tets, t.edges, m.num_vertices, P, use_t, use_e,
+                                   tris=t.tris_leaf if self.collision else None)
                 limit = solo_cap if P == 1 else cap
                 if r["max_local"] <= limit or parts is not None or P >= PART_MAX:
                     break
@@ -893,8 +904,10 @@ class SimModelDO:
         built = _parallel(build, range(len(self.topo)))
         first = np.cumsum([0] + [r["parts"] for r in built])
         voff = np.cumsum([0] + [r["verts"].size for r in built])
+        toff = np.cumsum([0] + [r["tris16"].size for r in built])
         rows = np.concatenate([r["rows"] for r in built])
         rows[:, 4] += np.repeat(voff[:-1], [r["parts"] for r in built])
+        rows[:, 6] += np.repeat(toff[:-1], [r["parts"] for r in built])
         env_parts = np.stack([first[self.env_mesh], first[self.env_mesh + 1] - first[self.env_mesh]], axis=1)
         self._parts_host = np.ascontiguousarray(env_parts.astype(np.int32))
         dev = self.device
@@ -904,6 +917,9 @@ class SimModelDO:
         self.p_verts = torch.from_numpy(cat([r["verts"] for r in built])).to(dev)
         self.p_tets16 = torch.from_numpy(cat([r["tets16"] for r in built])).to(dev)
         self.p_edges16 = torch.from_numpy(cat([r["edges16"] for r in built])).to(dev)
+        # with the collision pass, the parts' triangles (one entry at least, so the table is never NULL)
+        self.p_tris16 = (torch.from_numpy(cat([r["tris16"] for r in built] + [np.zeros(1, np.int64)])).to(dev)
+                         if self.collision else None)
         if getattr(self, "p_sums", None) is None:
             self.p_sums = torch.zeros((3, self.numVertices, 2), dtype=torch.int64, device=dev)
             self.p_ctr = torch.zeros(4 * self.numEnvs + 64, dtype=torch.int32, device=dev)
@@ -914,7 +930,7 @@ class SimModelDO:
         pt.max_local = max([r["max_local"] for r in built if r["parts"] > 1], default=0)
         pt.max_solo = max([r["max_local"] for r in built if r["parts"] == 1], default=0)
         pt.parts, pt.env_parts, pt.verts = _ptr(self.p_rows), _ptr(self.p_env), _ptr(self.p_verts)
-        pt.tets16, pt.edges16 = _ptr(self.p_tets16), _ptr(self.p_edges16)
+        pt.tets16, pt.edges16, pt.tris16 = _ptr(self.p_tets16), _ptr(self.p_edges16), _ptr(self.p_tris16)
         pt.sums, pt.stride, pt.ctr = _ptr(self.p_sums), self.numVertices, _ptr(self.p_ctr)
         n_claims = int(self._parts_host[:, 1].sum())
         if getattr(self, "p_claims", None) is None or self.p_claims.numel() < n_claims:

@@ -273,12 +273,14 @@ typedef struct ffsrl_parts {
   int32_t reserved;
   const int32_t* parts;           /* int [n_parts][8]: tet begin, tet end, edge begin, edge end (within
                                      the mesh's tets / edges, current order), first verts entry, local
-                                     vertex count, 0, 0 */
+                                     vertex count, first tris16 entry, triangle count (collision) */
   const int32_t* env_parts;       /* int2 [n_env]: first part and part count of each env (its mesh's) */
   const int32_t* verts;           /* per part, its local vertices in local-id order: mesh-local id,
                                      | FFSRL_PART_SHARED, | FFSRL_PART_OWNER on exactly one part */
   const uint64_t* tets16;         /* [T] each tet's ids local to its part, 4 x 16 bits (a low) */
   const uint32_t* edges16;        /* [E] each edge's ids local to its part, 2 x 16 bits */
+  const uint64_t* tris16;         /* by part, its surface triangles' local ids, 3 x 16 bits (collision
+                                     pass; NULL without it) */
   uint64_t* sums;                 /* [3][stride][2] fixed-point sums of shared vertices, by state
                                      vertex; zero before the first frame, left zero by every frame */
   int64_t stride;                 /* vertices per plane of sums (>= every env's vbase + nv) */
@@ -294,9 +296,9 @@ typedef struct ffsrl_parts {
  * workgroup (1024 threads) per part for the whole frame: positions and sums
  * of a part's vertices stay in LDS, and only vertices shared between parts
  * exchange sums through global memory, once per pass.  Results equal
- * ffsrl_frame_envs bit for bit (integer Jacobi sums); envs larger than the
- * batched frame's ffsrl_frame_max_vertices() run too, when their parts fit
- * (ffsrl_parts_max_local()).  No collision pass (FFSRL_E_ARG).  All parts of
+ * ffsrl_frame_envs bit for bit (integer Jacobi sums), the collision pass
+ * included; envs larger than the batched frame's ffsrl_frame_max_vertices()
+ * run too, when their parts fit (ffsrl_parts_max_local()).  All parts of
  * an env run on one XCD at once: at most CUs / XCDs (and 63) parts per env
  * (FFSRL_E_LIMIT).  Two launches: the claim table, then the frame.  The launch should be the only persistent frame on the
  * device: waits are bounded, and a frame that could not complete sets
@@ -534,20 +536,24 @@ int ffsrl_order_constraints(const int32_t* elems, int64_t n, int32_t arity, cons
 
 /* Partition of one mesh for ffsrl_frame_parts (host memory): part p takes
  * tets [p nt / parts, (p+1) nt / parts) and edges [p ne / parts, (p+1) ne /
- * parts) of the current order (tets / edges: 4 / 2 mesh-local ids each).
- * Its vertices (local ids 0 ..) are those its constraints in use touch, in
- * increasing mesh id, plus the untouched vertices dealt round-robin in
- * increasing id; verts gets each part's vertices (mesh-local id |
- * FFSRL_PART_SHARED when several parts hold it | FFSRL_PART_OWNER on the
- * lowest part holding it), rows the parts' [8] rows with vertex entries
- * relative to this mesh, tets16 / edges16 each constraint's local ids
- * (zero for a family not in use), *n_verts the entries written (verts_cap
- * >= 4 nt + 2 ne + nv always suffices) and *max_local the largest part.
- * No reference counterpart (set-up of this library's partitioned frame). */
-int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, int64_t ne, int32_t nv,
-                         int32_t parts, int32_t use_tets, int32_t use_edges, int32_t* rows, int32_t* verts,
-                         int64_t verts_cap, int64_t* n_verts, uint64_t* tets16, uint32_t* edges16,
-                         int32_t* max_local);
+ * parts) of the current order (tets / edges: 4 / 2 mesh-local ids each) and,
+ * with use_tris (the collision pass), the surface triangles (tris: int4
+ * a, b, c, face id in BVH leaf order, like ffsrl_batch.tris) whose first
+ * vertex it is the lowest part to hold.  Its vertices (local ids 0 ..) are
+ * those its constraints in use and its triangles touch, in increasing mesh
+ * id, plus the untouched vertices dealt round-robin in increasing id; verts
+ * gets each part's vertices (mesh-local id | FFSRL_PART_SHARED when several
+ * parts hold it | FFSRL_PART_OWNER on the lowest part holding it), rows the
+ * parts' [8] rows (vertex and triangle entries relative to this mesh),
+ * tets16 / edges16 each constraint's local ids and tris16 each part's
+ * triangles' local ids, by part (zero for a family not in use), *n_verts
+ * the entries written (verts_cap >= 4 nt + 2 ne + 3 nf + nv always
+ * suffices) and *max_local the largest part.  No reference counterpart
+ * (set-up of this library's partitioned frame). */
+int ffsrl_partition_mesh(const int32_t* tets, int64_t nt, const int32_t* edges, int64_t ne, const int32_t* tris,
+                         int64_t nf, int32_t nv, int32_t parts, int32_t use_tets, int32_t use_edges,
+                         int32_t use_tris, int32_t* rows, int32_t* verts, int64_t verts_cap, int64_t* n_verts,
+                         uint64_t* tets16, uint32_t* edges16, uint64_t* tris16, int32_t* max_local);
 
 /* ---- HIP graphs --------------------------------------------------------- */
 /* Capture everything issued on `stream` between begin and end into a graph

@@ -156,8 +156,17 @@ def test_argument_errors_are_reported(pkg):
     with pytest.raises(pkg.FFSRLError, match="in one part"):
         pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None)
     pt.max_solo = 0
+    # the collision pass needs the batch's collision arrays and the parts' triangles
     prm.collide = 1
-    with pytest.raises(pkg.FFSRLError, match="no collision pass"):
+    for f in ("tris", "caps_base", "caps_radius", "in_collision"):
+        setattr(b, f, dummy.ctypes.data)
+    with pytest.raises(pkg.FFSRLError, match="parts' triangles"):
+        pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None)
+    pt.tris16 = dummy.ctypes.data
+    assert pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None) == 0
+    for f in ("tris", "caps_base", "caps_radius", "in_collision"):
+        setattr(b, f, None)
+    with pytest.raises(pkg.FFSRLError, match="in_collision"):
         pkg.lib.ffsrl_frame_parts(b, prm, pt, None, 0, 0, None)
     prm.collide = 0
     with pytest.raises(pkg.FFSRLError, match="5 environments listed"):

@@ -247,3 +247,62 @@ def test_parts_graph_replays_equal_eager(pkg, gpu):
             _clean(pkg, b)
     finally:
         lib.ffsrl_graph_destroy(g)
+
+
+def _collision_models(pkg, gpu, meshes, kw, R=0.12):
+    """Two models of the same envs: capsule 0 pressing on each env's highest
+    surface vertex, capsule 1 beside it, capsule 2 far above."""
+    sc = pkg.scenes
+    bases, tips = [], []
+    for m in meshes:
+        q = sc.top_point(m).astype(np.float64)
+        c0 = q + np.array([0.0, 0.7 * R, 0.0])
+        bases.append(np.stack([c0 - [0, R, 0], q + [-0.15, 0.0, 0.0], q + [0, 3.0, 0], q + [0, 0.5, 0]]))
+        tips.append(np.stack([c0 + [0, 0.5 + R, 0], q + [-0.15, 0.6, 0.0], q + [0, 3.5, 0], q + [0, 0.5, 0]]))
+    f32 = lambda v: np.asarray(v, np.float32)
+    n = len(meshes)
+    return [pkg.SimModelDO(meshes, list(range(n)), device=gpu, tips=f32(tips), capsuleBases=f32(bases),
+                           capsuleRadii=f32([[R, R, 0.05, 0.0]] * n), collision=True, globalKsContact=1.0, **kw)
+            for _ in range(2)]
+
+
+@pytest.mark.parametrize("tets_per_part,kw,name", [(600, NH_DV, "nh-600"), (1500, XPBD_I2, "xpbd-i2-1500"),
+                                                   (1 << 30, NH_DV, "nh-whole")])
+def test_parts_collision_equal_batched(pkg, gpu, tets_per_part, kw, name):
+    """With the instrument collision pass (capsules pressing 4 mm per frame
+    into ellipsoids): state, carried lambdas and the per-env contact counts
+    equal the batched frame's bit for bit.  Each part tests its own surface
+    triangles; sums of vertices shared between parts cross a second exchange
+    per pass.  1 << 30 tets per part: every env in one part."""
+    sc = pkg.scenes
+    meshes = [sc.solid("ellipsoid", t, seed=21 + i) for i, t in enumerate((5000, 3000, 12000))]
+    a, b = _collision_models(pkg, gpu, meshes, kw)
+    b.enable_parts(tets_per_part=tets_per_part)
+    # surface triangles with a vertex held by several parts
+    v = b.p_verts.cpu().numpy().view(np.uint32)
+    rows = b.p_rows.cpu().numpy()
+    t16 = b.p_tris16.cpu().numpy()
+    crossing = 0
+    for r in rows:
+        w = t16[r[6]:r[6] + r[7]].view(np.int64)         # 3 x 16 bits: the high bits are zero
+        ids = np.stack([(w >> (16 * k)) & 0xffff for k in range(3)])
+        assert (ids < r[5]).all()                          # local ids inside the part's vertex list
+        crossing += int(((v[r[4] + ids] & 0x40000000) != 0).any(axis=0).sum())
+    if tets_per_part < (1 << 30):
+        assert crossing > 0
+    else:
+        assert max(b.parts_per_mesh) == 1
+    ref, pt = pkg.SimIntegratorDO(gpu), pkg.SimIntegratorDO(gpu, parts=True)
+    extra = ("laparoscopeInCollision",) + (("lam_t", "lam_e") if a.lam_t is not None else ())
+    d = torch.tensor([[0.0, -0.004, 0.0]] * len(meshes), dtype=torch.float32, device=gpu)
+    for f in range(4):
+        for m in (a, b):
+            pkg.lib.ffsrl_move_tools(m.batch, d.data_ptr(), None, None, m.stream())
+        ref.stepModel(a, refit=True)
+        pt.stepModel(b, refit=True)
+        _equal(a, b, extra)
+        _clean(pkg, b)
+    contacts = b.laparoscopeInCollision.cpu().numpy()
+    print(f"\n[parts collision {name}] parts per mesh {b.parts_per_mesh}, {crossing} triangles with a shared "
+          f"vertex, contacts {contacts.tolist()}")
+    assert (contacts > 0).all()

@@ -17,7 +17,8 @@ def _unpack(w, k):
     return np.stack([(w >> np.uint64(16 * j)) & np.uint64(0xffff) for j in range(k)], axis=1).astype(np.int64)
 
 
-def _brute(tets, edges, nv, P, use_t, use_e):
+def _brute(tets, edges, nv, P, use_t, use_e, tris=None):
+    """Each part's vertex list, and (with tris) each part's triangles."""
     nt, ne = len(tets), len(edges)
     sets = [set() for _ in range(P)]
     for p in range(P):
@@ -31,12 +32,23 @@ def _brute(tets, edges, nv, P, use_t, use_e):
     loose = [v for v in range(nv) if v not in touched]
     for i, v in enumerate(loose):
         sets[i % P].add(v)
-    return [sorted(s) for s in sets]
+    owned = [[] for _ in range(P)]
+    if tris is not None:
+        lowest = {v: min(p for p in range(P) if v in sets[p]) for v in range(nv)}
+        for f, t in enumerate(tris):
+            owned[lowest[int(t[0])]].append(f)
+        for p in range(P):
+            for f in owned[p]:
+                sets[p].update(int(v) for v in tris[f][:3])
+    return [sorted(s) for s in sets], owned
 
 
-@pytest.mark.parametrize("P,use_t,use_e", [(1, True, True), (3, True, True), (4, True, False), (5, False, True),
-                                           (7, True, True)])
-def test_partition_matches_brute_force(pkg, P, use_t, use_e):
+@pytest.mark.parametrize("P,use_t,use_e,use_f", [(1, True, True, False), (3, True, True, False),
+                                                 (4, True, False, False), (5, False, True, False),
+                                                 (7, True, True, False), (4, True, True, True), (6, True, False, True)])
+def test_partition_matches_brute_force(pkg, P, use_t, use_e, use_f):
+    """use_f: the surface triangles of the collision pass, each in the lowest
+    part holding its first vertex, their vertices joining that part."""
     from importlib import import_module
     model = import_module("ff-srl_amd.model")
     m = pkg.scenes.solid("ellipsoid", 900, seed=3)
@@ -44,8 +56,11 @@ def test_partition_matches_brute_force(pkg, P, use_t, use_e):
     tets = m.tets[m.tets.max(axis=1) < m.num_vertices - 1]
     edges = m.edges[m.edges.max(axis=1) < m.num_vertices - 1]
     nv = m.num_vertices
-    r = model.partition_mesh(tets, edges, nv, P, use_t, use_e)
-    want = _brute(tets, edges, nv, P, use_t, use_e)
+    tris = None
+    if use_f:
+        tris = np.concatenate([m.tris, np.arange(m.num_tris)[:, None]], axis=1).astype(np.int32)[::-1].copy()
+    r = model.partition_mesh(tets, edges, nv, P, use_t, use_e, tris=tris)
+    want, owned = _brute(tets, edges, nv, P, use_t, use_e, tris)
     rows, verts = r["rows"], r["verts"].view(np.uint32)
     assert r["parts"] == P and rows.shape == (P, 8)
     holders = {}
@@ -63,6 +78,10 @@ def test_partition_matches_brute_force(pkg, P, use_t, use_e):
             assert np.array_equal(ids[_unpack(r["tets16"], 4)[t0:t1]], tets[t0:t1])
         if use_e:
             assert np.array_equal(ids[_unpack(r["edges16"], 2)[e0:e1]], edges[e0:e1])
+        if use_f:
+            f0, nfp = int(rows[p, 6]), int(rows[p, 7])
+            assert nfp == len(owned[p])
+            assert np.array_equal(ids[_unpack(r["tris16"], 4)[f0:f0 + nfp, :3]], tris[owned[p], :3])
         for v, f in zip(ids, lv):
             hs = [q for q in range(P) if int(v) in want_sets[q]]
             assert bool(f & SHARED) == (len(hs) > 1)
