@@ -295,16 +295,18 @@ typedef struct ffsrl_parts {
 /* ffsrl_frame_envs with each listed environment cut into parts, one
  * workgroup (1024 threads) per part for the whole frame: positions and sums
  * of a part's vertices stay in LDS, and only vertices shared between parts
- * exchange sums through global memory, once per pass.  Results equal
- * ffsrl_frame_envs bit for bit (integer Jacobi sums), the collision pass
- * included; envs larger than the batched frame's ffsrl_frame_max_vertices()
- * run too, when their parts fit (ffsrl_parts_max_local()).  All parts of
- * an env run on one XCD at once: at most CUs / XCDs (and 63) parts per env
- * (FFSRL_E_LIMIT).  Two launches: the claim table, then the frame.  The launch should be the only persistent frame on the
- * device: waits are bounded, and a frame that could not complete sets
- * FFSRL_STATUS_INCOMPLETE (its sums are cleared and its counters reset).
- * envs NULL: environments 0 .. n-1.  Same reference counterpart as
- * ffsrl_frame. */
+ * exchange sums through global memory, once per pass (the projection and,
+ * with prm->collide, the collision pass over each part's surface triangles,
+ * parts->tris16).  Results equal ffsrl_frame_envs bit for bit (integer
+ * Jacobi sums), contact counts included; envs larger than the batched
+ * frame's ffsrl_frame_max_vertices() run too, when their parts fit
+ * (ffsrl_parts_max_local()).  All parts of an env run on one XCD at once:
+ * at most CUs / XCDs (and 63) parts per env (FFSRL_E_LIMIT).  Two
+ * launches: the claim table, then the frame.  The launch should be the
+ * only persistent frame on the device: waits are bounded, and a frame that
+ * could not complete sets FFSRL_STATUS_INCOMPLETE (its sums are cleared and
+ * its counters reset).  envs NULL: environments 0 .. n-1.  Same reference
+ * counterpart as ffsrl_frame. */
 int ffsrl_frame_parts(const ffsrl_batch* b, const ffsrl_params* prm, const ffsrl_parts* parts,
                       const int32_t* envs, int32_t n, int32_t refit, void* stream);
 
