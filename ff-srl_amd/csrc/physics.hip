@@ -1215,7 +1215,12 @@ struct XcdMap {
 template <int BLOCK, bool CARRY, bool COLLIDE, bool FOLD>
 __global__ __launch_bounds__(BLOCK) void k_frame_shared(ffsrl_batch b, ffsrl_params prm, const int32_t* list, int n,
                                                         int refit, int nxcd, XcdMap map, ShareWs ws) {
-  extern __shared__ __align__(16) unsigned long long lds[];
+  // float4-typed and named for this kernel, so the dynamic LDS starts 16-byte
+  // aligned after the static LDS whatever that size is (block-scope extern
+  // __shared__ arrays of one name are one entity, whose alignment is its
+  // type's; misaligned 16-byte LDS accesses run far slower)
+  extern __shared__ float4 lds_shared[];
+  unsigned long long* lds = reinterpret_cast<unsigned long long*>(lds_shared);
   __shared__ int s_lastwg, s_incomplete;
   const int x = map.index[xcc_id() & 15];
   const int mine = (x >= 0 && x < nxcd && x < n) ? (n - 1 - x) / nxcd + 1 : 0;  // slots x, x + nxcd, ...
@@ -1748,7 +1753,10 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
 template <bool CARRY, bool COLLIDE>
 __global__ __launch_bounds__(1024) void k_frame_parts(ffsrl_batch b, ffsrl_params prm, const int32_t* list, int n,
                                                       int refit, int nxcd, XcdMap map, PartsWs ws) {
-  extern __shared__ __align__(16) unsigned long long lds[];
+  // float4-typed and named for this kernel: the dynamic LDS starts 16-byte
+  // aligned (the part's positions are float4s)
+  extern __shared__ float4 lds_parts[];
+  unsigned long long* lds = reinterpret_cast<unsigned long long*>(lds_parts);
   __shared__ int s_claim, s_lastwg, s_incomplete;
   const int x = map.index[xcc_id() & 15];
   if (x >= 0 && x < nxcd) {
