@@ -79,6 +79,16 @@ struct FrameCtx {
   float q;                  // 2^dp_exp
 };
 
+// A wavefront-uniform float held in a scalar register (its value unchanged),
+// so that constants of the projection do not take vector registers.
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ FrameCtx uniform(FrameCtx c) { return FrameCtx{c.acc, uniform(c.inv_q), uniform(c.q)}; }
+__device__ __forceinline__ NHConst uniform(NHConst k) {
+  return NHConst{uniform(k.gamma), uniform(k.alpha_h), uniform(k.alpha_d)};
+}
+
 // A term that left the range or was not finite adds kSatMark to its count
 // (counts stay below 2^16: at most FFSRL_MAX_TERMS terms of weight <= 2), so
 // the apply phase sees it in the sum's count and sets the workgroup's
@@ -1412,7 +1422,7 @@ __device__ void solo_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
   constexpr int BLOCK = 1024;
   const ffsrl_env_desc e = b.envs[env];
   const int nv = e.nv, tid = threadIdx.x;
-  FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
+  const FrameCtx c = uniform(FrameCtx{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)});
   float4* Pl = reinterpret_cast<float4*>(lds + 2 * nv);
   const int64_t vb = e.vbase;
   float* X = b.x;
@@ -1424,7 +1434,7 @@ __device__ void solo_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
   vs.inv_dt = 1.0f / prm.dt;
   vs.ground = prm.ground;
   vs.ks = prm.drag_ks;
-  const NHConst nhk = nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt);
+  const NHConst nhk = uniform(nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt));
   const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
   const int64_t T = b.n_tet_total;
   const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
@@ -1546,7 +1556,7 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
   const int t0 = pr[0], t1 = pr[1], e0 = pr[2], e1 = pr[3], voff = pr[4], nloc = pr[5];
   const int tid = threadIdx.x, I = prm.iterations, U = prm.substeps * I;
   const int64_t vb = e.vbase;
-  FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
+  const FrameCtx c = uniform(FrameCtx{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)});
   float4* Pl = reinterpret_cast<float4*>(lds + 2 * nloc);
   float4* Xl = Pl + nloc;
   uint32_t* Ml = reinterpret_cast<uint32_t*>(Xl + nloc);
@@ -1558,7 +1568,7 @@ __device__ void part_env(const ffsrl_batch& b, const ffsrl_params& prm, const Pa
   vs.inv_dt = 1.0f / prm.dt;
   vs.ground = prm.ground;
   vs.ks = prm.drag_ks;
-  const NHConst nhk = nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt);
+  const NHConst nhk = uniform(nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt));
   const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
   const int64_t T = b.n_tet_total;
   const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
