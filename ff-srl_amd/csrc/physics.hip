@@ -479,7 +479,7 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
                                           unsigned long long* lds) {
   const ffsrl_env_desc e = b.envs[env];
   const int nv = e.nv;
-  FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
+  const FrameCtx c = uniform(FrameCtx{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)});
   const int tid = threadIdx.x;
   if (tid == 0) sat_flag() = 0;  // read after the frame's barriers
   const int64_t vb = e.vbase;
@@ -494,7 +494,7 @@ __device__ __forceinline__ void frame_env(const ffsrl_batch& b, const ffsrl_para
   vs.inv_dt = 1.0f / prm.dt;
   vs.ground = prm.ground;
   vs.ks = prm.drag_ks;
-  const NHConst nhk = nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt);
+  const NHConst nhk = uniform(nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt));
   const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
   const int64_t T = b.n_tet_total;
 
@@ -808,7 +808,7 @@ __device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, const S
   unsigned long long* gacc = ws.acc_of(el, e);
   const int nv = e.nv, tid = threadIdx.x;
   const int64_t vb = e.vbase;
-  FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
+  const FrameCtx c = uniform(FrameCtx{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)});
   float* X = b.x;
   float* P = b.p;
   float* Vv = b.v;
@@ -819,7 +819,7 @@ __device__ void share_env(const ffsrl_batch& b, const ffsrl_params& prm, const S
   vs.inv_dt = 1.0f / prm.dt;
   vs.ground = prm.ground;
   vs.ks = prm.drag_ks;
-  const NHConst nhk = nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt);
+  const NHConst nhk = uniform(nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt));
   const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
   const int64_t T = b.n_tet_total;
   const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
@@ -1014,7 +1014,7 @@ __device__ void share_fold_env(const ffsrl_batch& b, const ffsrl_params& prm, co
   uint32_t* ctr = ws.ctr_of(el, env);
   const int nv = e.nv, tid = threadIdx.x;
   const int64_t vb = e.vbase;
-  FrameCtx c{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)};
+  const FrameCtx c = uniform(FrameCtx{lds, ldexpf(1.0f, -e.dp_exp), ldexpf(1.0f, e.dp_exp)});
   float4* Pl = reinterpret_cast<float4*>(lds + 2 * nv);  // positions of the unit this workgroup projects
   VertexStep vs;
   vs.g = mk(prm.gravity[0], prm.gravity[1], prm.gravity[2]);
@@ -1023,7 +1023,7 @@ __device__ void share_fold_env(const ffsrl_batch& b, const ffsrl_params& prm, co
   vs.inv_dt = 1.0f / prm.dt;
   vs.ground = prm.ground;
   vs.ks = prm.drag_ks;
-  const NHConst nhk = nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt);
+  const NHConst nhk = uniform(nh_const(prm.nh_vol_c, prm.nh_dev_c, prm.dt));
   const f3 tip = mk(b.tips[(env * 4 + 2) * 3 + 0], b.tips[(env * 4 + 2) * 3 + 1], b.tips[(env * 4 + 2) * 3 + 2]);
   const int64_t T = b.n_tet_total;
   const bool reset_lambda = !prm.lambda_legacy || prm.iterations > 1;
