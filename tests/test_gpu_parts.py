@@ -306,3 +306,38 @@ def test_parts_collision_equal_batched(pkg, gpu, tets_per_part, kw, name):
     print(f"\n[parts collision {name}] parts per mesh {b.parts_per_mesh}, {crossing} triangles with a shared "
           f"vertex, contacts {contacts.tolist()}")
     assert (contacts > 0).all()
+
+
+def test_autotune_with_collision(pkg, gpu, tmp_path):
+    """The autotuner times the partitioned frame with the collision pass too
+    (its trials are finite), restores the state and contact counts it timed
+    on, and its choice gives the batched frame's results bit for bit."""
+    sc = pkg.scenes
+    meshes = [sc.solid("ellipsoid", t, seed=61 + i) for i, t in enumerate((5000, 3000, 12000))]
+    a, b = _collision_models(pkg, gpu, meshes, NH_DV)
+    ref, integ = pkg.SimIntegratorDO(gpu), pkg.SimIntegratorDO(gpu)
+    d = torch.tensor([[0.0, -0.004, 0.0]] * len(meshes), dtype=torch.float32, device=gpu)
+    for _ in range(2):
+        for m in (a, b):
+            pkg.lib.ffsrl_move_tools(m.batch, d.data_ptr(), None, None, m.stream())
+        ref.stepModel(a, refit=True)
+        integ.stepModel(b, refit=True)
+    before = [t.clone() for t in (b.x, b.v, b.laparoscopeInCollision)]
+    cache = str(tmp_path / "tune.json")
+    cfg = integ.autotune(b, cache_path=cache)
+    trials = pkg.autotune.Autotuner(cache).cache[pkg.autotune.key_of(b)]["trials"]
+    parted = [t for t in trials if t["parts"]]
+    print(f"\n[autotune collision] {cfg}; {len(parted)} partitioned trials, best "
+          f"{min(t['ms'] for t in parted):.3f} ms against {min(t['ms'] for t in trials if not t['parts']):.3f} ms")
+    assert parted and all(np.isfinite(t["ms"]) for t in parted)
+    for x, y in zip(before, (b.x, b.v, b.laparoscopeInCollision)):
+        assert torch.equal(x, y)
+    for _ in range(3):
+        for m in (a, b):
+            pkg.lib.ffsrl_move_tools(m.batch, d.data_ptr(), None, None, m.stream())
+        ref.stepModel(a, refit=True)
+        integ.stepModel(b, refit=True)
+    _equal(a, b, ("laparoscopeInCollision",))
+    if integ.parts:
+        _clean(pkg, b)
+    assert int(b.laparoscopeInCollision.sum()) > 0
