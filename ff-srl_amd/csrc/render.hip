@@ -6,70 +6,22 @@
 // one pixel, its quad of 2x-supersampled rays and a 32-entry traversal stack
 // in LDS (slot-major, so the 64 lanes of a slot fall in distinct banks).  The
 // 2x2 box filter sums the quad in the lane in the reference's summation
-// order, so no ray buffer goes to HBM.  Traversal order, slab test, the
-// Moller-Trumbore test and the strict "0 < t < tMin" acceptance follow the
-// reference expression by expression for every ray, so hit triangles are
-// bit-exact.
+// order, so no ray buffer goes to HBM.  Traversal order, the Moller-Trumbore
+// test and the strict "0 < t < tMin" acceptance follow the reference
+// expression by expression for every ray, and the box tests' decisions equal
+// the reference's slab test (a fast form with an exact fallback,
+// render_box.hpp), so hit triangles are bit-exact.
 #include <algorithm>
 #include <climits>
 
 #include "common.hpp"
+#include "render_box.hpp"
 
 namespace ffsrl {
 
 constexpr int kStack = 32;  // deepest supported BVH (levels); the reference's stack size
 constexpr int kWaves = 1;   // wavefronts per block (one 8x8 ray tile each)
 constexpr int kXcds = 8;    // gfx950 dispatches workgroups round-robin over 8 XCDs
-
-__device__ __forceinline__ int isect_aabb(f3 o, f3 d, f3 lo, f3 hi, float* tmin_out) {
-  const float t1 = (lo.x - o.x) / d.x, t2 = (hi.x - o.x) / d.x;
-  const float t3 = (lo.y - o.y) / d.y, t4 = (hi.y - o.y) / d.y;
-  const float t5 = (lo.z - o.z) / d.z, t6 = (hi.z - o.z) / d.z;
-  const float tmin = fmax_(fmax_(fmin_(t1, t2), fmin_(t3, t4)), fmin_(t5, t6));
-  const float tmax = fmin_(fmin_(fmax_(t1, t2), fmax_(t3, t4)), fmax_(t5, t6));
-  *tmin_out = tmin;
-  return (tmax >= 0.0f) && (tmax >= tmin);
-}
-
-// The traversal only needs the decision "hit && tmin < tMin" of intersectAABB.
-// It is taken from products with per-ray reciprocals, (lo - o) * (1/d), which
-// are within 3 ulp of the quotients (lo - o) / d; special values (d = 0) give
-// the same infinities and NaNs either way.  When a comparison of the decision
-// is closer than 1e-6 relative, or a slab value is not finite, the decision is
-// recomputed with the exact quotients.  Either way the decision equals the
-// one of the division form, so the visited nodes and hits are unchanged.
-__device__ __forceinline__ bool close_rel(float a, float b) { return fabsf(a - b) <= 1e-6f * (fabsf(a) + fabsf(b)); }
-
-//
-// The fast path takes min/max with IEEE 754-2019 minimum/maximum
-// (v_minimum3 / v_maximum3), which propagate NaN: a NaN among the six
-// products makes tmin and tmax NaN, and |NaN| > 1e-30 is false, so such a
-// node goes to the exact path.  Otherwise min/max select exactly what the
-// reference's compare-select wp.min/wp.max select, up to the sign of a zero,
-// which no comparison below can see.  A product that overflows where the
-// quotient does not could change a selection only among values beyond
-// 1e37: such tmin / tmax go to the exact path, and so do rays with a
-// direction component in (0, 1e-30) (ray_ok; with |1/d| <= 1e30 only
-// coordinate differences above 1e8 can overflow).  Other infinities come
-// from a direction component that is exactly zero, where the quotient is the
-// same infinity.
-__device__ __forceinline__ float vmin_(float a, float b) { return __builtin_elementwise_minimum(a, b); }
-__device__ __forceinline__ float vmax_(float a, float b) { return __builtin_elementwise_maximum(a, b); }
-
-__device__ __forceinline__ bool visit_node(f3 o, f3 d, f3 inv, f3 lo, f3 hi, float tMin, bool ray_ok) {
-  const float t1 = (lo.x - o.x) * inv.x, t2 = (hi.x - o.x) * inv.x;
-  const float t3 = (lo.y - o.y) * inv.y, t4 = (hi.y - o.y) * inv.y;
-  const float t5 = (lo.z - o.z) * inv.z, t6 = (hi.z - o.z) * inv.z;
-  const float tmin = vmax_(vmax_(vmin_(t1, t2), vmin_(t3, t4)), vmin_(t5, t6));
-  const float tmax = vmin_(vmin_(vmax_(t1, t2), vmax_(t3, t4)), vmax_(t5, t6));
-  // every condition is evaluated (bitwise &, no short circuit), so the whole
-  // fast path is one basic block with a single branch
-  const bool robust = ray_ok & (fabsf(tmax) > 1e-30f) & (fabsf(tmax) < 1e37f) & (fabsf(tmin) < 1e37f) &
-                      !close_rel(tmax, tmin) & !close_rel(tmin, tMin);
-  if (robust) return (tmax >= 0.0f) & (tmax >= tmin) & (tmin < tMin);
-  float tb;
-  return isect_aabb(o, d, lo, hi, &tb) && tb < tMin;
-}
 
 __device__ __forceinline__ float mt_isect(f3 a, f3 b, f3 c, f3 o, f3 d) {
   const f3 e1 = b - a, e2 = c - a;
@@ -232,7 +184,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_render_quad(ffsrl_batch b, ffsr
   const ffsrl_env_desc e = b.envs[env];
   const int64_t vb = e.vbase, sb = e.boundbase, fb = e.fbase;
   const f3 o = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
-  auto dir_ok = [](float c) { return (c == 0.0f) | (fabsf(c) >= 1e-30f); };
   f3 d[4], inv[4];
   bool ray_ok[4];
   float tMin[4];
@@ -242,7 +193,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_render_quad(ffsrl_batch b, ffsr
     const int r = (2 * py + (q >> 1)) * RW + 2 * px + (q & 1);
     d[q] = xyz(ld4(cam.dirs, r));
     inv[q] = mk(1.0f / d[q].x, 1.0f / d[q].y, 1.0f / d[q].z);
-    ray_ok[q] = dir_ok(d[q].x) & dir_ok(d[q].y) & dir_ok(d[q].z);
+    ray_ok[q] = ray_dir_ok(d[q]);
     tMin[q] = 1e6f;
     best[q] = -1;
   }
@@ -352,8 +303,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_render(ffsrl_batch b, ffsrl_cam
   const f3 o = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
   const f3 d = xyz(ld4(cam.dirs, r));
   const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  auto dir_ok = [](float c) { return (c == 0.0f) | (fabsf(c) >= 1e-30f); };
-  const bool ray_ok = dir_ok(d.x) & dir_ok(d.y) & dir_ok(d.z);
+  const bool ray_ok = ray_dir_ok(d);
 
   // env-uniform bases; node / triangle / vertex ids are non-negative int32,
   // so loads take a 32-bit unsigned offset from a scalar base
