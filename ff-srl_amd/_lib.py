@@ -15,7 +15,7 @@ LIB_PATH = os.environ.get("FFSRL_LIB") or os.path.join(HERE, "_build", "libffsrl
 
 OFF, PBD, XPBD = 0, 1, 2
 E_ARG, E_LIMIT, E_STATE = -1, -2, -3
-ABI_VERSION = 13
+ABI_VERSION = 14
 STATUS_SATURATED, STATUS_INCOMPLETE = 1, 2   # FFSRL_STATUS_* bits of ffsrl_batch.status
 MAX_TERMS = 128                              # FFSRL_MAX_TERMS
 PART_OWNER, PART_SHARED = 0x80000000, 0x40000000  # FFSRL_PART_* flags of ffsrl_parts.verts

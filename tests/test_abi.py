@@ -33,7 +33,7 @@ def test_every_declared_symbol_is_exported(pkg):
 
 
 def test_abi_version(pkg):
-    assert pkg.lib.ffsrl_abi_version() == 13
+    assert pkg.lib.ffsrl_abi_version() == 14
     assert pkg.lib.ffsrl_frame_max_vertices() >= 10000
 
 
