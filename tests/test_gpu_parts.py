@@ -214,15 +214,22 @@ def test_parts_abandoned_frame_is_reported_and_cleaned(pkg, gpu):
     _clean(pkg, b)
 
 
-def test_parts_graph_replays_equal_eager(pkg, gpu):
+@pytest.mark.parametrize("collision", [False, True])
+def test_parts_graph_replays_equal_eager(pkg, gpu, collision):
     """The partitioned frame (claim table and frame) captured in a HIP graph
     and replayed four times, with eager work and synchronisations between
     the replays, gives after every replay the state of the same number of
-    eager batched frames; the frame resets its own counters (no memset)."""
+    eager batched frames; the frame resets its own counters and planes (no
+    memset), with the collision pass's second exchange per pass too."""
     import ctypes as C
     lib = pkg.lib
-    meshes = _meshes(pkg, 8, 1000, 12000, seed=41)
-    a, b = _pair(pkg, gpu, meshes, NH_DV)
+    if collision:
+        sc = pkg.scenes
+        meshes = [sc.solid("ellipsoid", t, seed=71 + i) for i, t in enumerate((5000, 3000, 12000, 1500))]
+        a, b = _collision_models(pkg, gpu, meshes, NH_DV)
+    else:
+        meshes = _meshes(pkg, 8, 1000, 12000, seed=41)
+        a, b = _pair(pkg, gpu, meshes, NH_DV)
     b.enable_parts(tets_per_part=3000)          # partition, allocation and the XCD probe outside the capture
     assert max(b.parts_per_mesh) > 1 and min(b.parts_per_mesh) == 1
     integ = pkg.SimIntegratorDO(gpu)
@@ -243,10 +250,12 @@ def test_parts_graph_replays_equal_eager(pkg, gpu):
                 lib.ffsrl_graph_launch(g, cs.cuda_stream)
             torch.cuda.synchronize()
             _ = float(b.x[:, :3].sum())                # eager work between the replays
-            _equal(a, b)
+            _equal(a, b, ("laparoscopeInCollision",) if collision else ())
             _clean(pkg, b)
     finally:
         lib.ffsrl_graph_destroy(g)
+    if collision:
+        assert int(b.laparoscopeInCollision.sum()) > 0
 
 
 def _collision_models(pkg, gpu, meshes, kw, R=0.12):
