@@ -168,6 +168,10 @@ class SceneServer:
         self.ready: collections.deque = collections.deque()     # actors with pending requests, FIFO
         self.in_ready = set()
         self.completions: List[Completion] = []
+        # on_complete(completion, server): server.depth[slot] and server.obs[slot]
+        # hold the request's observation until that actor's next tick; work the
+        # callback enqueues on torch's current stream (a copy of them) is
+        # ordered before every later tick
         self.on_complete = None
         self.stats = collections.Counter()
         self._next_stream = 0
@@ -295,6 +299,10 @@ class SceneServer:
         stream = self.streams[self._next_stream % len(self.streams)]
         self._next_stream += 1
         model = self.model
+        # completion callbacks copy a slot's outputs on the current stream: the
+        # tick that overwrites them runs after those copies
+        if self.on_complete is not None:
+            stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(stream):
             tick.dev.copy_(tick.host, non_blocking=True)
             dv = tick.dev.view(torch.int32)
